@@ -1,0 +1,166 @@
+"""Benchmark of the fused D2Q9 two-component Shan-Chen step on one MI355X.
+
+Workload (one "step" = one fused launch advancing every site of the lattice
+by one reference time step): the 4096 x 4096 fully periodic two-component
+lattice (spinodal initial condition, rho_A = rho_B = 1 +- 1% seeded noise,
+tau = 1, G = 1.2), synthetic and resident in HBM before the timed region.
+The headline value is the FP32-deviation form; the FP64 form (bit-exact with
+the reference) and the FP16-deviation storage form are timed beside it.
+
+Prints one JSON line (rank 0, N = 1).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, 8 TB/s
+BYTES_PER_LU = {"fp64": 288, "fp32dev": 144, "fp16dev": 72}   # 2 * 9 * 2 comps * bytes
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=1000)
+    p.add_argument("--warmup", type=int, default=50)
+    p.add_argument("--n", type=int, default=4096, help="lattice side")
+    p.add_argument("--forms", default="fp32dev,fp64,fp16dev")
+    p.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU baseline sample")
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--tile", default="", help="cols,rows override for the fused kernel")
+    return p.parse_args()
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def time_form(nat, wl, case, form, steps, warmup, tile):
+    g = nat.Geometry(case["mask"])
+    if tile:
+        c, r = [int(v) for v in tile.split(",")]
+        g.set_tile(c, r)
+    prec = nat.PRECISION[form]
+    fl = []
+    for c in range(2):
+        f = nat.Field(g, case["tau"][c], prec, 1.0, g_solid=0.0, g_fluid=case["g_fluid"][c])
+        f.set_f(case["f"][c])
+        fl.append(f)
+    nat.solid_surface_force(g, fl[0], fl[1])
+    nat.step2(g, fl[0], fl[1], None, None, warmup)
+    nat.synchronize()
+    nat.timing(1)                       # record every launch of the timed region
+    t0 = time.perf_counter()
+    nat.step2(g, fl[0], fl[1], None, None, steps)
+    nat.synchronize()
+    dt = time.perf_counter() - t0
+    avg_ms, cnt = nat.timing(0)
+    nat.timing(-1)
+    mass = [f.moments()[0] for f in fl]
+    tile_used = g.tile()
+    for f in fl:
+        f.close()
+    g.close()
+    n_sites = case["mask"].size
+    mlups = n_sites * steps / dt / 1e6
+    return dict(form=form, seconds=dt, ms_per_step=dt * 1e3 / steps, mlups=mlups,
+                kernel_avg_ms=avg_ms, kernel_launches=cnt, tile=list(tile_used),
+                achieved_gbs=n_sites * BYTES_PER_LU[form] / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else None,
+                pct_hbm=100.0 * mlups * 1e6 * BYTES_PER_LU[form] / (HBM_PEAK_GBS * 1e9), mass=mass)
+
+
+def cpu_baseline(seconds):
+    """Serial CPU time of the same two-component step on a bounded sample:
+    the compiled reference when it was built, else the oracle restatement."""
+    from oracle import pyoracle as po
+    from sc_lbm_amd import workloads as wl
+    kind = "reference" if po.ref_available() else "port"
+    n = 256
+    case = wl.spinodal(n, G=1.2, seed=5)
+    s = po.System(case["mask"], case["tau"], case["g_fluid"], case["g_solid"],
+                  kind="ref" if kind == "reference" else "oracle")
+    for c in range(2):
+        s.set_f(c, case["f"][c])
+    s.solid_surface_force()
+    s.run(2)
+    steps, t = 0, 0.0
+    chunk = 4
+    while t < seconds:
+        t0 = time.perf_counter()
+        s.run(chunk)
+        t += time.perf_counter() - t0
+        steps += chunk
+    mlups = n * n * steps / t / 1e6
+    cpu = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return dict(value=mlups, unit="MLUPS", cores=1, kind=kind,
+                sample="two-component Shan-Chen step, %dx%d periodic, %d steps in %.1f s, 1 thread of %d (%s)"
+                       % (n, n, steps, t, os.cpu_count() or 0, cpu))
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (--gpus %d refused)" % a.gpus)
+        sys.exit(2)
+    from sc_lbm_amd import native as nat
+    from sc_lbm_amd import workloads as wl
+    nat.init(0)
+    log("building the %dx%d two-component case" % (a.n, a.n))
+    case = wl.spinodal(a.n, G=1.2, seed=777)
+    res = {}
+    for form in a.forms.split(","):
+        log("timing %s: %d warmup + %d steps" % (form, a.warmup, a.steps))
+        res[form] = time_form(nat, wl, case, form, a.steps, a.warmup, a.tile)
+        log("  %s: %.1f MLUPS, kernel %.4f ms, %.1f%% of HBM peak"
+            % (form, res[form]["mlups"], res[form]["kernel_avg_ms"], res[form]["pct_hbm"]))
+    head = res.get("fp32dev") or next(iter(res.values()))
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_%s_%d.json" % (head["form"], a.n))
+    if os.path.exists(tpath):
+        traffic = json.load(open(tpath)).get("bytes_per_launch")
+    achieved = head["achieved_gbs"]
+    out = {
+        "metric": "MLUPS + %HBM peak (FP32-dev vs FP64), 2-comp SC; p50/p99 served-job latency",
+        "value": head["mlups"],
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if head["form"] == "fp32dev" else ("f64" if head["form"] == "fp64" else "f16-store/f32"),
+        "data": "synthetic",
+        "config": {"workload": "two-component Shan-Chen %dx%d periodic, tau 1, G 1.2, %s fused step"
+                               % (a.n, a.n, head["form"]),
+                   "lattice": [a.n, a.n], "components": 2, "form": head["form"],
+                   "tile": head["tile"], "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic},
+        "forms": {k: {kk: v[kk] for kk in ("mlups", "pct_hbm", "kernel_avg_ms", "achieved_gbs", "tile")}
+                  for k, v in res.items()},
+    }
+    if "fp64" in res and "fp32dev" in res:
+        out["fp32dev_over_fp64"] = res["fp32dev"]["mlups"] / res["fp64"]["mlups"]
+    if not a.no_cpu:
+        log("timing the CPU baseline (about %.0f s)" % a.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(a.cpu_seconds)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

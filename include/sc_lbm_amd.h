@@ -1,0 +1,167 @@
+/*
+ * sc_lbm_amd.h -- C-ABI of the MI355X (gfx950) D2Q9 Shan-Chen library
+ * (libsclbm.so).  Plain pointers and sizes only; every call returns an int
+ * status (SCL_OK = 0) and never throws; scl_last_error() returns the message
+ * of the last failure on the calling thread.
+ *
+ * The reference (LBMFluids/sc_lbm) has no FFI: its operator API is the C++
+ * classes LBM, Fluid and Geometry (include/lbm.h, include/fluid.h,
+ * include/geometry.h of the reference).  The entry points below are what
+ * those classes bind to; the drop-in classes in include/lbm.h, fluid.h and
+ * geometry.h of this repository call exactly these.  Each declaration names
+ * the reference method it replaces.
+ *
+ * Data model.
+ *   scl_geometry  the solid mask of an nx x ny lattice, kept on the device as
+ *                 one bit per site (1 = fluid), rows padded to 64 columns.
+ *   scl_field     one fluid component on that lattice: populations (double
+ *                 buffered, FP64, or FP32 / FP16 deviations f_i - w_i*rho0)
+ *                 plus the reference's per-site arrays (density, velocity or
+ *                 momentum, equilibrium velocity, forces, feq), all resident
+ *                 in device memory.
+ * Host arrays use the reference layout: site s = y*nx + x, population
+ * f[i*nx*ny + s], directions rest, E, N, W, S, NE, NW, SW, SE
+ * (reference include/fluid.h:24-39, include/lbm.h:126-128).
+ */
+#ifndef SC_LBM_AMD_H
+#define SC_LBM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+#define SCL_OK 0
+#define SCL_ERR_ARG 1        /* bad argument (sizes, null pointers, precision)   */
+#define SCL_ERR_HIP 2        /* HIP runtime failure (no device, launch, memory)  */
+#define SCL_ERR_RANGE 3      /* the reference would throw std::out_of_range      */
+#define SCL_ERR_UNSUPPORTED 4/* operation not available for this precision       */
+
+/* Population storage forms. */
+#define SCL_FP64 0           /* plain FP64, bit-exact with the reference          */
+#define SCL_FP32_DEV 1       /* FP32 deviations f_i - w_i*rho0, FP32 arithmetic   */
+#define SCL_FP16_DEV 2       /* FP16 of (f_i - w_i*rho0)*2^8, FP32 arithmetic     */
+
+/* Field selectors for scl_field_upload / scl_field_download. */
+#define SCL_F 0              /* populations, 9*nx*ny                                 */
+#define SCL_FEQ 1            /* equilibrium populations of the last collide, 9*nx*ny  */
+#define SCL_RHO 2            /* density, nx*ny                                        */
+#define SCL_UX 3             /* velocity (1 comp) or momentum (2 comp), nx*ny         */
+#define SCL_UY 4
+#define SCL_UEQX 5           /* equilibrium velocity, nx*ny                           */
+#define SCL_UEQY 6
+#define SCL_FRX 7            /* fluid-fluid (repulsive) force, nx*ny                  */
+#define SCL_FRY 8
+#define SCL_FSX 9            /* fluid-solid (adhesion) force, nx*ny                   */
+#define SCL_FSY 10
+
+typedef struct scl_geometry scl_geometry;
+typedef struct scl_field scl_field;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* scl_last_error(void);
+
+/* Select the HIP device used by the calling thread's subsequent creates
+ * (default 0).  Fails if no gfx950 device is present. */
+int scl_init(int device);
+
+/* Block until all work queued by this library has finished. */
+int scl_synchronize(void);
+
+/* ---- Geometry: replaces Geometry's mask storage and accessors
+ *      (include/geometry.h:58-59, 241-262 of the reference). ------------- */
+
+/* mask: nx*ny ints, row-major, 1 = fluid, 0 = solid (anything else is
+ * treated as solid except for the 1 test, as operator()==1 in the
+ * reference's loops). */
+int scl_geometry_create(int nx, int ny, const int* mask, scl_geometry** out);
+int scl_geometry_destroy(scl_geometry* g);
+
+/* ---- Field: replaces Fluid's storage, constructor and initialisers
+ *      (include/fluid.h:60-70, src/fluid.cpp:43-151 of the reference). --- */
+
+/* One component on geometry g.  omega is formed as 1/tau
+ * (include/fluid.h:62).  precision is SCL_FP64 / SCL_FP32_DEV / SCL_FP16_DEV;
+ * rho0 is the deviation reference density (ignored for SCL_FP64). All arrays
+ * start at zero, as zero_density_ini (src/fluid.cpp:43-64). */
+int scl_field_create(const scl_geometry* g, int precision, double tau, double rho0,
+                     scl_field** out);
+int scl_field_destroy(scl_field* f);
+
+/* Fluid::initialize_interactions(Gs, Gf) (include/fluid.h:110-114). */
+int scl_field_set_interactions(scl_field* f, double g_solid, double g_fluid);
+
+/* Copy a field in / out; n must equal its size.  Populations are converted
+ * to / from the storage form; the other arrays are FP64 and need SCL_FP64. */
+int scl_field_upload(scl_field* f, int which, const double* src, size_t n);
+int scl_field_download(scl_field* f, int which, double* dst, size_t n);
+
+/* ---- Per-operator entry points, reference semantics (SCL_FP64 only). --- */
+
+/* Fluid::compute_density, src/fluid.cpp:175-183 */
+int scl_compute_density(scl_field* f);
+/* Fluid::compute_velocities, src/fluid.cpp:186-200 */
+int scl_compute_velocities(const scl_geometry* g, scl_field* f);
+/* Fluid::compute_f_equilibrium(geom), src/fluid.cpp:210-243 */
+int scl_compute_f_equilibrium_geom(const scl_geometry* g, scl_field* f);
+/* Fluid::compute_f_equilibrium(), src/fluid.cpp:246-278 */
+int scl_compute_f_equilibrium(scl_field* f);
+/* LBM::compute_solid_surface_force + Fluid::add_surface_forces,
+ * src/lbm.cpp:102-141, src/fluid.cpp:154-160 (first call per field wins) */
+int scl_solid_surface_force(const scl_geometry* g, scl_field* a, scl_field* b);
+/* LBM::compute_fluid_repulsive_interactions, src/lbm.cpp:144-206 */
+int scl_fluid_repulsion(const scl_geometry* g, scl_field* a, scl_field* b);
+/* LBM::compute_equilibrium_velocities, src/lbm.cpp:209-272
+ * (SCL_ERR_RANGE before scl_solid_surface_force, as the reference's .at()) */
+int scl_equilibrium_velocities(const scl_geometry* g, scl_field* a, scl_field* b);
+/* LBM::collide(geom, fluid), src/lbm.cpp:275-288 */
+int scl_collide1(const scl_geometry* g, scl_field* f);
+/* LBM::collide(fluid, fluid), src/lbm.cpp:291-312 */
+int scl_collide2(scl_field* a, scl_field* b);
+/* LBM::add_volume_force(geom, fluid, force), src/lbm.cpp:315-325 */
+int scl_volume_force(const scl_geometry* g, scl_field* f, const double force[9]);
+/* LBM::stream(geom, fluid), src/lbm.cpp:343-384 */
+int scl_stream1(const scl_geometry* g, scl_field* f);
+/* LBM::stream(geom, fluid, fluid), src/lbm.cpp:387-435 */
+int scl_stream2(const scl_geometry* g, scl_field* a, scl_field* b);
+
+/* ---- Fused steps: nsteps repetitions of the reference's step loop. ----- */
+
+/* collide(geom,f); [add_volume_force(geom,f,force)]; stream(geom,f)
+ * (benchmarks/single_phase_one_component_flows/laminar_flow_x_dir.cpp:64-68).
+ * force may be NULL. */
+int scl_step1(const scl_geometry* g, scl_field* f, const double* force, int nsteps);
+
+/* a.compute_density(); b.compute_density();
+ * compute_fluid_repulsive_interactions; compute_equilibrium_velocities;
+ * collide(a,b); [add_volume_force(geom,a,force_a)];
+ * [add_volume_force(geom,b,force_b)]; stream(geom,a,b)
+ * (phase_separation.cpp:99-108, tests/lbm/lbm_tests.cpp:121-127).
+ * force_a / force_b may be NULL.  Requires scl_solid_surface_force first,
+ * as the reference does (SCL_ERR_RANGE otherwise). */
+int scl_step2(const scl_geometry* g, scl_field* a, scl_field* b,
+              const double* force_a, const double* force_b, int nsteps);
+
+/* ---- Diagnostics (no reference counterpart: FP64 device reductions). -- */
+
+/* Total mass and momentum of the current populations, summed in FP64 on the
+ * device in a fixed order: out = {mass, momentum_x, momentum_y}. */
+int scl_moments(const scl_geometry* g, scl_field* f, double out[3]);
+
+/* Launch shape of the fused kernels for geometry g: columns per workgroup
+ * (64, 128, 256 or 512) and rows per workgroup (>= 1).  0 = automatic.
+ * Results do not depend on the shape (per-site arithmetic is fixed). */
+int scl_set_tile(scl_geometry* g, int cols, int rows);
+int scl_get_tile(const scl_geometry* g, int* cols, int* rows);
+
+/* Average duration in milliseconds of the fused kernel launches since the
+ * last reset, measured with HIP events on the library's stream. */
+int scl_timing(int reset, double* avg_ms, int* count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,101 @@
+// lattice.h -- D2Q9 constants, device layout and population storage forms
+// shared by the gfx950 kernels.
+//
+// Device layout (all arrays resident in HBM):
+//   pitch   = nx rounded up to a multiple of 64 sites; site (x, y) lives at
+//             y*pitch + x in every per-site array.
+//   plane   = elements between consecutive population directions; population
+//             i of site s is at i*plane + s (structure of arrays, so a
+//             wavefront touches 64 consecutive sites of one direction).
+//   mask    = one bit per site, 1 = fluid, packed little-endian in 64-bit
+//             words, pitch/64 words per row.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace scl {
+
+constexpr int Q = 9;
+// Lattice velocities, direction order rest, E, N, W, S, NE, NW, SW, SE
+// (reference include/lbm.h:126-128).
+__host__ __device__ constexpr int cx(int i) { return i == 1 || i == 5 || i == 8 ? 1 : (i == 3 || i == 6 || i == 7 ? -1 : 0); }
+__host__ __device__ constexpr int cy(int i) { return i == 2 || i == 5 || i == 6 ? 1 : (i == 4 || i == 7 || i == 8 ? -1 : 0); }
+// Opposite direction (reference bounce-back rules, include/lbm.h:124).
+__host__ __device__ constexpr int opp(int i) { return i == 0 ? 0 : (i <= 4 ? ((i + 1) % 4) + 1 : ((i - 3) % 4) + 5); }
+// Interaction weights for the fluid-fluid and fluid-solid sums
+// (include/lbm.h:118-122): 1/9 axis, 1/36 diagonal.
+__host__ __device__ constexpr double wint(int i) { return i == 0 ? 0.0 : (i <= 4 ? 1.0 / 9 : 1.0 / 36); }
+// Equilibrium weights (include/fluid.h:282).
+__host__ __device__ constexpr double weq(int i) { return i == 0 ? 4.0 / 9.0 : (i <= 4 ? 1.0 / 9.0 : 1.0 / 36.0); }
+
+static_assert(opp(1) == 3 && opp(2) == 4 && opp(3) == 1 && opp(4) == 2, "opp axis");
+static_assert(opp(5) == 7 && opp(6) == 8 && opp(7) == 5 && opp(8) == 6, "opp diag");
+
+// Density below which the reference keeps the previous equilibrium velocity
+// (include/utils.h:145-150 with tol 1e-16, src/lbm.cpp:262-269).
+constexpr double kZeroTol = 1e-16;
+// Fused kernels record u_eq in HBM only where a component's density is below
+// this value (the only sites where the next step can fall back to it).
+constexpr double kUeqWatch = 1e-8;
+
+// FP16 deviation scale: stored value = (f_i - w_i*rho0) * 2^8.
+constexpr float kHalfScale = 256.0f;
+constexpr float kHalfInv = 1.0f / 256.0f;
+
+// ---------------------------------------------------------------------------
+// Storage forms.  T is the arithmetic type, S the stored type.  The
+// deviation forms store d_i = f_i - w_i*rho0.
+// ---------------------------------------------------------------------------
+struct StoreF64 {
+    using S = double;
+    using T = double;
+    static constexpr bool kDev = false;
+    __device__ static inline T ld(const S* p) { return *p; }
+    __device__ static inline void st(S* p, T v) { *p = v; }
+};
+
+struct StoreF32 {
+    using S = float;
+    using T = float;
+    static constexpr bool kDev = true;
+    __device__ static inline T ld(const S* p) { return *p; }
+    __device__ static inline void st(S* p, T v) { *p = v; }
+};
+
+struct StoreF16 {
+    using S = _Float16;
+    using T = float;
+    static constexpr bool kDev = true;
+    __device__ static inline T ld(const S* p) { return (float)(*p) * kHalfInv; }
+    __device__ static inline void st(S* p, T v) { *p = (_Float16)(v * kHalfScale); }
+};
+
+// Wrap a column / row index that is at most one step outside [0, n).
+__device__ inline int wrap1(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
+
+__device__ inline int fluid_bit(const uint64_t* __restrict__ mask, int wpr, int row, int col) {
+    return (int)((mask[(size_t)row * wpr + (col >> 6)] >> (col & 63)) & 1ull);
+}
+
+// Bits (col-1, col, col+1) of one row: bit0 west, bit1 centre, bit2 east.
+__device__ inline unsigned triple(const uint64_t* __restrict__ mask, int wpr, int row, int colm,
+                                  int col, int colp) {
+    return (unsigned)fluid_bit(mask, wpr, row, colm) | ((unsigned)fluid_bit(mask, wpr, row, col) << 1) |
+           ((unsigned)fluid_bit(mask, wpr, row, colp) << 2);
+}
+
+// 9-bit neighbourhood: bit i = fluid(x + c_i, y + c_i) from the triples of
+// rows y-1 (s), y (c) and y+1 (n).
+__device__ inline unsigned neigh_bits(unsigned s, unsigned c, unsigned n) {
+    return ((c >> 1) & 1u)               // 0 rest
+           | (((c >> 2) & 1u) << 1)      // 1 E
+           | (((n >> 1) & 1u) << 2)      // 2 N
+           | ((c & 1u) << 3)             // 3 W
+           | (((s >> 1) & 1u) << 4)      // 4 S
+           | (((n >> 2) & 1u) << 5)      // 5 NE
+           | ((n & 1u) << 6)             // 6 NW
+           | ((s & 1u) << 7)             // 7 SW
+           | (((s >> 2) & 1u) << 8);     // 8 SE
+}
+
+}  // namespace scl

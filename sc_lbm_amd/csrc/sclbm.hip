@@ -1,0 +1,901 @@
+// sclbm.hip -- runtime and C-ABI of libsclbm.so (see include/sc_lbm_amd.h).
+//
+// Owns the device state of geometries and fields, tracks the population
+// phase of each field (reference streamed state S, or post-collision P whose
+// streamed state is Gather(P)), dispatches the fused and per-operator
+// kernels, and recomputes the reference's intermediate arrays of the last
+// fused step on demand ("replay": the input buffer of that step is still
+// intact, so rerunning it in replay mode writes exactly the arrays the
+// reference would hold).
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <type_traits>
+#include <string>
+#include <vector>
+
+#include "../../include/sc_lbm_amd.h"
+#include "lattice.h"
+#include "ops.h"
+#include "step.h"
+
+using namespace scl;
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local std::string g_err;
+
+static int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                            \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            return fail(SCL_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
+                        __LINE__);                                                              \
+    } while (0)
+
+#define CHK(expr)                  \
+    do {                           \
+        int rc_ = (expr);          \
+        if (rc_ != SCL_OK) return rc_; \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// library state
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Lib {
+    bool ready = false;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+};
+Lib g_lib;
+const size_t kMaxTimed = 1 << 15;
+
+int lib_ready() {
+    if (g_lib.ready) return SCL_OK;
+    int n = 0;
+    HIPCHK(hipGetDeviceCount(&n));
+    if (n <= g_lib.device) return fail(SCL_ERR_HIP, "HIP device %d not present (%d devices)", g_lib.device, n);
+    HIPCHK(hipSetDevice(g_lib.device));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, g_lib.device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(SCL_ERR_HIP, "device %d is %s; this library is built for gfx950", g_lib.device, prop.gcnArchName);
+    // default priority (no priority argument): the library never changes it
+    HIPCHK(hipStreamCreateWithFlags(&g_lib.stream, hipStreamNonBlocking));
+    g_lib.ready = true;
+    return SCL_OK;
+}
+
+template <class T>
+int dalloc(T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) return SCL_OK;
+    HIPCHK(hipMalloc((void**)p, count * sizeof(T)));
+    return SCL_OK;
+}
+
+void dfree(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+}  // namespace
+
+struct scl_geometry {
+    int nx = 0, ny = 0, pitch = 0, wpr = 0;
+    long long plane = 0;
+    uint64_t* mask = nullptr;
+    std::vector<int> host_mask;
+    int tile_cols = 0, tile_rows = 0;   // 0 = automatic
+    double* part = nullptr;             // moment partials (3 per row + 3)
+    // Fields keep their geometry alive: destroying a geometry that fields
+    // still use only marks it, and the last field destroyed frees it.
+    int refs = 0;
+    bool dead = false;
+    Grid2 grid() const { return Grid2{nx, ny, pitch, wpr, plane, mask}; }
+};
+
+namespace {
+enum Phase { PH_S = 0, PH_P = 1 };
+
+struct Replay;
+}  // namespace
+
+struct scl_field {
+    scl_geometry* g = nullptr;
+    int precision = SCL_FP64;
+    double tau = 1.0, omega = 1.0, omc = 0.0, inv_omega = 1.0, rho0 = 0.0;
+    double g_solid = 0.0, g_fluid = 0.0;
+    bool has_solid_force = false;
+    double gsneg_captured = 0.0;
+    void* f[2] = {nullptr, nullptr};
+    int cur = 0;
+    int phase = PH_S;
+    // reference per-site arrays (FP64, pitched)
+    double *rho = nullptr, *ux = nullptr, *uy = nullptr, *ueqx = nullptr, *ueqy = nullptr;
+    double *frx = nullptr, *fry = nullptr, *fsx = nullptr, *fsy = nullptr, *feq = nullptr;
+    double* stage = nullptr;   // FP64 staging for conversions
+    std::shared_ptr<Replay> replay;
+    size_t esize() const { return precision == SCL_FP64 ? 8 : (precision == SCL_FP32_DEV ? 4 : 2); }
+};
+
+namespace {
+
+struct Replay {
+    int nc = 0;
+    bool gather = false;
+    scl_field* fl[2] = {nullptr, nullptr};
+    StepArgs<double> args{};
+};
+
+size_t n_sites(const scl_geometry* g) { return (size_t)g->pitch * g->ny; }
+size_t n_pop(const scl_geometry* g) { return (size_t)g->plane * Q; }
+
+int ensure_aux(scl_field* f) {
+    if (f->rho) return SCL_OK;
+    const scl_geometry* g = f->g;
+    const size_t n = n_sites(g);
+    double** arrs[] = {&f->rho, &f->ux, &f->uy, &f->frx, &f->fry, &f->fsx, &f->fsy};
+    for (double** a : arrs) {
+        CHK(dalloc(a, n));
+        HIPCHK(hipMemsetAsync(*a, 0, n * sizeof(double), g_lib.stream));
+    }
+    CHK(dalloc(&f->feq, n_pop(g)));
+    HIPCHK(hipMemsetAsync(f->feq, 0, n_pop(g) * sizeof(double), g_lib.stream));
+    return SCL_OK;
+}
+
+int ensure_stage(scl_field* f) {
+    if (f->stage) return SCL_OK;
+    return dalloc(&f->stage, n_pop(f->g));
+}
+
+dim3 site_grid(const scl_geometry* g) { return dim3((g->nx + 255) / 256, g->ny); }
+
+// ---- fused-kernel dispatch ------------------------------------------------
+void auto_tile(const scl_geometry* g, int& nw, int& rows) {
+    nw = g->tile_cols ? g->tile_cols / 64 : 0;
+    rows = g->tile_rows;
+    if (!nw) {
+        nw = (g->nx + 63) / 64;
+        if (nw > 4) nw = 4;
+        if (nw == 3) nw = 4;
+    }
+    if (!rows) {
+        const int nbx = (g->nx + 64 * nw - 1) / (64 * nw);
+        long long r = (long long)nbx * g->ny / 1024;
+        rows = (int)(r < 4 ? 4 : (r > 64 ? 64 : r));
+        if (rows > g->ny) rows = g->ny;
+    }
+}
+
+template <int NC, class ST, int NW, bool GATHER, bool REPLAY>
+void launch_step_t(const StepArgs<typename ST::T>& a, int nblk) {
+    hipLaunchKernelGGL((k_step<NC, ST, NW, GATHER, REPLAY>), dim3(nblk), dim3(64 * NW), 0, g_lib.stream, a);
+}
+
+template <int NC, class ST, bool GATHER>
+int launch_nw(int nw, const StepArgs<typename ST::T>& a, int nblk) {
+    switch (nw) {
+        case 1: launch_step_t<NC, ST, 1, GATHER, false>(a, nblk); break;
+        case 2: launch_step_t<NC, ST, 2, GATHER, false>(a, nblk); break;
+        case 4: launch_step_t<NC, ST, 4, GATHER, false>(a, nblk); break;
+        case 8: launch_step_t<NC, ST, 8, GATHER, false>(a, nblk); break;
+        default: return fail(SCL_ERR_ARG, "unsupported tile width %d", 64 * nw);
+    }
+    return SCL_OK;
+}
+
+template <class T>
+void fill_comp(CompArgs<T>& c, const scl_field* f, int src_buf, const double* force) {
+    c.src = f->f[src_buf];
+    c.dst = f->f[src_buf ^ 1];
+    c.omega = (T)f->omega;
+    c.omc = (T)f->omc;
+    c.inv_omega = (T)f->inv_omega;
+    c.gneg = (T)(-1.0 * f->g_fluid);
+    c.gsneg = (T)f->gsneg_captured;
+    c.rho0 = (T)f->rho0;
+    c.has_force = force ? 1 : 0;
+    for (int i = 0; i < Q; ++i) c.force[i] = force ? (T)force[i] : T(0);
+    c.ueqx = f->ueqx;
+    c.ueqy = f->ueqy;
+    c.rho = f->rho;
+    c.jx = f->ux;
+    c.jy = f->uy;
+    c.frx = f->frx;
+    c.fry = f->fry;
+    c.feq = f->feq;
+}
+
+template <class T>
+StepArgs<T> make_args(const scl_geometry* g, int rows, int nw) {
+    StepArgs<T> a{};
+    a.nx = g->nx;
+    a.ny = g->ny;
+    a.pitch = g->pitch;
+    a.wpr = g->wpr;
+    a.plane = g->plane;
+    a.mask = g->mask;
+    a.rows = rows;
+    a.gx = (g->nx + 64 * nw - 1) / (64 * nw);
+    return a;
+}
+
+int nblocks(const scl_geometry* g, int nw, int rows) {
+    const int gx = (g->nx + 64 * nw - 1) / (64 * nw);
+    const int gy = (g->ny + rows - 1) / rows;
+    return gx * gy;
+}
+
+template <int NC, class ST>
+int launch_fused(const scl_geometry* g, scl_field* const* fl, const double* const* forces, bool gather,
+                 std::shared_ptr<Replay>* rec) {
+    using T = typename ST::T;
+    int nw, rows;
+    auto_tile(g, nw, rows);
+    StepArgs<T> a = make_args<T>(g, rows, nw);
+    for (int c = 0; c < NC; ++c) fill_comp(a.c[c], fl[c], fl[c]->cur, forces[c]);
+    const int nblk = nblocks(g, nw, rows);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (g_lib.timing && g_lib.ev.size() < kMaxTimed) {
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, g_lib.stream));
+    }
+    int rc = gather ? launch_nw<NC, ST, true>(nw, a, nblk) : launch_nw<NC, ST, false>(nw, a, nblk);
+    if (rc != SCL_OK) return rc;
+    HIPCHK(hipGetLastError());
+    if (e0) {
+        HIPCHK(hipEventRecord(e1, g_lib.stream));
+        g_lib.ev.emplace_back(e0, e1);
+    }
+    if (rec) {
+        if constexpr (std::is_same<ST, StoreF64>::value) {
+            auto r = std::make_shared<Replay>();
+            r->nc = NC;
+            r->gather = gather;
+            for (int c = 0; c < NC; ++c) r->fl[c] = fl[c];
+            // the replay reads the same source buffers; its outputs are the
+            // reference arrays, so the replay launch uses the 4-wave shape
+            StepArgs<double> ra = make_args<double>(g, rows, 4);
+            for (int c = 0; c < NC; ++c) ra.c[c] = a.c[c];
+            r->args = ra;
+            *rec = r;
+        }
+    }
+    return SCL_OK;
+}
+
+int run_replay(Replay& r) {
+    const scl_geometry* g = r.fl[0]->g;
+    const int nblk = nblocks(g, 4, r.args.rows);
+    if (r.nc == 1) {
+        if (r.gather) hipLaunchKernelGGL((k_step<1, StoreF64, 4, true, true>), dim3(nblk), dim3(256), 0, g_lib.stream, r.args);
+        else hipLaunchKernelGGL((k_step<1, StoreF64, 4, false, true>), dim3(nblk), dim3(256), 0, g_lib.stream, r.args);
+    } else {
+        if (r.gather) hipLaunchKernelGGL((k_step<2, StoreF64, 4, true, true>), dim3(nblk), dim3(256), 0, g_lib.stream, r.args);
+        else hipLaunchKernelGGL((k_step<2, StoreF64, 4, false, true>), dim3(nblk), dim3(256), 0, g_lib.stream, r.args);
+    }
+    HIPCHK(hipGetLastError());
+    return SCL_OK;
+}
+
+// Write the intermediate arrays of the last fused step touching f, if any.
+int materialize(scl_field* f) {
+    if (!f->replay) return SCL_OK;
+    std::shared_ptr<Replay> r = f->replay;
+    CHK(run_replay(*r));
+    for (int c = 0; c < r->nc; ++c)
+        if (r->fl[c] && r->fl[c]->replay == r) r->fl[c]->replay.reset();
+    return SCL_OK;
+}
+
+// Bring the populations of f to the reference's streamed state in place.
+int ensure_s(scl_field* f) {
+    if (f->phase == PH_S) return SCL_OK;
+    CHK(materialize(f));
+    const Grid2 gr = f->g->grid();
+    const int nxt = f->cur ^ 1;
+    switch (f->precision) {
+        case SCL_FP64:
+            hipLaunchKernelGGL(k_gather<StoreF64>, site_grid(f->g), dim3(256), 0, g_lib.stream, gr,
+                               (const double*)f->f[f->cur], (double*)f->f[nxt]);
+            break;
+        case SCL_FP32_DEV:
+            hipLaunchKernelGGL(k_gather<StoreF32>, site_grid(f->g), dim3(256), 0, g_lib.stream, gr,
+                               (const float*)f->f[f->cur], (float*)f->f[nxt]);
+            break;
+        default:
+            hipLaunchKernelGGL(k_gather<StoreF16>, site_grid(f->g), dim3(256), 0, g_lib.stream, gr,
+                               (const _Float16*)f->f[f->cur], (_Float16*)f->f[nxt]);
+            break;
+    }
+    HIPCHK(hipGetLastError());
+    f->cur = nxt;
+    f->phase = PH_S;
+    return SCL_OK;
+}
+
+// Every per-operator entry: keep the last step's intermediates, then work
+// on the streamed state.
+int prepare_op(scl_field* f) {
+    if (!f) return fail(SCL_ERR_ARG, "null field");
+    if (f->precision != SCL_FP64)
+        return fail(SCL_ERR_UNSUPPORTED, "per-operator calls need an SCL_FP64 field");
+    CHK(ensure_aux(f));
+    CHK(materialize(f));
+    CHK(ensure_s(f));
+    return SCL_OK;
+}
+
+int check_geom(const scl_geometry* g, const scl_field* f) {
+    if (!g || !f) return fail(SCL_ERR_ARG, "null geometry or field");
+    if (g->nx != f->g->nx || g->ny != f->g->ny)
+        return fail(SCL_ERR_ARG, "geometry %dx%d does not match the field's %dx%d", g->nx, g->ny, f->g->nx,
+                    f->g->ny);
+    return SCL_OK;
+}
+
+// Drop or materialize replay records before a fused step on `fl`.
+int settle_replays(scl_field* const* fl, int nc) {
+    for (int c = 0; c < nc; ++c) {
+        std::shared_ptr<Replay> r = fl[c]->replay;
+        if (!r) continue;
+        bool same = r->nc == nc;
+        for (int k = 0; k < r->nc && same; ++k) same = (r->fl[k] == fl[0] || (nc == 2 && r->fl[k] == fl[1]));
+        if (same) {
+            for (int k = 0; k < r->nc; ++k)
+                if (r->fl[k]) r->fl[k]->replay.reset();
+        } else {
+            CHK(materialize(fl[c]));
+        }
+    }
+    return SCL_OK;
+}
+
+}  // namespace
+
+namespace {
+template <int NC>
+int fused_steps(const scl_geometry* g, scl_field** fl, const double* const* forces, int nsteps) {
+    const int prec = fl[0]->precision;
+    for (int c = 0; c < NC; ++c) {
+        CHK(check_geom(g, fl[c]));
+        if (fl[c]->precision != prec) return fail(SCL_ERR_ARG, "components have different precisions");
+        if (prec == SCL_FP64) CHK(ensure_aux(fl[c]));
+    }
+    if (NC == 2) {
+        if (fl[0] == fl[1]) return fail(SCL_ERR_ARG, "the two components must be different fields");
+        if (!fl[0]->has_solid_force || !fl[1]->has_solid_force)
+            return fail(SCL_ERR_RANGE, "fluid-solid force not computed (compute_solid_surface_force first)");
+    }
+    if (nsteps <= 0) return SCL_OK;
+    CHK(settle_replays(fl, NC));
+    // both components must be in the same phase
+    if (NC == 2 && fl[0]->phase != fl[1]->phase) {
+        CHK(ensure_s(fl[0]));
+        CHK(ensure_s(fl[1]));
+    }
+    for (int t = 0; t < nsteps; ++t) {
+        const bool gather = fl[0]->phase == PH_P;
+        std::shared_ptr<Replay> rec;
+        const bool last = t == nsteps - 1;
+        int rc;
+        switch (prec) {
+            case SCL_FP64: rc = launch_fused<NC, StoreF64>(g, fl, forces, gather, last ? &rec : nullptr); break;
+            case SCL_FP32_DEV: rc = launch_fused<NC, StoreF32>(g, fl, forces, gather, nullptr); break;
+            default: rc = launch_fused<NC, StoreF16>(g, fl, forces, gather, nullptr); break;
+        }
+        if (rc != SCL_OK) return rc;
+        for (int c = 0; c < NC; ++c) {
+            fl[c]->cur ^= 1;
+            fl[c]->phase = PH_P;
+            if (rec) fl[c]->replay = rec;
+        }
+    }
+    return SCL_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+extern "C" {
+
+const char* scl_last_error(void) { return g_err.c_str(); }
+
+int scl_init(int device) {
+    if (g_lib.ready && g_lib.device != device)
+        return fail(SCL_ERR_ARG, "library already initialised on device %d", g_lib.device);
+    g_lib.device = device;
+    return lib_ready();
+}
+
+int scl_synchronize(void) {
+    CHK(lib_ready());
+    HIPCHK(hipStreamSynchronize(g_lib.stream));
+    return SCL_OK;
+}
+
+int scl_geometry_create(int nx, int ny, const int* mask, scl_geometry** out) {
+    if (!out) return fail(SCL_ERR_ARG, "null output");
+    *out = nullptr;
+    if (nx <= 0 || ny <= 0 || !mask) return fail(SCL_ERR_ARG, "bad geometry size %dx%d or null mask", nx, ny);
+    if ((long long)nx * ny * 9 > (1ll << 31) - 1) return fail(SCL_ERR_ARG, "lattice %dx%d too large", nx, ny);
+    CHK(lib_ready());
+    auto g = std::unique_ptr<scl_geometry>(new scl_geometry);
+    g->nx = nx;
+    g->ny = ny;
+    g->pitch = (nx + 63) / 64 * 64;
+    g->wpr = g->pitch / 64;
+    g->plane = (long long)g->pitch * ny;
+    g->host_mask.assign(mask, mask + (size_t)nx * ny);
+    std::vector<uint64_t> bits((size_t)g->wpr * ny, 0ull);
+    for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x)
+            if (mask[(size_t)y * nx + x] == 1) bits[(size_t)y * g->wpr + (x >> 6)] |= 1ull << (x & 63);
+    CHK(dalloc(&g->mask, bits.size()));
+    HIPCHK(hipMemcpy(g->mask, bits.data(), bits.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+    CHK(dalloc(&g->part, (size_t)3 * ny + 3));
+    *out = g.release();
+    return SCL_OK;
+}
+
+static void geometry_free(scl_geometry* g) {
+    if (g_lib.ready) (void)hipStreamSynchronize(g_lib.stream);
+    dfree(g->mask);
+    dfree(g->part);
+    delete g;
+}
+
+int scl_geometry_destroy(scl_geometry* g) {
+    if (!g) return SCL_OK;
+    if (g->refs > 0) {
+        g->dead = true;   // freed by the last field that uses it
+        return SCL_OK;
+    }
+    geometry_free(g);
+    return SCL_OK;
+}
+
+int scl_set_tile(scl_geometry* g, int cols, int rows) {
+    if (!g) return fail(SCL_ERR_ARG, "null geometry");
+    if (cols != 0 && cols != 64 && cols != 128 && cols != 256 && cols != 512)
+        return fail(SCL_ERR_ARG, "tile columns must be 64, 128, 256 or 512 (got %d)", cols);
+    if (rows < 0) return fail(SCL_ERR_ARG, "tile rows must be >= 0");
+    g->tile_cols = cols;
+    g->tile_rows = rows;
+    return SCL_OK;
+}
+
+int scl_get_tile(const scl_geometry* g, int* cols, int* rows) {
+    if (!g || !cols || !rows) return fail(SCL_ERR_ARG, "null argument");
+    int nw, r;
+    auto_tile(g, nw, r);
+    *cols = 64 * nw;
+    *rows = r;
+    return SCL_OK;
+}
+
+int scl_field_create(const scl_geometry* g, int precision, double tau, double rho0, scl_field** out) {
+    if (!out) return fail(SCL_ERR_ARG, "null output");
+    *out = nullptr;
+    if (!g) return fail(SCL_ERR_ARG, "null geometry");
+    if (precision < SCL_FP64 || precision > SCL_FP16_DEV) return fail(SCL_ERR_ARG, "bad precision %d", precision);
+    CHK(lib_ready());
+    auto f = std::unique_ptr<scl_field>(new scl_field);
+    if (g->dead) return fail(SCL_ERR_ARG, "geometry already destroyed");
+    f->g = const_cast<scl_geometry*>(g);
+    f->precision = precision;
+    f->tau = tau;
+    f->omega = 1 / tau;              // include/fluid.h:62
+    f->omc = 1.0 - f->omega;         // src/lbm.cpp:284, 308
+    f->inv_omega = 1.0 / f->omega;   // src/lbm.cpp:213, 226
+    f->rho0 = precision == SCL_FP64 ? 0.0 : rho0;
+    const size_t bytes = n_pop(g) * f->esize();
+    for (int b = 0; b < 2; ++b) {
+        HIPCHK(hipMalloc(&f->f[b], bytes));
+        HIPCHK(hipMemsetAsync(f->f[b], 0, bytes, g_lib.stream));
+    }
+    if (precision != SCL_FP64) {
+        // zero populations in deviation form: d_i = -w_i*rho0 on every site
+        CHK(ensure_stage(f.get()));
+        HIPCHK(hipMemsetAsync(f->stage, 0, n_pop(g) * sizeof(double), g_lib.stream));
+        const Grid2 gr = g->grid();
+        if (precision == SCL_FP32_DEV)
+            hipLaunchKernelGGL(k_from_f64<StoreF32>, site_grid(g), dim3(256), 0, g_lib.stream, gr, f->stage, f->rho0,
+                               (float*)f->f[0]);
+        else
+            hipLaunchKernelGGL(k_from_f64<StoreF16>, site_grid(g), dim3(256), 0, g_lib.stream, gr, f->stage, f->rho0,
+                               (_Float16*)f->f[0]);
+        HIPCHK(hipGetLastError());
+    }
+    const size_t n = n_sites(g);
+    CHK(dalloc(&f->ueqx, n));
+    CHK(dalloc(&f->ueqy, n));
+    HIPCHK(hipMemsetAsync(f->ueqx, 0, n * sizeof(double), g_lib.stream));
+    HIPCHK(hipMemsetAsync(f->ueqy, 0, n * sizeof(double), g_lib.stream));
+    f->g->refs++;
+    *out = f.release();
+    return SCL_OK;
+}
+
+int scl_field_destroy(scl_field* f) {
+    if (!f) return SCL_OK;
+    if (g_lib.ready) (void)hipStreamSynchronize(g_lib.stream);
+    if (f->replay) {
+        std::shared_ptr<Replay> r = f->replay;
+        for (int c = 0; c < r->nc; ++c)
+            if (r->fl[c] && r->fl[c] != f) {
+                (void)materialize(r->fl[c]);
+            }
+        for (int c = 0; c < r->nc; ++c)
+            if (r->fl[c]) r->fl[c]->replay.reset();
+        (void)hipStreamSynchronize(g_lib.stream);
+    }
+    for (void* p : {f->f[0], f->f[1]}) dfree(p);
+    for (double* p : {f->rho, f->ux, f->uy, f->ueqx, f->ueqy, f->frx, f->fry, f->fsx, f->fsy, f->feq, f->stage})
+        dfree(p);
+    scl_geometry* g = f->g;
+    delete f;
+    if (--g->refs == 0 && g->dead) geometry_free(g);
+    return SCL_OK;
+}
+
+int scl_field_set_interactions(scl_field* f, double g_solid, double g_fluid) {
+    if (!f) return fail(SCL_ERR_ARG, "null field");
+    f->g_solid = g_solid;
+    f->g_fluid = g_fluid;
+    return SCL_OK;
+}
+
+static double* aux_ptr(scl_field* f, int which) {
+    switch (which) {
+        case SCL_FEQ: return f->feq;
+        case SCL_RHO: return f->rho;
+        case SCL_UX: return f->ux;
+        case SCL_UY: return f->uy;
+        case SCL_UEQX: return f->ueqx;
+        case SCL_UEQY: return f->ueqy;
+        case SCL_FRX: return f->frx;
+        case SCL_FRY: return f->fry;
+        case SCL_FSX: return f->fsx;
+        case SCL_FSY: return f->fsy;
+        default: return nullptr;
+    }
+}
+
+static int copy_planes(const scl_geometry* g, void* dev, const double* host, int planes, bool up) {
+    const size_t spitch = (size_t)g->nx * sizeof(double), dpitch = (size_t)g->pitch * sizeof(double);
+    for (int i = 0; i < planes; ++i) {
+        double* d = (double*)dev + (size_t)i * g->plane;
+        const double* h = host + (size_t)i * g->nx * g->ny;
+        if (up)
+            HIPCHK(hipMemcpy2DAsync(d, dpitch, h, spitch, spitch, g->ny, hipMemcpyHostToDevice, g_lib.stream));
+        else
+            HIPCHK(hipMemcpy2DAsync((void*)h, spitch, d, dpitch, spitch, g->ny, hipMemcpyDeviceToHost, g_lib.stream));
+    }
+    return SCL_OK;
+}
+
+int scl_field_upload(scl_field* f, int which, const double* src, size_t n) {
+    if (!f || !src) return fail(SCL_ERR_ARG, "null argument");
+    const scl_geometry* g = f->g;
+    const size_t ns = (size_t)g->nx * g->ny;
+    if (which == SCL_F) {
+        if (n != 9 * ns) return fail(SCL_ERR_ARG, "populations need %zu values, got %zu", 9 * ns, n);
+        CHK(materialize(f));
+        if (f->precision == SCL_FP64) {
+            CHK(copy_planes(g, f->f[f->cur], src, 9, true));
+        } else {
+            CHK(ensure_stage(f));
+            CHK(copy_planes(g, f->stage, src, 9, true));
+            const Grid2 gr = g->grid();
+            if (f->precision == SCL_FP32_DEV)
+                hipLaunchKernelGGL(k_from_f64<StoreF32>, site_grid(g), dim3(256), 0, g_lib.stream, gr, f->stage,
+                                   f->rho0, (float*)f->f[f->cur]);
+            else
+                hipLaunchKernelGGL(k_from_f64<StoreF16>, site_grid(g), dim3(256), 0, g_lib.stream, gr, f->stage,
+                                   f->rho0, (_Float16*)f->f[f->cur]);
+            HIPCHK(hipGetLastError());
+        }
+        f->phase = PH_S;
+        HIPCHK(hipStreamSynchronize(g_lib.stream));
+        return SCL_OK;
+    }
+    if (which < SCL_FEQ || which > SCL_FSY) return fail(SCL_ERR_ARG, "bad field selector %d", which);
+    const size_t want = which == SCL_FEQ ? 9 * ns : ns;
+    if (n != want) return fail(SCL_ERR_ARG, "field %d needs %zu values, got %zu", which, want, n);
+    if (which != SCL_UEQX && which != SCL_UEQY) {
+        if (f->precision != SCL_FP64) return fail(SCL_ERR_UNSUPPORTED, "field %d needs an SCL_FP64 field", which);
+        CHK(ensure_aux(f));
+    }
+    CHK(materialize(f));
+    CHK(copy_planes(g, aux_ptr(f, which), src, which == SCL_FEQ ? 9 : 1, true));
+    HIPCHK(hipStreamSynchronize(g_lib.stream));
+    return SCL_OK;
+}
+
+int scl_field_download(scl_field* f, int which, double* dst, size_t n) {
+    if (!f || !dst) return fail(SCL_ERR_ARG, "null argument");
+    const scl_geometry* g = f->g;
+    const size_t ns = (size_t)g->nx * g->ny;
+    if (which == SCL_F) {
+        if (n != 9 * ns) return fail(SCL_ERR_ARG, "populations need %zu values, got %zu", 9 * ns, n);
+        if (f->precision == SCL_FP64 && f->phase == PH_S) {
+            CHK(copy_planes(g, f->f[f->cur], dst, 9, false));
+        } else {
+            CHK(ensure_stage(f));
+            const Grid2 gr = g->grid();
+            const bool gth = f->phase == PH_P;
+            const void* s = f->f[f->cur];
+            switch (f->precision) {
+                case SCL_FP64:
+                    hipLaunchKernelGGL((k_to_f64<StoreF64, true>), site_grid(g), dim3(256), 0, g_lib.stream, gr,
+                                       (const double*)s, 0.0, f->stage);
+                    break;
+                case SCL_FP32_DEV:
+                    if (gth)
+                        hipLaunchKernelGGL((k_to_f64<StoreF32, true>), site_grid(g), dim3(256), 0, g_lib.stream, gr,
+                                           (const float*)s, f->rho0, f->stage);
+                    else
+                        hipLaunchKernelGGL((k_to_f64<StoreF32, false>), site_grid(g), dim3(256), 0, g_lib.stream, gr,
+                                           (const float*)s, f->rho0, f->stage);
+                    break;
+                default:
+                    if (gth)
+                        hipLaunchKernelGGL((k_to_f64<StoreF16, true>), site_grid(g), dim3(256), 0, g_lib.stream, gr,
+                                           (const _Float16*)s, f->rho0, f->stage);
+                    else
+                        hipLaunchKernelGGL((k_to_f64<StoreF16, false>), site_grid(g), dim3(256), 0, g_lib.stream, gr,
+                                           (const _Float16*)s, f->rho0, f->stage);
+                    break;
+            }
+            HIPCHK(hipGetLastError());
+            CHK(copy_planes(g, f->stage, dst, 9, false));
+        }
+        HIPCHK(hipStreamSynchronize(g_lib.stream));
+        return SCL_OK;
+    }
+    if (which < SCL_FEQ || which > SCL_FSY) return fail(SCL_ERR_ARG, "bad field selector %d", which);
+    const size_t want = which == SCL_FEQ ? 9 * ns : ns;
+    if (n != want) return fail(SCL_ERR_ARG, "field %d needs %zu values, got %zu", which, want, n);
+    if (which != SCL_UEQX && which != SCL_UEQY) {
+        if (f->precision != SCL_FP64) return fail(SCL_ERR_UNSUPPORTED, "field %d needs an SCL_FP64 field", which);
+        CHK(ensure_aux(f));
+    }
+    CHK(materialize(f));
+    CHK(copy_planes(g, aux_ptr(f, which), dst, which == SCL_FEQ ? 9 : 1, false));
+    HIPCHK(hipStreamSynchronize(g_lib.stream));
+    return SCL_OK;
+}
+
+// ---- per-operator entry points -------------------------------------------
+
+int scl_compute_density(scl_field* f) {
+    CHK(prepare_op(f));
+    hipLaunchKernelGGL(k_density, site_grid(f->g), dim3(256), 0, g_lib.stream, f->g->grid(),
+                       (const double*)f->f[f->cur], f->rho);
+    HIPCHK(hipGetLastError());
+    return SCL_OK;
+}
+
+int scl_compute_velocities(const scl_geometry* g, scl_field* f) {
+    CHK(check_geom(g, f));
+    CHK(prepare_op(f));
+    hipLaunchKernelGGL(k_velocities, site_grid(g), dim3(256), 0, g_lib.stream, g->grid(), (const double*)f->f[f->cur],
+                       f->rho, f->ux, f->uy);
+    HIPCHK(hipGetLastError());
+    return SCL_OK;
+}
+
+int scl_compute_f_equilibrium_geom(const scl_geometry* g, scl_field* f) {
+    CHK(scl_compute_density(f));
+    CHK(scl_compute_velocities(g, f));
+    hipLaunchKernelGGL(k_feq, site_grid(g), dim3(256), 0, g_lib.stream, g->grid(), f->rho, f->ux, f->uy, f->feq);
+    HIPCHK(hipGetLastError());
+    return SCL_OK;
+}
+
+int scl_compute_f_equilibrium(scl_field* f) {
+    CHK(prepare_op(f));
+    hipLaunchKernelGGL(k_feq, site_grid(f->g), dim3(256), 0, g_lib.stream, f->g->grid(), f->rho, f->ueqx, f->ueqy,
+                       f->feq);
+    HIPCHK(hipGetLastError());
+    return SCL_OK;
+}
+
+int scl_solid_surface_force(const scl_geometry* g, scl_field* a, scl_field* b) {
+    CHK(check_geom(g, a));
+    CHK(check_geom(g, b));
+    for (scl_field* f : {a, b}) {
+        if (f->has_solid_force) continue;   // the reference appends; the first N entries stay in use
+        f->gsneg_captured = -1.0 * f->g_solid;
+        f->has_solid_force = true;
+        if (f->precision != SCL_FP64) continue;   // deviation forms recompute it from the mask
+        CHK(ensure_aux(f));
+        CHK(materialize(f));
+        hipLaunchKernelGGL(k_solid_force, site_grid(g), dim3(256), 0, g_lib.stream, g->grid(), f->gsneg_captured,
+                           f->fsx, f->fsy);
+        HIPCHK(hipGetLastError());
+    }
+    return SCL_OK;
+}
+
+int scl_fluid_repulsion(const scl_geometry* g, scl_field* a, scl_field* b) {
+    CHK(check_geom(g, a));
+    CHK(check_geom(g, b));
+    CHK(prepare_op(a));
+    CHK(prepare_op(b));
+    hipLaunchKernelGGL(k_repulsion, site_grid(g), dim3(256), 0, g_lib.stream, g->grid(), a->rho, b->rho,
+                       -1.0 * a->g_fluid, -1.0 * b->g_fluid, a->frx, a->fry, b->frx, b->fry);
+    HIPCHK(hipGetLastError());
+    return SCL_OK;
+}
+
+int scl_equilibrium_velocities(const scl_geometry* g, scl_field* a, scl_field* b) {
+    CHK(check_geom(g, a));
+    CHK(check_geom(g, b));
+    if (!a->has_solid_force || !b->has_solid_force)
+        return fail(SCL_ERR_RANGE, "fluid-solid force not computed (compute_solid_surface_force first)");
+    CHK(prepare_op(a));
+    CHK(prepare_op(b));
+    EqVelComp A{(const double*)a->f[a->cur], a->rho, a->omega, a->inv_omega, a->frx, a->fry, a->fsx, a->fsy,
+                a->ux, a->uy, a->ueqx, a->ueqy};
+    EqVelComp B{(const double*)b->f[b->cur], b->rho, b->omega, b->inv_omega, b->frx, b->fry, b->fsx, b->fsy,
+                b->ux, b->uy, b->ueqx, b->ueqy};
+    hipLaunchKernelGGL(k_eq_velocities, site_grid(g), dim3(256), 0, g_lib.stream, g->grid(), A, B);
+    HIPCHK(hipGetLastError());
+    return SCL_OK;
+}
+
+int scl_collide1(const scl_geometry* g, scl_field* f) {
+    CHK(check_geom(g, f));
+    CHK(scl_compute_f_equilibrium_geom(g, f));
+    hipLaunchKernelGGL(k_bgk, site_grid(g), dim3(256), 0, g_lib.stream, g->grid(), (double*)f->f[f->cur], f->feq,
+                       f->omc, f->omega);
+    HIPCHK(hipGetLastError());
+    return SCL_OK;
+}
+
+int scl_collide2(scl_field* a, scl_field* b) {
+    if (!a || !b) return fail(SCL_ERR_ARG, "null field");
+    CHK(check_geom(a->g, b));
+    CHK(scl_compute_f_equilibrium(a));
+    CHK(scl_compute_f_equilibrium(b));
+    for (scl_field* f : {a, b}) {
+        hipLaunchKernelGGL(k_bgk, site_grid(f->g), dim3(256), 0, g_lib.stream, f->g->grid(), (double*)f->f[f->cur],
+                           f->feq, f->omc, f->omega);
+        HIPCHK(hipGetLastError());
+    }
+    return SCL_OK;
+}
+
+int scl_volume_force(const scl_geometry* g, scl_field* f, const double force[9]) {
+    CHK(check_geom(g, f));
+    if (!force) return fail(SCL_ERR_ARG, "null force");
+    CHK(prepare_op(f));
+    Force9 F;
+    for (int i = 0; i < Q; ++i) F.v[i] = force[i];
+    hipLaunchKernelGGL(k_volume_force, site_grid(g), dim3(256), 0, g_lib.stream, g->grid(), (double*)f->f[f->cur], F);
+    HIPCHK(hipGetLastError());
+    return SCL_OK;
+}
+
+static int stream_one(const scl_geometry* g, scl_field* f) {
+    CHK(prepare_op(f));
+    hipLaunchKernelGGL(k_gather<StoreF64>, site_grid(g), dim3(256), 0, g_lib.stream, g->grid(),
+                       (const double*)f->f[f->cur], (double*)f->f[f->cur ^ 1]);
+    HIPCHK(hipGetLastError());
+    f->cur ^= 1;
+    return SCL_OK;
+}
+
+int scl_stream1(const scl_geometry* g, scl_field* f) {
+    CHK(check_geom(g, f));
+    return stream_one(g, f);
+}
+
+int scl_stream2(const scl_geometry* g, scl_field* a, scl_field* b) {
+    CHK(check_geom(g, a));
+    CHK(check_geom(g, b));
+    CHK(stream_one(g, a));
+    return stream_one(g, b);
+}
+
+// ---- fused steps ----------------------------------------------------------
+
+int scl_step1(const scl_geometry* g, scl_field* f, const double* force, int nsteps) {
+    if (!g || !f) return fail(SCL_ERR_ARG, "null argument");
+    scl_field* fl[1] = {f};
+    const double* forces[1] = {force};
+    return fused_steps<1>(g, fl, forces, nsteps);
+}
+
+int scl_step2(const scl_geometry* g, scl_field* a, scl_field* b, const double* force_a, const double* force_b,
+              int nsteps) {
+    if (!g || !a || !b) return fail(SCL_ERR_ARG, "null argument");
+    scl_field* fl[2] = {a, b};
+    const double* forces[2] = {force_a, force_b};
+    return fused_steps<2>(g, fl, forces, nsteps);
+}
+
+// ---- diagnostics ----------------------------------------------------------
+
+int scl_moments(const scl_geometry* g, scl_field* f, double out[3]) {
+    CHK(check_geom(g, f));
+    if (!out) return fail(SCL_ERR_ARG, "null output");
+    CHK(ensure_stage(f));
+    const Grid2 gr = g->grid();
+    const void* s = f->f[f->cur];
+    const bool gth = f->phase == PH_P;
+    switch (f->precision) {
+        case SCL_FP64:
+            if (gth) hipLaunchKernelGGL((k_to_f64<StoreF64, true>), site_grid(g), dim3(256), 0, g_lib.stream, gr, (const double*)s, 0.0, f->stage);
+            else hipLaunchKernelGGL((k_to_f64<StoreF64, false>), site_grid(g), dim3(256), 0, g_lib.stream, gr, (const double*)s, 0.0, f->stage);
+            break;
+        case SCL_FP32_DEV:
+            if (gth) hipLaunchKernelGGL((k_to_f64<StoreF32, true>), site_grid(g), dim3(256), 0, g_lib.stream, gr, (const float*)s, f->rho0, f->stage);
+            else hipLaunchKernelGGL((k_to_f64<StoreF32, false>), site_grid(g), dim3(256), 0, g_lib.stream, gr, (const float*)s, f->rho0, f->stage);
+            break;
+        default:
+            if (gth) hipLaunchKernelGGL((k_to_f64<StoreF16, true>), site_grid(g), dim3(256), 0, g_lib.stream, gr, (const _Float16*)s, f->rho0, f->stage);
+            else hipLaunchKernelGGL((k_to_f64<StoreF16, false>), site_grid(g), dim3(256), 0, g_lib.stream, gr, (const _Float16*)s, f->rho0, f->stage);
+            break;
+    }
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_moments_rows, dim3(g->ny), dim3(256), 0, g_lib.stream, gr, (const double*)f->stage, g->part);
+    hipLaunchKernelGGL(k_moments_final, dim3(1), dim3(256), 0, g_lib.stream, g->ny, (const double*)g->part,
+                       g->part + 3 * (size_t)g->ny);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, g->part + 3 * (size_t)g->ny, 3 * sizeof(double), hipMemcpyDeviceToHost, g_lib.stream));
+    HIPCHK(hipStreamSynchronize(g_lib.stream));
+    return SCL_OK;
+}
+
+int scl_timing(int reset, double* avg_ms, int* count) {
+    CHK(lib_ready());
+    if (reset) {
+        HIPCHK(hipStreamSynchronize(g_lib.stream));
+        for (auto& p : g_lib.ev) {
+            (void)hipEventDestroy(p.first);
+            (void)hipEventDestroy(p.second);
+        }
+        g_lib.ev.clear();
+        g_lib.timing = reset > 0;
+        if (avg_ms) *avg_ms = 0.0;
+        if (count) *count = 0;
+        return SCL_OK;
+    }
+    HIPCHK(hipStreamSynchronize(g_lib.stream));
+    double tot = 0.0;
+    for (auto& p : g_lib.ev) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, p.first, p.second));
+        tot += ms;
+    }
+    if (avg_ms) *avg_ms = g_lib.ev.empty() ? 0.0 : tot / g_lib.ev.size();
+    if (count) *count = (int)g_lib.ev.size();
+    return SCL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,280 @@
+"""ctypes front end to libsclbm.so (the C-ABI of include/sc_lbm_amd.h).
+
+This is the path every test, smoke check and benchmark calls.  It loads the
+in-tree library and raises if it is missing: there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsclbm.so")
+
+SCL_OK, SCL_ERR_ARG, SCL_ERR_HIP, SCL_ERR_RANGE, SCL_ERR_UNSUPPORTED = 0, 1, 2, 3, 4
+FP64, FP32_DEV, FP16_DEV = 0, 1, 2
+F, FEQ, RHO, UX, UY, UEQX, UEQY, FRX, FRY, FSX, FSY = range(11)
+PRECISION = {"fp64": FP64, "fp32dev": FP32_DEV, "fp16dev": FP16_DEV}
+
+# every entry point declared in include/sc_lbm_amd.h
+EXPORTS = [
+    "scl_last_error", "scl_init", "scl_synchronize", "scl_geometry_create", "scl_geometry_destroy",
+    "scl_field_create", "scl_field_destroy", "scl_field_set_interactions", "scl_field_upload",
+    "scl_field_download", "scl_compute_density", "scl_compute_velocities",
+    "scl_compute_f_equilibrium_geom", "scl_compute_f_equilibrium", "scl_solid_surface_force",
+    "scl_fluid_repulsion", "scl_equilibrium_velocities", "scl_collide1", "scl_collide2",
+    "scl_volume_force", "scl_stream1", "scl_stream2", "scl_step1", "scl_step2", "scl_moments",
+    "scl_set_tile", "scl_get_tile", "scl_timing",
+]
+
+
+class SclError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("sclbm error %d: %s" % (code, msg))
+        self.code = code
+
+
+class RangeError(SclError, IndexError):
+    """The reference would throw std::out_of_range here."""
+
+
+_lib = None
+_vp = ctypes.c_void_p
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("libsclbm.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+    lib = ctypes.CDLL(LIB_PATH)
+    lib.scl_last_error.restype = ctypes.c_char_p
+    sig = {
+        "scl_init": [ctypes.c_int],
+        "scl_synchronize": [],
+        "scl_geometry_create": [ctypes.c_int, ctypes.c_int, _ip, ctypes.POINTER(_vp)],
+        "scl_geometry_destroy": [_vp],
+        "scl_field_create": [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.POINTER(_vp)],
+        "scl_field_destroy": [_vp],
+        "scl_field_set_interactions": [_vp, ctypes.c_double, ctypes.c_double],
+        "scl_field_upload": [_vp, ctypes.c_int, _dp, ctypes.c_size_t],
+        "scl_field_download": [_vp, ctypes.c_int, _dp, ctypes.c_size_t],
+        "scl_compute_density": [_vp],
+        "scl_compute_velocities": [_vp, _vp],
+        "scl_compute_f_equilibrium_geom": [_vp, _vp],
+        "scl_compute_f_equilibrium": [_vp],
+        "scl_solid_surface_force": [_vp, _vp, _vp],
+        "scl_fluid_repulsion": [_vp, _vp, _vp],
+        "scl_equilibrium_velocities": [_vp, _vp, _vp],
+        "scl_collide1": [_vp, _vp],
+        "scl_collide2": [_vp, _vp],
+        "scl_volume_force": [_vp, _vp, _dp],
+        "scl_stream1": [_vp, _vp],
+        "scl_stream2": [_vp, _vp, _vp],
+        "scl_step1": [_vp, _vp, _dp, ctypes.c_int],
+        "scl_step2": [_vp, _vp, _vp, _dp, _dp, ctypes.c_int],
+        "scl_moments": [_vp, _vp, _dp],
+        "scl_set_tile": [_vp, ctypes.c_int, ctypes.c_int],
+        "scl_get_tile": [_vp, _ip, _ip],
+        "scl_timing": [ctypes.c_int, _dp, _ip],
+    }
+    for name, args in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_int
+    _lib = lib
+    return lib
+
+
+def check(rc):
+    if rc != SCL_OK:
+        msg = load().scl_last_error().decode(errors="replace")
+        if rc == SCL_ERR_RANGE:
+            raise RangeError(rc, msg)
+        raise SclError(rc, msg)
+
+
+def _arr(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _force(v):
+    if v is None:
+        return None
+    a = _arr(v)
+    assert a.shape == (9,)
+    return a
+
+
+def init(device=0):
+    check(load().scl_init(device))
+
+
+def synchronize():
+    check(load().scl_synchronize())
+
+
+class Geometry:
+    def __init__(self, mask):
+        lib = load()
+        m = np.ascontiguousarray(mask, dtype=np.int32)
+        assert m.ndim == 2
+        self.ny, self.nx = m.shape
+        self.mask = m
+        h = _vp()
+        check(lib.scl_geometry_create(self.nx, self.ny, m.ctypes.data_as(_ip), ctypes.byref(h)))
+        self.h = h
+
+    def set_tile(self, cols, rows):
+        check(load().scl_set_tile(self.h, cols, rows))
+
+    def tile(self):
+        c, r = ctypes.c_int(), ctypes.c_int()
+        check(load().scl_get_tile(self.h, ctypes.byref(c), ctypes.byref(r)))
+        return c.value, r.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().scl_geometry_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Field:
+    def __init__(self, geom, tau, precision=FP64, rho0=1.0, g_solid=0.0, g_fluid=0.0):
+        lib = load()
+        if isinstance(precision, str):
+            precision = PRECISION[precision]
+        if getattr(geom, "h", None) is None:
+            raise SclError(SCL_ERR_ARG, "geometry is closed")
+        self.geom = geom
+        self.precision = precision
+        h = _vp()
+        check(lib.scl_field_create(geom.h, precision, float(tau), float(rho0), ctypes.byref(h)))
+        self.h = h
+        check(lib.scl_field_set_interactions(h, float(g_solid), float(g_fluid)))
+        self.n = geom.nx * geom.ny
+
+    def set_interactions(self, g_solid, g_fluid):
+        check(load().scl_field_set_interactions(self.h, float(g_solid), float(g_fluid)))
+
+    def _size(self, which):
+        return 9 * self.n if which in (F, FEQ) else self.n
+
+    def upload(self, which, values):
+        a = _arr(values).ravel()
+        check(load().scl_field_upload(self.h, which, a.ctypes.data_as(_dp), a.size))
+
+    def download(self, which=F):
+        out = np.empty(self._size(which), dtype=np.float64)
+        check(load().scl_field_download(self.h, which, out.ctypes.data_as(_dp), out.size))
+        return out
+
+    def set_f(self, f):
+        self.upload(F, f)
+
+    def get_f(self):
+        return self.download(F)
+
+    def moments(self):
+        out = np.zeros(3)
+        check(load().scl_moments(self.geom.h, self.h, out.ctypes.data_as(_dp)))
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().scl_field_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _fp(a):
+    return a.ctypes.data_as(_dp) if a is not None else None
+
+
+def step1(geom, f, force=None, nsteps=1):
+    fa = _force(force)
+    check(load().scl_step1(geom.h, f.h, _fp(fa), int(nsteps)))
+
+
+def step2(geom, a, b, force_a=None, force_b=None, nsteps=1):
+    fa, fb = _force(force_a), _force(force_b)
+    check(load().scl_step2(geom.h, a.h, b.h, _fp(fa), _fp(fb), int(nsteps)))
+
+
+def solid_surface_force(geom, a, b):
+    check(load().scl_solid_surface_force(geom.h, a.h, b.h))
+
+
+# per-operator API (reference method names)
+def compute_density(f):
+    check(load().scl_compute_density(f.h))
+
+
+def compute_velocities(geom, f):
+    check(load().scl_compute_velocities(geom.h, f.h))
+
+
+def compute_f_equilibrium_geom(geom, f):
+    check(load().scl_compute_f_equilibrium_geom(geom.h, f.h))
+
+
+def compute_f_equilibrium(f):
+    check(load().scl_compute_f_equilibrium(f.h))
+
+
+def fluid_repulsion(geom, a, b):
+    check(load().scl_fluid_repulsion(geom.h, a.h, b.h))
+
+
+def equilibrium_velocities(geom, a, b):
+    check(load().scl_equilibrium_velocities(geom.h, a.h, b.h))
+
+
+def collide1(geom, f):
+    check(load().scl_collide1(geom.h, f.h))
+
+
+def collide2(a, b):
+    check(load().scl_collide2(a.h, b.h))
+
+
+def volume_force(geom, f, force):
+    fa = _force(force)
+    check(load().scl_volume_force(geom.h, f.h, _fp(fa)))
+
+
+def stream1(geom, f):
+    check(load().scl_stream1(geom.h, f.h))
+
+
+def stream2(geom, a, b):
+    check(load().scl_stream2(geom.h, a.h, b.h))
+
+
+def timing(reset=0):
+    avg, cnt = ctypes.c_double(), ctypes.c_int()
+    check(load().scl_timing(int(reset), ctypes.byref(avg), ctypes.byref(cnt)))
+    return avg.value, cnt.value
+
+
+def gpu_available():
+    """True when a gfx950 device answers scl_init (never raises)."""
+    try:
+        lib = load()
+    except ImportError:
+        return False
+    return lib.scl_init(0) == SCL_OK
