@@ -33,7 +33,7 @@ def parse():
     p.add_argument("--forms", default="fp32dev,fp64,fp16dev")
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU baseline sample")
     p.add_argument("--no-cpu", action="store_true")
-    p.add_argument("--tile", default="", help="cols,rows override for the fused kernel")
+    p.add_argument("--tile", default="", help="cols,rows[,lookahead] override (default: autotuned)")
     return p.parse_args()
 
 
@@ -43,10 +43,14 @@ def log(msg):
 
 def time_form(nat, wl, case, form, steps, warmup, tile):
     g = nat.Geometry(case["mask"])
-    if tile:
-        c, r = [int(v) for v in tile.split(",")]
-        g.set_tile(c, r)
     prec = nat.PRECISION[form]
+    if tile:
+        v = [int(t) for t in tile.split(",")]
+        g.set_tile(v[0], v[1])
+        if len(v) > 2:
+            g.set_lookahead(v[2])
+    else:
+        g.autotune(2, prec)          # launch shape chosen by measurement, before the timed region
     fl = []
     for c in range(2):
         f = nat.Field(g, case["tau"][c], prec, 1.0, g_solid=0.0, g_fluid=case["g_fluid"][c])
@@ -63,7 +67,7 @@ def time_form(nat, wl, case, form, steps, warmup, tile):
     avg_ms, cnt = nat.timing(0)
     nat.timing(-1)
     mass = [f.moments()[0] for f in fl]
-    tile_used = g.tile()
+    tile_used = g.tile(2, prec)
     for f in fl:
         f.close()
     g.close()

@@ -95,7 +95,11 @@ int scl_field_destroy(scl_field* f);
 int scl_field_set_interactions(scl_field* f, double g_solid, double g_fluid);
 
 /* Copy a field in / out; n must equal its size.  Populations are converted
- * to / from the storage form; the other arrays are FP64 and need SCL_FP64. */
+ * to / from the storage form; the other arrays are FP64 and need SCL_FP64.
+ * Uploading SCL_FSX / SCL_FSY sets the fluid-solid force from the host (the
+ * reference's add_surface_forces); scl_step2 then refuses the field
+ * (SCL_ERR_UNSUPPORTED) because the fused kernel derives that force from
+ * the mask, and the per-operator entry points must be used. */
 int scl_field_upload(scl_field* f, int which, const double* src, size_t n);
 int scl_field_download(scl_field* f, int which, double* dst, size_t n);
 
@@ -155,11 +159,31 @@ int scl_moments(const scl_geometry* g, scl_field* f, double out[3]);
  * (64, 128, 256 or 512) and rows per workgroup (>= 1).  0 = automatic.
  * Results do not depend on the shape (per-site arithmetic is fixed). */
 int scl_set_tile(scl_geometry* g, int cols, int rows);
-int scl_get_tile(const scl_geometry* g, int* cols, int* rows);
+/* Rows of populations in flight in the fused kernel's software pipeline
+ * (1 or 2; 0 = automatic: 1 for FP64, 2 for the deviation forms).  Results
+ * do not depend on it. */
+int scl_set_lookahead(scl_geometry* g, int rows_ahead);
+/* The shape the fused kernel uses for `ncomp` components of `precision` on
+ * g: the caller's overrides, else the autotuned one, else the default. */
+int scl_get_tile(const scl_geometry* g, int ncomp, int precision, int* cols, int* rows, int* lookahead);
 
-/* Average duration in milliseconds of the fused kernel launches since the
- * last reset, measured with HIP events on the library's stream. */
+/* Autotuner (no reference counterpart): times the fused step of `ncomp`
+ * components of `precision` on g for every launch shape (columns 64..512,
+ * rows 4..128, lookahead 1..2) on scratch fields at rest, keeps the fastest
+ * for later steps on g and returns it (outputs may be NULL).  Takes about
+ * 3 ms of GPU time per shape; needs memory for two scratch fields. */
+int scl_autotune(scl_geometry* g, int ncomp, int precision, int* cols, int* rows, int* lookahead);
+
+/* Average duration in milliseconds of the fused kernel launches, measured
+ * with HIP events on the library's stream.  reset > 0 clears the record and
+ * starts recording, reset < 0 clears it and stops, reset == 0 reads it. */
 int scl_timing(int reset, double* avg_ms, int* count);
+
+/* Bandwidth probe (no reference counterpart): a plain device copy of
+ * `bytes` bytes (kernel k_probe_copy, 16-byte vector loads and stores),
+ * repeated `reps` times; avg_ms is the mean duration of one copy.  Used to
+ * calibrate the HBM byte counters against a kernel of known traffic. */
+int scl_probe_copy(size_t bytes, int reps, double* avg_ms);
 
 #ifdef __cplusplus
 }

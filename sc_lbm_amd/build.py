@@ -28,11 +28,26 @@ def _run(cmd, cwd=None):
     return p.stdout
 
 
+HOST = os.path.join(HERE, "host")
+INCLUDE = os.path.join(ROOT, "include")
+CXX_FLAGS = ["-std=c++14", "-O2", "-fPIC", "-Wall", "-I" + INCLUDE, "-I" + HOST]
+
+
 def build_library():
+    """libsclbm.so = the HIP kernels and C-ABI (hipcc) + the C++ drop-in
+    classes Geometry / Fluid / LBM (g++), linked into one shared library."""
+    bdir = os.path.join(HERE, "_build")
+    os.makedirs(bdir, exist_ok=True)
+    objs = [os.path.join(bdir, "sclbm.o")]
+    _run([HIPCC] + HIP_FLAGS + ["-c", "-o", objs[0], os.path.join(CSRC, "sclbm.hip")])
+    for src in sorted(os.listdir(HOST)):
+        if src.endswith(".cpp"):
+            o = os.path.join(bdir, src[:-4] + ".o")
+            _run(["g++"] + CXX_FLAGS + ["-c", "-o", o, os.path.join(HOST, src)])
+            objs.append(o)
     out = os.path.join(HERE, "libsclbm.so")
     tmp = out + ".tmp"
-    srcs = [os.path.join(CSRC, "sclbm.hip")]
-    _run([HIPCC] + HIP_FLAGS + ["-shared", "-o", tmp] + srcs)
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, out)
     return out
 

@@ -306,4 +306,24 @@ __global__ void k_moments_final(int rows, const double* __restrict__ part, doubl
     }
 }
 
+// Populations of a fluid at rest with density rho everywhere (f_i = w_i*rho),
+// in the storage form; used for the autotuner's scratch fields.
+template <class ST>
+__global__ void k_fill_rest(Grid2 g, typename ST::S* __restrict__ dst, double rho, double rho0) {
+    int x, y, s;
+    if (!site_of(g, x, y, s)) return;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        const double v = weq(i) * rho - (ST::kDev ? weq(i) * rho0 : 0.0);
+        ST::st(dst + i * g.plane + s, (typename ST::T)v);
+    }
+}
+
+// Plain copy used to calibrate the HBM byte counters (16 B per lane).
+__global__ void k_probe_copy(const float4* __restrict__ src, float4* __restrict__ dst, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+
 }  // namespace scl

@@ -103,7 +103,12 @@ struct scl_geometry {
     long long plane = 0;
     uint64_t* mask = nullptr;
     std::vector<int> host_mask;
-    int tile_cols = 0, tile_rows = 0;   // 0 = automatic
+    // Launch shape of the fused kernel: the caller's choice (scl_set_tile /
+    // scl_set_lookahead; 0 = not set), else the autotuner's per component
+    // count and precision (scl_autotune), else the built-in default.
+    int tile_cols = 0, tile_rows = 0;
+    int lookahead = 0;
+    int tuned[2][3][3] = {};            // [ncomp-1][precision] -> {cols, rows, lookahead}
     double* part = nullptr;             // moment partials (3 per row + 3)
     // Fields keep their geometry alive: destroying a geometry that fields
     // still use only marks it, and the last field destroyed frees it.
@@ -124,6 +129,7 @@ struct scl_field {
     double tau = 1.0, omega = 1.0, omc = 0.0, inv_omega = 1.0, rho0 = 0.0;
     double g_solid = 0.0, g_fluid = 0.0;
     bool has_solid_force = false;
+    bool solid_force_uploaded = false;   // F_solid came from the host, not from the mask
     double gsneg_captured = 0.0;
     void* f[2] = {nullptr, nullptr};
     int cur = 0;
@@ -170,37 +176,77 @@ int ensure_stage(scl_field* f) {
 dim3 site_grid(const scl_geometry* g) { return dim3((g->nx + 255) / 256, g->ny); }
 
 // ---- fused-kernel dispatch ------------------------------------------------
-void auto_tile(const scl_geometry* g, int& nw, int& rows) {
-    nw = g->tile_cols ? g->tile_cols / 64 : 0;
-    rows = g->tile_rows;
-    if (!nw) {
-        nw = (g->nx + 63) / 64;
-        if (nw > 4) nw = 4;
-        if (nw == 3) nw = 4;
-    }
-    if (!rows) {
-        const int nbx = (g->nx + 64 * nw - 1) / (64 * nw);
-        long long r = (long long)nbx * g->ny / 1024;
-        rows = (int)(r < 4 ? 4 : (r > 64 ? 64 : r));
-        if (rows > g->ny) rows = g->ny;
-    }
+struct Shape {
+    int nw, rows, la;
+};
+
+// Default shape, from measurements on MI355X at 4096^2: 256-column strips
+// (4 waves), about 2048 workgroups (8 per CU), one row in flight for the FP64
+// and FP32 forms and two for the FP16 form (its register sets are smaller
+// than its loads' latency budget).
+Shape default_shape(const scl_geometry* g, int prec) {
+    Shape s;
+    s.nw = (g->nx + 63) / 64;
+    if (s.nw > 4) s.nw = 4;
+    if (s.nw == 3) s.nw = 4;
+    const int nbx = (g->nx + 64 * s.nw - 1) / (64 * s.nw);
+    long long r = (long long)nbx * g->ny / 2048;
+    s.rows = (int)(r < 4 ? 4 : (r > 64 ? 64 : r));
+    if (s.rows > g->ny) s.rows = g->ny;
+    s.la = prec == SCL_FP16_DEV ? 2 : 1;
+    return s;
 }
 
-template <int NC, class ST, int NW, bool GATHER, bool REPLAY>
+Shape resolve_shape(const scl_geometry* g, int nc, int prec) {
+    Shape s = default_shape(g, prec);
+    const int* t = g->tuned[nc - 1][prec];
+    if (t[0]) s.nw = t[0] / 64;
+    if (t[1]) s.rows = t[1];
+    if (t[2]) s.la = t[2];
+    if (g->tile_cols) s.nw = g->tile_cols / 64;
+    if (g->tile_rows) s.rows = g->tile_rows;
+    if (g->lookahead) s.la = g->lookahead;
+    if (s.rows > g->ny) s.rows = g->ny;
+    return s;
+}
+
+int prec_of(size_t esize) { return esize == 8 ? SCL_FP64 : (esize == 4 ? SCL_FP32_DEV : SCL_FP16_DEV); }
+
+// 32-bit byte offsets address one component's population buffer while it
+// is below 4 GiB; larger buffers use the 64-bit variant (4-wave tiles).
+bool needs_wide(const scl_geometry* g, size_t esize) {
+    return (unsigned long long)g->plane * Q * esize >= (1ull << 32);
+}
+
+template <int NC, class ST, int NW, bool GATHER, bool REPLAY, class OffT, int LA>
 void launch_step_t(const StepArgs<typename ST::T>& a, int nblk) {
-    hipLaunchKernelGGL((k_step<NC, ST, NW, GATHER, REPLAY>), dim3(nblk), dim3(64 * NW), 0, g_lib.stream, a);
+    hipLaunchKernelGGL((k_step<NC, ST, NW, GATHER, REPLAY, OffT, LA>), dim3(nblk), dim3(64 * NW), 0, g_lib.stream,
+                       a);
 }
 
-template <int NC, class ST, bool GATHER>
-int launch_nw(int nw, const StepArgs<typename ST::T>& a, int nblk) {
+template <int NC, class ST, bool GATHER, int LA>
+int launch_nw_la(int nw, const StepArgs<typename ST::T>& a, int nblk) {
     switch (nw) {
-        case 1: launch_step_t<NC, ST, 1, GATHER, false>(a, nblk); break;
-        case 2: launch_step_t<NC, ST, 2, GATHER, false>(a, nblk); break;
-        case 4: launch_step_t<NC, ST, 4, GATHER, false>(a, nblk); break;
-        case 8: launch_step_t<NC, ST, 8, GATHER, false>(a, nblk); break;
+        case 1: launch_step_t<NC, ST, 1, GATHER, false, uint32_t, LA>(a, nblk); break;
+        case 2: launch_step_t<NC, ST, 2, GATHER, false, uint32_t, LA>(a, nblk); break;
+        case 4: launch_step_t<NC, ST, 4, GATHER, false, uint32_t, LA>(a, nblk); break;
+        case 8: launch_step_t<NC, ST, 8, GATHER, false, uint32_t, LA>(a, nblk); break;
         default: return fail(SCL_ERR_ARG, "unsupported tile width %d", 64 * nw);
     }
     return SCL_OK;
+}
+
+// The first step after an upload (no gather) runs once, so it has one
+// shape per width; the gathering step also chooses the lookahead.
+template <int NC, class ST, bool GATHER>
+int launch_nw(int nw, int la, bool wide, const StepArgs<typename ST::T>& a, int nblk) {
+    if (wide) {
+        if (nw != 4) return fail(SCL_ERR_ARG, "lattices above 4 GiB per component use 256-column tiles");
+        launch_step_t<NC, ST, 4, GATHER, false, uint64_t, 1>(a, nblk);
+        return SCL_OK;
+    }
+    if (GATHER && la == 2) return launch_nw_la<NC, ST, GATHER, 2>(nw, a, nblk);
+    return launch_nw_la<NC, ST, GATHER, 1>(nw, a, nblk);
 }
 
 template <class T>
@@ -249,18 +295,25 @@ template <int NC, class ST>
 int launch_fused(const scl_geometry* g, scl_field* const* fl, const double* const* forces, bool gather,
                  std::shared_ptr<Replay>* rec) {
     using T = typename ST::T;
-    int nw, rows;
-    auto_tile(g, nw, rows);
+    const Shape sh = resolve_shape(g, NC, prec_of(sizeof(typename ST::S)));
+    int nw = sh.nw, rows = sh.rows;
     StepArgs<T> a = make_args<T>(g, rows, nw);
     for (int c = 0; c < NC; ++c) fill_comp(a.c[c], fl[c], fl[c]->cur, forces[c]);
-    const int nblk = nblocks(g, nw, rows);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (g_lib.timing && g_lib.ev.size() < kMaxTimed) {
         HIPCHK(hipEventCreate(&e0));
         HIPCHK(hipEventCreate(&e1));
         HIPCHK(hipEventRecord(e0, g_lib.stream));
     }
-    int rc = gather ? launch_nw<NC, ST, true>(nw, a, nblk) : launch_nw<NC, ST, false>(nw, a, nblk);
+    const bool wide = needs_wide(g, sizeof(typename ST::S));
+    if (wide && nw != 4) {
+        nw = 4;
+        a = make_args<T>(g, rows, nw);
+        for (int c = 0; c < NC; ++c) fill_comp(a.c[c], fl[c], fl[c]->cur, forces[c]);
+    }
+    const int nblk = nblocks(g, nw, rows);
+    const int la = sh.la;
+    int rc = gather ? launch_nw<NC, ST, true>(nw, la, wide, a, nblk) : launch_nw<NC, ST, false>(nw, la, wide, a, nblk);
     if (rc != SCL_OK) return rc;
     HIPCHK(hipGetLastError());
     if (e0) {
@@ -284,15 +337,26 @@ int launch_fused(const scl_geometry* g, scl_field* const* fl, const double* cons
     return SCL_OK;
 }
 
+template <int NC, bool GATHER>
+void replay_t(const Replay& r, int nblk, bool wide) {
+    if (wide)
+        hipLaunchKernelGGL((k_step<NC, StoreF64, 4, GATHER, true, uint64_t, 1>), dim3(nblk), dim3(256), 0, g_lib.stream,
+                           r.args);
+    else
+        hipLaunchKernelGGL((k_step<NC, StoreF64, 4, GATHER, true, uint32_t, 1>), dim3(nblk), dim3(256), 0, g_lib.stream,
+                           r.args);
+}
+
 int run_replay(Replay& r) {
     const scl_geometry* g = r.fl[0]->g;
     const int nblk = nblocks(g, 4, r.args.rows);
+    const bool wide = needs_wide(g, sizeof(double));
     if (r.nc == 1) {
-        if (r.gather) hipLaunchKernelGGL((k_step<1, StoreF64, 4, true, true>), dim3(nblk), dim3(256), 0, g_lib.stream, r.args);
-        else hipLaunchKernelGGL((k_step<1, StoreF64, 4, false, true>), dim3(nblk), dim3(256), 0, g_lib.stream, r.args);
+        if (r.gather) replay_t<1, true>(r, nblk, wide);
+        else replay_t<1, false>(r, nblk, wide);
     } else {
-        if (r.gather) hipLaunchKernelGGL((k_step<2, StoreF64, 4, true, true>), dim3(nblk), dim3(256), 0, g_lib.stream, r.args);
-        else hipLaunchKernelGGL((k_step<2, StoreF64, 4, false, true>), dim3(nblk), dim3(256), 0, g_lib.stream, r.args);
+        if (r.gather) replay_t<2, true>(r, nblk, wide);
+        else replay_t<2, false>(r, nblk, wide);
     }
     HIPCHK(hipGetLastError());
     return SCL_OK;
@@ -386,6 +450,10 @@ int fused_steps(const scl_geometry* g, scl_field** fl, const double* const* forc
         if (fl[0] == fl[1]) return fail(SCL_ERR_ARG, "the two components must be different fields");
         if (!fl[0]->has_solid_force || !fl[1]->has_solid_force)
             return fail(SCL_ERR_RANGE, "fluid-solid force not computed (compute_solid_surface_force first)");
+        if (fl[0]->solid_force_uploaded || fl[1]->solid_force_uploaded)
+            return fail(SCL_ERR_UNSUPPORTED,
+                        "fluid-solid force was uploaded from the host; the fused step derives it from the mask "
+                        "(use the per-operator entry points)");
     }
     if (nsteps <= 0) return SCL_OK;
     CHK(settle_replays(fl, NC));
@@ -487,12 +555,21 @@ int scl_set_tile(scl_geometry* g, int cols, int rows) {
     return SCL_OK;
 }
 
-int scl_get_tile(const scl_geometry* g, int* cols, int* rows) {
-    if (!g || !cols || !rows) return fail(SCL_ERR_ARG, "null argument");
-    int nw, r;
-    auto_tile(g, nw, r);
-    *cols = 64 * nw;
-    *rows = r;
+int scl_set_lookahead(scl_geometry* g, int rows_ahead) {
+    if (!g) return fail(SCL_ERR_ARG, "null geometry");
+    if (rows_ahead < 0 || rows_ahead > 2) return fail(SCL_ERR_ARG, "lookahead must be 0 (automatic), 1 or 2");
+    g->lookahead = rows_ahead;
+    return SCL_OK;
+}
+
+int scl_get_tile(const scl_geometry* g, int ncomp, int precision, int* cols, int* rows, int* lookahead) {
+    if (!g || !cols || !rows || !lookahead) return fail(SCL_ERR_ARG, "null argument");
+    if (ncomp < 1 || ncomp > 2 || precision < SCL_FP64 || precision > SCL_FP16_DEV)
+        return fail(SCL_ERR_ARG, "bad component count or precision");
+    const Shape s = resolve_shape(g, ncomp, precision);
+    *cols = 64 * s.nw;
+    *rows = s.rows;
+    *lookahead = s.la;
     return SCL_OK;
 }
 
@@ -631,6 +708,11 @@ int scl_field_upload(scl_field* f, int which, const double* src, size_t n) {
     }
     CHK(materialize(f));
     CHK(copy_planes(g, aux_ptr(f, which), src, which == SCL_FEQ ? 9 : 1, true));
+    if (which == SCL_FSX || which == SCL_FSY) {
+        // the reference's first F_solid entries are in use from now on
+        f->has_solid_force = true;
+        f->solid_force_uploaded = true;
+    }
     HIPCHK(hipStreamSynchronize(g_lib.stream));
     return SCL_OK;
 }
@@ -895,6 +977,137 @@ int scl_timing(int reset, double* avg_ms, int* count) {
     }
     if (avg_ms) *avg_ms = g_lib.ev.empty() ? 0.0 : tot / g_lib.ev.size();
     if (count) *count = (int)g_lib.ev.size();
+    return SCL_OK;
+}
+
+int scl_autotune(scl_geometry* g, int ncomp, int precision, int* cols, int* rows, int* lookahead) {
+    if (!g) return fail(SCL_ERR_ARG, "null geometry");
+    if (ncomp < 1 || ncomp > 2 || precision < SCL_FP64 || precision > SCL_FP16_DEV)
+        return fail(SCL_ERR_ARG, "bad component count or precision");
+    CHK(lib_ready());
+    static const size_t esz[3] = {8, 4, 2};
+    const bool wide = needs_wide(g, esz[precision]);
+    // scratch fields at rest (their contents do not affect the timing)
+    scl_field* fl[2] = {nullptr, nullptr};
+    int rc = SCL_OK;
+    for (int c = 0; c < ncomp && rc == SCL_OK; ++c) {
+        rc = scl_field_create(g, precision, 1.0, 1.0, &fl[c]);
+        if (rc == SCL_OK) rc = scl_field_set_interactions(fl[c], 0.0, 0.9);
+        if (rc == SCL_OK) {
+            const Grid2 gr = g->grid();
+            void* d = fl[c]->f[fl[c]->cur];
+            if (precision == SCL_FP64)
+                hipLaunchKernelGGL(k_fill_rest<StoreF64>, site_grid(g), dim3(256), 0, g_lib.stream, gr, (double*)d, 1.0, 1.0);
+            else if (precision == SCL_FP32_DEV)
+                hipLaunchKernelGGL(k_fill_rest<StoreF32>, site_grid(g), dim3(256), 0, g_lib.stream, gr, (float*)d, 1.0, 1.0);
+            else
+                hipLaunchKernelGGL(k_fill_rest<StoreF16>, site_grid(g), dim3(256), 0, g_lib.stream, gr, (_Float16*)d, 1.0, 1.0);
+            if (hipGetLastError() != hipSuccess) rc = fail(SCL_ERR_HIP, "fill kernel launch failed");
+        }
+    }
+    if (rc == SCL_OK && ncomp == 2) rc = scl_solid_surface_force(g, fl[0], fl[1]);
+    // candidates (the caller's own overrides are set aside while timing)
+    const int saved[3] = {g->tile_cols, g->tile_rows, g->lookahead};
+    g->tile_cols = g->tile_rows = g->lookahead = 0;
+    int* slot = g->tuned[ncomp - 1][precision];
+    const int saved_tuned[3] = {slot[0], slot[1], slot[2]};
+    double best = 1e300;
+    int bc = 0, br = 0, bl = 0;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (rc == SCL_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+        rc = fail(SCL_ERR_HIP, "event creation failed");
+    const int colset[4] = {64, 128, 256, 512};
+    const int rowset[6] = {4, 8, 16, 32, 64, 128};
+    const int pitch = g->pitch;
+    for (int ci = 0; ci < 4 && rc == SCL_OK; ++ci) {
+        const int c = colset[ci];
+        if (wide && c != 256) continue;
+        if (c > 2 * pitch && c > 64) continue;
+        for (int ri = 0; ri < 6 && rc == SCL_OK; ++ri) {
+            const int r = rowset[ri];
+            if (r > g->ny && ri > 0) continue;
+            for (int la = 1; la <= 2 && rc == SCL_OK; ++la) {
+                if (wide && la != 1) continue;
+                slot[0] = c;
+                slot[1] = r;
+                slot[2] = la;
+                const double* none[2] = {nullptr, nullptr};
+                rc = ncomp == 1 ? fused_steps<1>(g, fl, none, 2) : fused_steps<2>(g, fl, none, 2);
+                if (rc != SCL_OK) break;
+                int reps = 3;
+                float ms = 0.f;
+                for (int pass = 0; pass < 2 && rc == SCL_OK; ++pass) {
+                    if (hipEventRecord(e0, g_lib.stream) != hipSuccess) rc = fail(SCL_ERR_HIP, "event record failed");
+                    if (rc == SCL_OK) rc = ncomp == 1 ? fused_steps<1>(g, fl, none, reps) : fused_steps<2>(g, fl, none, reps);
+                    if (rc == SCL_OK && (hipEventRecord(e1, g_lib.stream) != hipSuccess ||
+                                         hipEventSynchronize(e1) != hipSuccess ||
+                                         hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                        rc = fail(SCL_ERR_HIP, "event timing failed");
+                    if (pass == 0 && rc == SCL_OK) {
+                        const double per = ms / reps;
+                        int want = per > 0 ? (int)(3.0 / per) : 50;   // about 3 ms per candidate
+                        reps = want < 3 ? 3 : (want > 50 ? 50 : want);
+                    }
+                }
+                if (rc == SCL_OK && ms / reps < best) {
+                    best = ms / reps;
+                    bc = c;
+                    br = r;
+                    bl = la;
+                }
+            }
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    g->tile_cols = saved[0];
+    g->tile_rows = saved[1];
+    g->lookahead = saved[2];
+    for (int c = 0; c < ncomp; ++c)
+        if (fl[c]) (void)scl_field_destroy(fl[c]);
+    if (rc != SCL_OK || !bc) {
+        slot[0] = saved_tuned[0];
+        slot[1] = saved_tuned[1];
+        slot[2] = saved_tuned[2];
+        return rc != SCL_OK ? rc : fail(SCL_ERR_ARG, "no launch shape could be timed");
+    }
+    slot[0] = bc;
+    slot[1] = br;
+    slot[2] = bl;
+    if (cols) *cols = bc;
+    if (rows) *rows = br;
+    if (lookahead) *lookahead = bl;
+    return SCL_OK;
+}
+
+int scl_probe_copy(size_t bytes, int reps, double* avg_ms) {
+    if (!avg_ms || bytes < 16 || reps <= 0) return fail(SCL_ERR_ARG, "bad probe arguments");
+    CHK(lib_ready());
+    const size_t n = bytes / 16;
+    float4 *a = nullptr, *b = nullptr;
+    CHK(dalloc(&a, n));
+    int rc = dalloc(&b, n);
+    if (rc != SCL_OK) {
+        dfree(a);
+        return rc;
+    }
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipMemsetAsync(a, 0, n * 16, g_lib.stream));
+    hipLaunchKernelGGL(k_probe_copy, dim3(4096), dim3(256), 0, g_lib.stream, a, b, n);
+    HIPCHK(hipEventRecord(e0, g_lib.stream));
+    for (int r = 0; r < reps; ++r)
+        hipLaunchKernelGGL(k_probe_copy, dim3(4096), dim3(256), 0, g_lib.stream, a, b, n);
+    HIPCHK(hipEventRecord(e1, g_lib.stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    *avg_ms = ms / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    dfree(a);
+    dfree(b);
     return SCL_OK;
 }
 

@@ -11,13 +11,15 @@
 //
 // Work decomposition: a workgroup owns a strip of BW = 64*NW columns and
 // `rows` rows and walks it from south to north.  Each thread owns one
-// column.  Per row it gathers the populations of the row ahead (y+1), sums
-// the densities of both components into a 4-slot LDS ring (with one halo
-// column at each side of the strip, computed by the edge threads), waits at
-// one barrier, and then finishes row y: the 8-neighbour Shan-Chen forces
-// read the three LDS rows, the rest stays in registers.  The halo densities
-// of the strip are recomputed from gathered populations, so no density
-// array crosses HBM.
+// column.  The densities of both components live in a 4-row LDS ring with
+// one halo column at each side of the strip; the halo densities are
+// recomputed from gathered populations, so no density array crosses HBM.
+// Per row y the loop is software-pipelined: the loads of row y+2 are issued
+// first (their addresses come from mask rows fetched one iteration earlier),
+// row y is finished while they are in flight (forces from the three LDS rows
+// y-1..y+1, the rest from registers), and the density of row y+2 enters the
+// ring afterwards.  One barrier per row.  Loads and stores use 32-bit byte
+// offsets from the component's base address, shared by both components.
 //
 // FP64 arithmetic reproduces the reference operation by operation (the file
 // is compiled with -ffp-contract=off): density src/fluid.cpp:177-182,
@@ -62,26 +64,53 @@ struct StepArgs {
     CompArgs<T> c[2];
 };
 
-// Gather (or read in place) the 9 populations of component c at site
-// (x, r); nb holds the fluid bits of the neighbourhood of (x, r).
-template <class ST, bool GATHER>
-__device__ inline void load_site(const typename ST::S* __restrict__ src, long long plane, int pitch, int x,
-                                 int xm, int xp, int r, int rm, int rp, unsigned nb,
-                                 typename ST::T (&p)[Q]) {
-    const int o_r = r * pitch, o_s = rm * pitch, o_n = rp * pitch;
+// Byte offsets of the 9 populations that site (col, r) reads: the pull
+// sources, or its own opposite slot where the source is solid.  The choice
+// is made with a bit mask rather than a branch, so the nine loads of a site
+// issue back to back without divergent control flow between them.
+template <class ST, bool GATHER, class OffT>
+__device__ inline void pull_offsets(OffT plane_b, OffT pitch_b, int col, int colm, int colp, int r, int rm,
+                                    int rp, unsigned nb, OffT (&off)[Q]) {
+    constexpr OffT es = sizeof(typename ST::S);
+    const OffT orr = (OffT)r * pitch_b, os = (OffT)rm * pitch_b, on = (OffT)rp * pitch_b;
+    const OffT cc = (OffT)col * es, cm = (OffT)colm * es, cp = (OffT)colp * es;
+    const OffT home = orr + cc;
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-        const typename ST::S* ptr;
-        if (GATHER) {
-            const int row = cy(i) == 1 ? o_s : (cy(i) == -1 ? o_n : o_r);
-            const int col = cx(i) == 1 ? xm : (cx(i) == -1 ? xp : x);
-            const bool ok = (nb >> opp(i)) & 1u;
-            ptr = ok ? (src + i * plane + row + col) : (src + opp(i) * plane + o_r + x);
+        if (!GATHER || i == 0) {
+            off[i] = (OffT)i * plane_b + home;
         } else {
-            ptr = src + i * plane + o_r + x;
+            const OffT row = cy(i) == 1 ? os : (cy(i) == -1 ? on : orr);
+            const OffT col_b = cx(i) == 1 ? cm : (cx(i) == -1 ? cp : cc);
+            const OffT pull = (OffT)i * plane_b + row + col_b;
+            const OffT back = (OffT)opp(i) * plane_b + home;
+            const OffT keep = (OffT)0 - (OffT)((nb >> opp(i)) & 1u);   // all ones where the source is fluid
+            off[i] = back ^ ((pull ^ back) & keep);
         }
-        p[i] = ST::ld(ptr);
     }
+}
+
+// Element-indexed form of the gather (used by the single-pass helper
+// kernels): populations of site (x, r) with 64-bit offsets.
+template <class ST, bool GATHER>
+__device__ inline void load_site(const typename ST::S* __restrict__ src, long long plane, int pitch, int x,
+                                 int xm, int xp, int r, int rm, int rp, unsigned nb, typename ST::T (&p)[Q]) {
+    constexpr unsigned long long es = sizeof(typename ST::S);
+    unsigned long long off[Q];
+    pull_offsets<ST, GATHER, unsigned long long>((unsigned long long)plane * es, (unsigned long long)pitch * es,
+                                                 x, xm, xp, r, rm, rp, nb, off);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) p[i] = ST::ld((const typename ST::S*)((const char*)src + off[i]));
+}
+
+template <class ST, class OffT>
+__device__ inline typename ST::T ld_at(const void* base, OffT off) {
+    return ST::ld((const typename ST::S*)((const char*)base + off));
+}
+
+template <class ST, class OffT>
+__device__ inline void st_at(void* base, OffT off, typename ST::T v) {
+    ST::st((typename ST::S*)((char*)base + off), v);
 }
 
 // Density in the reference order, ((0.0 + f0) + f1) + ... (deviation forms:
@@ -136,39 +165,280 @@ __device__ inline void feq_dev(float rho, float drho, float ux, float uy, float 
 __device__ inline bool is_dry(double r) { return fabs(r) <= kZeroTol * fmax(1.0, fabs(r)); }
 
 // ---------------------------------------------------------------------------
+// Per-site completion: everything after the gather, for one site whose
+// (gathered) populations are p, densities rho, neighbourhood bits nb and
+// neighbour densities nr[c][i] (i = 1..8, valid where bit i of nb is set).
+// ---------------------------------------------------------------------------
+template <class ST, bool REPLAY, class OffT>
+__device__ inline void finish1(const CompArgs<typename ST::T>& ca, long long plane, OffT plane_b, int s_idx,
+                               OffT home, const typename ST::T (&p)[Q], typename ST::T rho_in, bool fl) {
+    if constexpr (!ST::kDev) {
+        double ux = 0.0, uy = 0.0;
+        const double rho = rho_in;
+        if (fl) {
+            double jx, jy;
+            momentum(p, jx, jy);
+            ux = jx / rho;   // src/fluid.cpp:196-197
+            uy = jy / rho;
+        }
+        double e[Q];
+        feq_ref(rho, ux, uy, e);
+        if (REPLAY) {
+            ca.rho[s_idx] = rho;
+            ca.jx[s_idx] = ux;
+            ca.jy[s_idx] = uy;
+#pragma unroll
+            for (int i = 0; i < Q; ++i) ca.feq[i * plane + s_idx] = e[i];
+        } else if (fl) {
+#pragma unroll
+            for (int i = 0; i < Q; ++i) {
+                double v = ca.omc * p[i] + ca.omega * e[i];
+                if (ca.has_force) v += ca.force[i];
+                st_at<ST>(ca.dst, (OffT)i * plane_b + home, v);
+            }
+        }
+    } else {
+        if (!REPLAY && fl) {
+            const float drho = rho_in;
+            const float rho = ca.rho0 + drho;
+            float jx, jy;
+            momentum(p, jx, jy);
+            const float inv = 1.0f / rho;
+            const float ux = jx * inv, uy = jy * inv;
+            float e[Q];
+            feq_dev(rho, drho, ux, uy, e);
+#pragma unroll
+            for (int i = 0; i < Q; ++i) {
+                float v = ca.omc * p[i] + ca.omega * e[i];
+                if (ca.has_force) v += ca.force[i];
+                st_at<ST>(ca.dst, (OffT)i * plane_b + home, v);
+            }
+        }
+    }
+}
+
+template <class ST, bool REPLAY, class OffT>
+__device__ inline void finish2(const StepArgs<typename ST::T>& a, OffT plane_b, int s_idx, OffT home,
+                               const typename ST::T (&p)[2][Q], const typename ST::T (&rc)[2],
+                               const typename ST::T (&nr)[2][Q], unsigned nb) {
+    const bool fl = nb & 1u;
+    const long long plane = a.plane;
+    if constexpr (!ST::kDev) {
+        const auto& A = a.c[0];
+        const auto& B = a.c[1];
+        const double ra = rc[0], rb = rc[1];
+        double rep[2][2] = {{0.0, 0.0}, {0.0, 0.0}};   // [comp][x/y]
+        double jm[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+        double ue[2][2];
+        double e[2][Q];
+        if (fl) {
+            // repulsion: sum over fluid neighbours of (w_i c_i) psi_other
+            // A skipped (solid) neighbour adds +0.0, which leaves the sum
+            // unchanged: it starts at +0.0 and a round-to-nearest sum is -0.0
+            // only when both terms are, so selects replace the reference's
+            // `continue` exactly.
+            double sax = 0.0, say = 0.0, sbx = 0.0, sby = 0.0;
+#pragma unroll
+            for (int i = 1; i < Q; ++i) {
+                const bool f = (nb >> i) & 1u;
+                const double pa = nr[0][i], pb = nr[1][i];
+                if (cx(i) != 0) {
+                    const double wc = wint(i) * cx(i);
+                    sax += f ? wc * pb : 0.0;
+                    sbx += f ? wc * pa : 0.0;
+                }
+                if (cy(i) != 0) {
+                    const double wc = wint(i) * cy(i);
+                    say += f ? wc * pb : 0.0;
+                    sby += f ? wc * pa : 0.0;
+                }
+            }
+            rep[0][0] = sax * (A.gneg * ra);
+            rep[0][1] = say * (A.gneg * ra);
+            rep[1][0] = sbx * (B.gneg * rb);
+            rep[1][1] = sby * (B.gneg * rb);
+            // adhesion: weighted lattice vectors towards solid neighbours
+            double ssx = 0.0, ssy = 0.0;
+#pragma unroll
+            for (int i = 1; i < Q; ++i) {
+                const bool solid = !((nb >> i) & 1u);
+                ssx += solid ? wint(i) * cx(i) : 0.0;
+                ssy += solid ? wint(i) * cy(i) : 0.0;
+            }
+            const double fs[2][2] = {{A.gsneg * ssx, A.gsneg * ssy}, {B.gsneg * ssx, B.gsneg * ssy}};
+            momentum(p[0], jm[0][0], jm[0][1]);
+            momentum(p[1], jm[1][0], jm[1][1]);
+            const double den = ra * A.omega + rb * B.omega;
+            const double ucx = (jm[0][0] * A.omega + jm[1][0] * B.omega) / den;
+            const double ucy = (jm[0][1] * A.omega + jm[1][1] * B.omega) / den;
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const auto& C = a.c[c];
+                const double r = rc[c];
+                if (!is_dry(r)) {
+                    ue[c][0] = ucx + rep[c][0] * C.inv_omega / r + fs[c][0] * C.inv_omega;
+                    ue[c][1] = ucy + rep[c][1] * C.inv_omega / r + fs[c][1] * C.inv_omega;
+                    if (!REPLAY && fabs(r) < kUeqWatch) {
+                        C.ueqx[s_idx] = ue[c][0];
+                        C.ueqy[s_idx] = ue[c][1];
+                    }
+                } else {
+                    ue[c][0] = C.ueqx[s_idx];
+                    ue[c][1] = C.ueqy[s_idx];
+                }
+            }
+        } else if (REPLAY) {
+            ue[0][0] = A.ueqx[s_idx];
+            ue[0][1] = A.ueqy[s_idx];
+            ue[1][0] = B.ueqx[s_idx];
+            ue[1][1] = B.ueqy[s_idx];
+        }
+        if (fl || REPLAY) {
+#pragma unroll
+            for (int c = 0; c < 2; ++c) feq_ref(rc[c], ue[c][0], ue[c][1], e[c]);
+        }
+        if (REPLAY) {
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const auto& C = a.c[c];
+                C.rho[s_idx] = rc[c];
+                C.jx[s_idx] = jm[c][0];
+                C.jy[s_idx] = jm[c][1];
+                C.frx[s_idx] = rep[c][0];
+                C.fry[s_idx] = rep[c][1];
+                if (fl) {
+                    C.ueqx[s_idx] = ue[c][0];
+                    C.ueqy[s_idx] = ue[c][1];
+                }
+#pragma unroll
+                for (int i = 0; i < Q; ++i) C.feq[i * plane + s_idx] = e[c][i];
+            }
+        } else if (fl) {
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const auto& C = a.c[c];
+#pragma unroll
+                for (int i = 0; i < Q; ++i) {
+                    double v = C.omc * p[c][i] + C.omega * e[c][i];
+                    if (C.has_force) v += C.force[i];
+                    st_at<ST>(C.dst, (OffT)i * plane_b + home, v);
+                }
+            }
+        }
+    } else {
+        // deviation forms, FP32 arithmetic
+        if (!REPLAY && fl) {
+            const auto& A = a.c[0];
+            const auto& B = a.c[1];
+            float sd[2][2] = {{0.f, 0.f}, {0.f, 0.f}};   // sums over fluid nbrs of w c drho_other
+            float sfx = 0.f, sfy = 0.f;                  // sums over fluid nbrs of w c
+#pragma unroll
+            for (int i = 1; i < Q; ++i) {
+                const bool f = (nb >> i) & 1u;
+                const float da = f ? nr[0][i] : 0.f, db = f ? nr[1][i] : 0.f;
+                const float wx = (float)(wint(i) * cx(i)), wy = (float)(wint(i) * cy(i));
+                if (cx(i) != 0) {
+                    sd[0][0] += wx * db;
+                    sd[1][0] += wx * da;
+                    sfx += f ? wx : 0.f;
+                }
+                if (cy(i) != 0) {
+                    sd[0][1] += wy * db;
+                    sd[1][1] += wy * da;
+                    sfy += f ? wy : 0.f;
+                }
+            }
+            const float da0 = rc[0], db0 = rc[1];
+            const float ra = A.rho0 + da0, rb = B.rho0 + db0;
+            // psi_other over fluid neighbours = sum w c drho + rho0_other * sum w c
+            const float fa_x = A.gneg * ra * (sd[0][0] + B.rho0 * sfx);
+            const float fa_y = A.gneg * ra * (sd[0][1] + B.rho0 * sfy);
+            const float fb_x = B.gneg * rb * (sd[1][0] + A.rho0 * sfx);
+            const float fb_y = B.gneg * rb * (sd[1][1] + A.rho0 * sfy);
+            // adhesion: sum over solid nbrs of w c = -(sum over fluid nbrs)
+            const float fs_ax = A.gsneg * -sfx, fs_ay = A.gsneg * -sfy;
+            const float fs_bx = B.gsneg * -sfx, fs_by = B.gsneg * -sfy;
+            float jax, jay, jbx, jby;
+            momentum(p[0], jax, jay);
+            momentum(p[1], jbx, jby);
+            const float inv_den = 1.0f / (ra * A.omega + rb * B.omega);
+            const float ucx = (jax * A.omega + jbx * B.omega) * inv_den;
+            const float ucy = (jay * A.omega + jby * B.omega) * inv_den;
+            float uea[2], ueb[2];
+            if (!is_dry(ra)) {
+                const float ir = 1.0f / ra;
+                uea[0] = ucx + A.inv_omega * (fa_x * ir + fs_ax);
+                uea[1] = ucy + A.inv_omega * (fa_y * ir + fs_ay);
+                if (fabsf(ra) < (float)kUeqWatch) {
+                    A.ueqx[s_idx] = uea[0];
+                    A.ueqy[s_idx] = uea[1];
+                }
+            } else {
+                uea[0] = (float)A.ueqx[s_idx];
+                uea[1] = (float)A.ueqy[s_idx];
+            }
+            if (!is_dry(rb)) {
+                const float ir = 1.0f / rb;
+                ueb[0] = ucx + B.inv_omega * (fb_x * ir + fs_bx);
+                ueb[1] = ucy + B.inv_omega * (fb_y * ir + fs_by);
+                if (fabsf(rb) < (float)kUeqWatch) {
+                    B.ueqx[s_idx] = ueb[0];
+                    B.ueqy[s_idx] = ueb[1];
+                }
+            } else {
+                ueb[0] = (float)B.ueqx[s_idx];
+                ueb[1] = (float)B.ueqy[s_idx];
+            }
+            float ea[Q], eb[Q];
+            feq_dev(ra, da0, uea[0], uea[1], ea);
+            feq_dev(rb, db0, ueb[0], ueb[1], eb);
+#pragma unroll
+            for (int i = 0; i < Q; ++i) {
+                float va = A.omc * p[0][i] + A.omega * ea[i];
+                float vb = B.omc * p[1][i] + B.omega * eb[i];
+                if (A.has_force) va += A.force[i];
+                if (B.has_force) vb += B.force[i];
+                st_at<ST>(A.dst, (OffT)i * plane_b + home, va);
+                st_at<ST>(B.dst, (OffT)i * plane_b + home, vb);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // The fused kernel.  NC components, storage ST, NW wavefronts per workgroup,
 // GATHER (false only for the first step after an upload: populations are the
 // reference's streamed state and are read in place), REPLAY (recompute the
-// step and write the reference's intermediate arrays instead of P_n).
+// step and write the reference's intermediate arrays instead of P_n), OffT
+// the byte-offset type (32-bit while one component's buffer is < 4 GiB), LA
+// the row lookahead of the software pipeline (1 or 2).
 // ---------------------------------------------------------------------------
-template <int NC, class ST, int NW, bool GATHER, bool REPLAY>
+template <int NC, class ST, int NW, bool GATHER, bool REPLAY, class OffT, int LA>
 __global__ __launch_bounds__(64 * NW) void k_step(const StepArgs<typename ST::T> a) {
     using T = typename ST::T;
     using S = typename ST::S;
     constexpr int BW = 64 * NW;
-    constexpr bool DEV = ST::kDev;
+    constexpr OffT ES = sizeof(S);
 
     __shared__ T lds[NC == 2 ? 4 : 1][NC][BW + 2];
 
-    const int nx = a.nx, ny = a.ny, pitch = a.pitch, wpr = a.wpr;
-    const long long plane = a.plane;
+    const int nx = a.nx, ny = a.ny, wpr = a.wpr;
+    const OffT plane_b = (OffT)a.plane * ES;
+    const OffT pitch_b = (OffT)a.pitch * ES;
     const uint64_t* __restrict__ mask = a.mask;
 
     // Workgroup -> strip.  Workgroups are dealt round-robin to the 8 XCDs;
-    // consecutive ids of one XCD are given vertically adjacent strips so the
-    // halo rows they share stay in that XCD's L2.
+    // consecutive ids of one XCD are given adjacent strips so the halo rows
+    // they share stay in that XCD's L2.
     const int nbx = a.gx;
     const int nby = (ny + a.rows - 1) / a.rows;
     int bid = blockIdx.x;
     {
         const int nblk = nbx * nby;
         const int xcd = bid & 7, k = bid >> 3;
-        const int per = (nblk + 7) >> 3;      // ids per XCD (last XCD may hold fewer)
-        const int full = nblk - 8 * (per - 1);  // XCDs holding `per` ids
-        int lin;
-        if (xcd < full) lin = xcd * per + k;
-        else lin = full * per + (xcd - full) * (per - 1) + k;
-        bid = lin;
+        const int per = (nblk + 7) >> 3;          // ids per XCD (the last ones may hold one fewer)
+        const int full = nblk - 8 * (per - 1);    // XCDs holding `per` ids
+        bid = xcd < full ? xcd * per + k : full * per + (xcd - full) * (per - 1) + k;
     }
     const int bx = bid % nbx, by = bid / nbx;
 
@@ -181,374 +451,182 @@ __global__ __launch_bounds__(64 * NW) void k_step(const StepArgs<typename ST::T>
     const int y0 = by * a.rows;
     const int y1 = min(y0 + a.rows, ny);
 
-    // halo columns (density only)
+    // Halo columns (densities only).  The left halo belongs to thread 0, the
+    // right one to the last thread of the strip, or to thread 1 when the
+    // strip is one column wide (so no thread holds both).
     const int tlast = min(BW, nx - x0) - 1;
-    const bool lhalo = NC == 2 && tid == 0;
-    const bool rhalo = NC == 2 && tid == tlast;
-    const int hl = wrap1(x0 - 1, nx);
-    const int hr = wrap1(x0 + tlast + 1, nx);
-    const int hlm = wrap1(hl - 1, nx), hlp = wrap1(hl + 1, nx);
-    const int hrm = wrap1(hr - 1, nx), hrp = wrap1(hr + 1, nx);
+    const int hrt = tlast == 0 ? 1 : tlast;
+    const bool hrole = NC == 2 && (tid == 0 || tid == hrt);
+    const int hcol = tid == 0 ? wrap1(x0 - 1, nx) : wrap1(x0 + tlast + 1, nx);
+    const int hslot = tid == 0 ? 0 : tlast + 2;
+    const int hcm = wrap1(hcol - 1, nx), hcp = wrap1(hcol + 1, nx);
 
-    const S* src[NC];
-    S* dst[NC];
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-        src[c] = (const S*)a.c[c].src;
-        dst[c] = (S*)a.c[c].dst;
-    }
+    // Lookahead: the row whose loads are issued while row y is finished is
+    // y+LA.  With LA = 1 (two register sets) its densities are filed before
+    // the barrier; with LA = 2 (three sets) after finishing row y, so the
+    // loads stay in flight during the arithmetic of row y.
+    T pA[NC][Q], pB[NC][Q], pC[NC][Q], ph[NC][Q];
+    unsigned nbA = 0, nbB = 0, nbC = 0, hnb = 0;
+    // mask triples (west, centre, east bits) of rows y+LA-1 .. y+LA+1 for
+    // the main column (t*) and the halo column (h*), rotated every row
+    unsigned t1 = 0, t2 = 0, t3 = 0, h1 = 0, h2 = 0, h3 = 0;
 
-    // Gather one site for all components, return the neighbourhood bits and
-    // fill populations p and densities (or density deviations) rho.
-    auto site = [&](int col, int colm, int colp, int r, int rm, int rp, unsigned ts, unsigned tc,
-                    unsigned tn, T (&p)[NC][Q], T (&rho)[NC]) -> unsigned {
-        const unsigned nb = neigh_bits(ts, tc, tn);
-        const bool fl = nb & 1u;
+    // Loads of one site (all components) in one block; a solid site in a
+    // gathering step reads nothing (the reference's streamed state is zero
+    // there) and its lane is simply masked off for the whole block.
+    auto issue = [&](int col, int colm, int colp, int r, int rm, int rp, unsigned nb, T (&p)[NC][Q]) {
+        OffT off[Q];
+        pull_offsets<ST, GATHER, OffT>(plane_b, pitch_b, col, colm, colp, r, rm, rp, nb, off);
+        if (!GATHER || (nb & 1u)) {
 #pragma unroll
-        for (int c = 0; c < NC; ++c) {
-            if (!GATHER || fl) {
-                load_site<ST, GATHER>(src[c], plane, pitch, col, colm, colp, r, rm, rp, nb, p[c]);
-            } else {
+            for (int c = 0; c < NC; ++c) {
+#pragma unroll
+                for (int i = 0; i < Q; ++i) p[c][i] = ld_at<ST, OffT>(a.c[c].src, off[i]);
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
 #pragma unroll
                 for (int i = 0; i < Q; ++i) p[c][i] = T(0);
             }
-            rho[c] = dens(p[c]);
         }
-        return nb;
     };
-
-    T pc[NC][Q], pn[NC][Q];
-    T rc[NC], rn[NC];
-    unsigned nbc = 0, nbn = 0;
-
-    // rows needed: mask triples of y0-2 .. y1+1, gathered rows y0-1 .. y1
-    const int rS2 = wrap1(wrap1(y0 - 1, ny) - 1, ny);
-    const int rS1 = wrap1(y0 - 1, ny);
-    const int rN1 = wrap1(y0 + 1, ny);
-    unsigned t_s2 = 0, t_s1 = 0, t_0 = 0, t_1 = 0;
-    if (xin) {
-        t_s2 = triple(mask, wpr, rS2, xm, x, xp);
-        t_s1 = triple(mask, wpr, rS1, xm, x, xp);
-        t_0 = triple(mask, wpr, y0, xm, x, xp);
-        t_1 = triple(mask, wpr, rN1, xm, x, xp);
-    }
-
-    if constexpr (NC == 2) {
-        // row y0-1: densities only
-        if (xin) {
-            T pt[NC][Q], rt[NC];
-            site(x, xm, xp, rS1, rS2, y0, t_s2, t_s1, t_0, pt, rt);
-#pragma unroll
-            for (int c = 0; c < NC; ++c) lds[0][c][tid + 1] = rt[c];
-        }
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            if (!(h == 0 ? lhalo : rhalo)) continue;
-            const int hc = h == 0 ? hl : hr, hm = h == 0 ? hlm : hrm, hp = h == 0 ? hlp : hrp;
-            const int slot = h == 0 ? 0 : tid + 2;
-#pragma unroll
-            for (int k = 0; k < 2; ++k) {   // rows y0-1 and y0
-                const int r = k == 0 ? rS1 : y0;
-                const int rm = k == 0 ? rS2 : rS1;
-                const int rp = k == 0 ? y0 : rN1;
-                const unsigned ts = triple(mask, wpr, rm, hm, hc, hp);
-                const unsigned tc = triple(mask, wpr, r, hm, hc, hp);
-                const unsigned tn = triple(mask, wpr, rp, hm, hc, hp);
-                T pt[NC][Q], rt[NC];
-                site(hc, hm, hp, r, rm, rp, ts, tc, tn, pt, rt);
-#pragma unroll
-                for (int c = 0; c < NC; ++c) lds[k][c][slot] = rt[c];
-            }
-        }
-    }
-    if (xin) {
-        nbc = site(x, xm, xp, y0, rS1, rN1, t_s1, t_0, t_1, pc, rc);
+    auto file_rho = [&](const T (&p)[NC][Q], int slot, int col_slot) {
         if constexpr (NC == 2) {
 #pragma unroll
-            for (int c = 0; c < NC; ++c) lds[1][c][tid + 1] = rc[c];
+            for (int c = 0; c < NC; ++c) lds[slot][c][col_slot] = dens(p[c]);
+        }
+    };
+
+    // ---- prologue: rows y0-1 (densities), y0 .. y0+LA-1 -------------------
+    {
+        const int r_m2 = wrap1(wrap1(y0 - 1, ny) - 1, ny), r_m1 = wrap1(y0 - 1, ny);
+        const int r_p1 = wrap1(y0 + 1, ny), r_p2 = wrap1(r_p1 + 1, ny), r_p3 = wrap1(r_p2 + 1, ny);
+        unsigned tm2 = 0, tm1 = 0, t0 = 0, tp1 = 0, tp2 = 0, tp3 = 0;
+        unsigned hm2 = 0, hm1 = 0, h0 = 0, hp1 = 0, hp2 = 0, hp3 = 0;
+        if (xin) {
+            tm2 = triple(mask, wpr, r_m2, xm, x, xp);
+            tm1 = triple(mask, wpr, r_m1, xm, x, xp);
+            t0 = triple(mask, wpr, y0, xm, x, xp);
+            tp1 = triple(mask, wpr, r_p1, xm, x, xp);
+            tp2 = triple(mask, wpr, r_p2, xm, x, xp);
+            if (LA == 2) tp3 = triple(mask, wpr, r_p3, xm, x, xp);
+        }
+        if (hrole) {
+            hm2 = triple(mask, wpr, r_m2, hcm, hcol, hcp);
+            hm1 = triple(mask, wpr, r_m1, hcm, hcol, hcp);
+            h0 = triple(mask, wpr, y0, hcm, hcol, hcp);
+            hp1 = triple(mask, wpr, r_p1, hcm, hcol, hcp);
+            hp2 = triple(mask, wpr, r_p2, hcm, hcol, hcp);
+            if (LA == 2) hp3 = triple(mask, wpr, r_p3, hcm, hcol, hcp);
+        }
+        if constexpr (NC == 2) {
+            if (xin) {
+                issue(x, xm, xp, r_m1, r_m2, y0, neigh_bits(tm2, tm1, t0), pC);
+                file_rho(pC, 0, tid + 1);
+            }
+            if (hrole) {
+                issue(hcol, hcm, hcp, r_m1, r_m2, y0, neigh_bits(hm2, hm1, h0), ph);
+                file_rho(ph, 0, hslot);
+                issue(hcol, hcm, hcp, y0, r_m1, r_p1, neigh_bits(hm1, h0, hp1), ph);
+                file_rho(ph, 1, hslot);
+                if (LA == 2) {
+                    issue(hcol, hcm, hcp, r_p1, y0, r_p2, neigh_bits(h0, hp1, hp2), ph);
+                    file_rho(ph, 2, hslot);
+                }
+            }
+        }
+        if (xin) {
+            nbA = neigh_bits(tm1, t0, tp1);
+            issue(x, xm, xp, y0, r_m1, r_p1, nbA, pA);
+            file_rho(pA, 1, tid + 1);
+            if (LA == 2 && (NC == 2 || y0 + 1 < y1)) {
+                nbB = neigh_bits(t0, tp1, tp2);
+                issue(x, xm, xp, r_p1, y0, r_p2, nbB, pB);
+                file_rho(pB, 2, tid + 1);
+            }
+        }
+        if (LA == 1) {
+            t1 = t0; t2 = tp1; t3 = tp2;
+            h1 = h0; h2 = hp1; h3 = hp2;
+        } else {
+            t1 = tp1; t2 = tp2; t3 = tp3;
+            h1 = hp1; h2 = hp2; h3 = hp3;
         }
     }
 
-    for (int y = y0; y < y1; ++y) {
+    // ---- one row: issue row y+LA, finish row y ----------------------------
+    auto body = [&](int y, T (&p0)[NC][Q], unsigned nb0, T (&pn)[NC][Q], unsigned& nbn) {
         const int k = y - y0;
-        const int ym = wrap1(y - 1, ny);
-        const int yp = wrap1(y + 1, ny);
-        const int ypp = wrap1(yp + 1, ny);
-        // gather row y+1 (its populations are the next row's input)
-        const bool more = (y + 1 < y1);
-        if (NC == 2 || more) {
+        const bool more = NC == 2 ? (y + LA <= y1) : (y + LA < y1);
+        const int rA = wrap1(y + LA, ny);              // row issued now
+        const int rAm = wrap1(rA - 1, ny), rAp = wrap1(rA + 1, ny);
+        const int slotA = (k + LA + 1) & 3;
+        if (more) {
             if (xin) {
-                const unsigned t_2 = triple(mask, wpr, ypp, xm, x, xp);
-                nbn = site(x, xm, xp, yp, y, ypp, t_0, t_1, t_2, pn, rn);
-                t_0 = t_1;
-                t_1 = t_2;
-                if constexpr (NC == 2) {
-#pragma unroll
-                    for (int c = 0; c < NC; ++c) lds[(k + 2) & 3][c][tid + 1] = rn[c];
-                }
+                nbn = neigh_bits(t1, t2, t3);
+                issue(x, xm, xp, rA, rAm, rAp, nbn, pn);
             }
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                if (!(h == 0 ? lhalo : rhalo)) continue;
-                const int hc = h == 0 ? hl : hr, hm = h == 0 ? hlm : hrm, hp = h == 0 ? hlp : hrp;
-                const int slot = h == 0 ? 0 : tid + 2;
-                const unsigned ts = triple(mask, wpr, y, hm, hc, hp);
-                const unsigned tc = triple(mask, wpr, yp, hm, hc, hp);
-                const unsigned tn = triple(mask, wpr, ypp, hm, hc, hp);
-                T pt[NC][Q], rt[NC];
-                site(hc, hm, hp, yp, y, ypp, ts, tc, tn, pt, rt);
-#pragma unroll
-                for (int c = 0; c < NC; ++c) lds[(k + 2) & 3][c][slot] = rt[c];
+            if (hrole) {
+                hnb = neigh_bits(h1, h2, h3);
+                issue(hcol, hcm, hcp, rA, rAm, rAp, hnb, ph);
+            }
+            // mask row of the next issue
+            const int rnext = wrap1(rAp + 1, ny);
+            if (xin) {
+                t1 = t2;
+                t2 = t3;
+                t3 = triple(mask, wpr, rnext, xm, x, xp);
+            }
+            if (hrole) {
+                h1 = h2;
+                h2 = h3;
+                h3 = triple(mask, wpr, rnext, hcm, hcol, hcp);
+            }
+            if (LA == 1) {
+                if (xin) file_rho(pn, slotA, tid + 1);
+                if (hrole) file_rho(ph, slotA, hslot);
             }
         }
-        if (NC == 2) __syncthreads();
-
+        if constexpr (NC == 2) __syncthreads();
         if (xin) {
-            const int s_idx = y * pitch + x;
-            const bool fl = nbc & 1u;
+            const int s_idx = y * a.pitch + x;
+            const OffT home = (OffT)y * pitch_b + (OffT)x * ES;
             if constexpr (NC == 1) {
-                // ---------------- one component: collide(geom,f) + force -----------
-                const auto& ca = a.c[0];
-                if constexpr (!DEV) {
-                    double ux = 0.0, uy = 0.0;
-                    const double rho = rc[0];
-                    if (fl) {
-                        double jx, jy;
-                        momentum(pc[0], jx, jy);
-                        ux = jx / rho;   // src/fluid.cpp:196-197
-                        uy = jy / rho;
-                    }
-                    double e[Q];
-                    feq_ref(rho, ux, uy, e);
-                    if (REPLAY) {
-                        ca.rho[s_idx] = rho;
-                        ca.jx[s_idx] = ux;
-                        ca.jy[s_idx] = uy;
-#pragma unroll
-                        for (int i = 0; i < Q; ++i) ca.feq[i * plane + s_idx] = e[i];
-                    } else if (fl) {
-                        S* d = dst[0];
-#pragma unroll
-                        for (int i = 0; i < Q; ++i) {
-                            double v = ca.omc * pc[0][i] + ca.omega * e[i];
-                            if (ca.has_force) v += ca.force[i];
-                            ST::st(d + i * plane + s_idx, v);
-                        }
-                    }
-                } else {
-                    if (!REPLAY && fl) {
-                        const float drho = rc[0];
-                        const float rho = ca.rho0 + drho;
-                        float jx, jy;
-                        momentum(pc[0], jx, jy);
-                        const float ux = jx / rho, uy = jy / rho;
-                        float e[Q];
-                        feq_dev(rho, drho, ux, uy, e);
-                        S* d = dst[0];
-#pragma unroll
-                        for (int i = 0; i < Q; ++i) {
-                            float v = ca.omc * pc[0][i] + ca.omega * e[i];
-                            if (ca.has_force) v += ca.force[i];
-                            ST::st(d + i * plane + s_idx, v);
-                        }
-                    }
-                }
+                finish1<ST, REPLAY, OffT>(a.c[0], a.plane, plane_b, s_idx, home, p0[0], dens(p0[0]), nb0 & 1u);
             } else {
-                // ---------------- two components: Shan-Chen ----------------------
                 const int sS = k & 3, sC = (k + 1) & 3, sN = (k + 2) & 3;
-                const int cl = tid + 1;
-                if constexpr (!DEV) {
-                    const auto& A = a.c[0];
-                    const auto& B = a.c[1];
-                    const double ra = rc[0], rb = rc[1];
-                    double rep[2][2] = {{0.0, 0.0}, {0.0, 0.0}};   // [comp][x/y]
-                    double jm[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
-                    double ue[2][2];
-                    double e[2][Q];
-                    if (fl) {
-                        // repulsion: sum over fluid neighbours of (w_i c_i) psi_other
-                        double sax = 0.0, say = 0.0, sbx = 0.0, sby = 0.0;
+                T nr[2][Q], r0[2];
 #pragma unroll
-                        for (int i = 1; i < Q; ++i) {
-                            if ((nbc >> i) & 1u) {
-                                const int sl = cy(i) == 1 ? sN : (cy(i) == -1 ? sS : sC);
-                                const double pa = lds[sl][0][cl + cx(i)];
-                                const double pb = lds[sl][1][cl + cx(i)];
-                                if (cx(i) != 0) {
-                                    const double wc = wint(i) * cx(i);
-                                    sax += wc * pb;
-                                    sbx += wc * pa;
-                                }
-                                if (cy(i) != 0) {
-                                    const double wc = wint(i) * cy(i);
-                                    say += wc * pb;
-                                    sby += wc * pa;
-                                }
-                            }
-                        }
-                        rep[0][0] = sax * (A.gneg * ra);
-                        rep[0][1] = say * (A.gneg * ra);
-                        rep[1][0] = sbx * (B.gneg * rb);
-                        rep[1][1] = sby * (B.gneg * rb);
-                        // adhesion: weighted lattice vectors towards solid neighbours
-                        double ssx = 0.0, ssy = 0.0;
+                for (int i = 1; i < Q; ++i) {
+                    const int sl = cy(i) == 1 ? sN : (cy(i) == -1 ? sS : sC);
 #pragma unroll
-                        for (int i = 1; i < Q; ++i) {
-                            if (!((nbc >> i) & 1u)) {
-                                ssx += wint(i) * cx(i);
-                                ssy += wint(i) * cy(i);
-                            }
-                        }
-                        const double fs[2][2] = {{A.gsneg * ssx, A.gsneg * ssy},
-                                                 {B.gsneg * ssx, B.gsneg * ssy}};
-                        momentum(pc[0], jm[0][0], jm[0][1]);
-                        momentum(pc[1], jm[1][0], jm[1][1]);
-                        const double den = ra * A.omega + rb * B.omega;
-                        const double ucx = (jm[0][0] * A.omega + jm[1][0] * B.omega) / den;
-                        const double ucy = (jm[0][1] * A.omega + jm[1][1] * B.omega) / den;
-#pragma unroll
-                        for (int c = 0; c < 2; ++c) {
-                            const auto& C = a.c[c];
-                            const double r = rc[c];
-                            if (!is_dry(r)) {
-                                ue[c][0] = ucx + rep[c][0] * C.inv_omega / r + fs[c][0] * C.inv_omega;
-                                ue[c][1] = ucy + rep[c][1] * C.inv_omega / r + fs[c][1] * C.inv_omega;
-                                if (!REPLAY && fabs(r) < kUeqWatch) {
-                                    C.ueqx[s_idx] = ue[c][0];
-                                    C.ueqy[s_idx] = ue[c][1];
-                                }
-                            } else {
-                                ue[c][0] = C.ueqx[s_idx];
-                                ue[c][1] = C.ueqy[s_idx];
-                            }
-                        }
-                    } else if (REPLAY) {
-                        ue[0][0] = A.ueqx[s_idx];
-                        ue[0][1] = A.ueqy[s_idx];
-                        ue[1][0] = B.ueqx[s_idx];
-                        ue[1][1] = B.ueqy[s_idx];
-                    }
-                    if (fl || REPLAY) {
-#pragma unroll
-                        for (int c = 0; c < 2; ++c) feq_ref(rc[c], ue[c][0], ue[c][1], e[c]);
-                    }
-                    if (REPLAY) {
-#pragma unroll
-                        for (int c = 0; c < 2; ++c) {
-                            const auto& C = a.c[c];
-                            C.rho[s_idx] = rc[c];
-                            C.jx[s_idx] = jm[c][0];
-                            C.jy[s_idx] = jm[c][1];
-                            C.frx[s_idx] = rep[c][0];
-                            C.fry[s_idx] = rep[c][1];
-                            if (fl) {
-                                C.ueqx[s_idx] = ue[c][0];
-                                C.ueqy[s_idx] = ue[c][1];
-                            }
-#pragma unroll
-                            for (int i = 0; i < Q; ++i) C.feq[i * plane + s_idx] = e[c][i];
-                        }
-                    } else if (fl) {
-#pragma unroll
-                        for (int c = 0; c < 2; ++c) {
-                            const auto& C = a.c[c];
-                            S* d = dst[c];
-#pragma unroll
-                            for (int i = 0; i < Q; ++i) {
-                                double v = C.omc * pc[c][i] + C.omega * e[c][i];
-                                if (C.has_force) v += C.force[i];
-                                ST::st(d + i * plane + s_idx, v);
-                            }
-                        }
-                    }
-                } else {
-                    // deviation forms, FP32 arithmetic
-                    if (!REPLAY && fl) {
-                        const auto& A = a.c[0];
-                        const auto& B = a.c[1];
-                        float sd[2][2] = {{0.f, 0.f}, {0.f, 0.f}};   // sums over fluid nbrs of w c drho_other
-                        float sfx = 0.f, sfy = 0.f;                  // sums over fluid nbrs of w c
-#pragma unroll
-                        for (int i = 1; i < Q; ++i) {
-                            if ((nbc >> i) & 1u) {
-                                const int sl = cy(i) == 1 ? sN : (cy(i) == -1 ? sS : sC);
-                                const float da = lds[sl][0][cl + cx(i)];
-                                const float db = lds[sl][1][cl + cx(i)];
-                                const float wx = (float)(wint(i) * cx(i)), wy = (float)(wint(i) * cy(i));
-                                sd[0][0] += wx * db;
-                                sd[0][1] += wy * db;
-                                sd[1][0] += wx * da;
-                                sd[1][1] += wy * da;
-                                sfx += wx;
-                                sfy += wy;
-                            }
-                        }
-                        const float da0 = rc[0], db0 = rc[1];
-                        const float ra = A.rho0 + da0, rb = B.rho0 + db0;
-                        // psi_other summed over fluid neighbours = sum w c drho + rho0_other * sum w c
-                        const float fa_x = A.gneg * ra * (sd[0][0] + B.rho0 * sfx);
-                        const float fa_y = A.gneg * ra * (sd[0][1] + B.rho0 * sfy);
-                        const float fb_x = B.gneg * rb * (sd[1][0] + A.rho0 * sfx);
-                        const float fb_y = B.gneg * rb * (sd[1][1] + A.rho0 * sfy);
-                        // adhesion: sum over solid nbrs of w c = -(sum over fluid nbrs)
-                        const float fs_ax = A.gsneg * -sfx, fs_ay = A.gsneg * -sfy;
-                        const float fs_bx = B.gsneg * -sfx, fs_by = B.gsneg * -sfy;
-                        float jax, jay, jbx, jby;
-                        momentum(pc[0], jax, jay);
-                        momentum(pc[1], jbx, jby);
-                        const float den = ra * A.omega + rb * B.omega;
-                        const float ucx = (jax * A.omega + jbx * B.omega) / den;
-                        const float ucy = (jay * A.omega + jby * B.omega) / den;
-                        float uea[2], ueb[2];
-                        if (!is_dry(ra)) {
-                            uea[0] = ucx + A.inv_omega * (fa_x / ra + fs_ax);
-                            uea[1] = ucy + A.inv_omega * (fa_y / ra + fs_ay);
-                            if (fabsf(ra) < (float)kUeqWatch) {
-                                A.ueqx[s_idx] = uea[0];
-                                A.ueqy[s_idx] = uea[1];
-                            }
-                        } else {
-                            uea[0] = (float)A.ueqx[s_idx];
-                            uea[1] = (float)A.ueqy[s_idx];
-                        }
-                        if (!is_dry(rb)) {
-                            ueb[0] = ucx + B.inv_omega * (fb_x / rb + fs_bx);
-                            ueb[1] = ucy + B.inv_omega * (fb_y / rb + fs_by);
-                            if (fabsf(rb) < (float)kUeqWatch) {
-                                B.ueqx[s_idx] = ueb[0];
-                                B.ueqy[s_idx] = ueb[1];
-                            }
-                        } else {
-                            ueb[0] = (float)B.ueqx[s_idx];
-                            ueb[1] = (float)B.ueqy[s_idx];
-                        }
-                        float ea[Q], eb[Q];
-                        feq_dev(ra, da0, uea[0], uea[1], ea);
-                        feq_dev(rb, db0, ueb[0], ueb[1], eb);
-                        S* d0 = dst[0];
-                        S* d1 = dst[1];
-#pragma unroll
-                        for (int i = 0; i < Q; ++i) {
-                            float va = A.omc * pc[0][i] + A.omega * ea[i];
-                            float vb = B.omc * pc[1][i] + B.omega * eb[i];
-                            if (A.has_force) va += A.force[i];
-                            if (B.has_force) vb += B.force[i];
-                            ST::st(d0 + i * plane + s_idx, va);
-                            ST::st(d1 + i * plane + s_idx, vb);
-                        }
-                    }
+                    for (int c = 0; c < 2; ++c) nr[c][i] = lds[sl][c][tid + 1 + cx(i)];
                 }
+#pragma unroll
+                for (int c = 0; c < 2; ++c) {
+                    r0[c] = lds[sC][c][tid + 1];
+                    nr[c][0] = r0[c];
+                }
+                finish2<ST, REPLAY, OffT>(a, plane_b, s_idx, home, p0, r0, nr, nb0);
             }
         }
-        // rotate registers
-        if (xin) {
-            nbc = nbn;
-#pragma unroll
-            for (int c = 0; c < NC; ++c) {
-                rc[c] = rn[c];
-#pragma unroll
-                for (int i = 0; i < Q; ++i) pc[c][i] = pn[c][i];
-            }
+        if (LA == 2 && more) {
+            if (xin) file_rho(pn, slotA, tid + 1);
+            if (hrole) file_rho(ph, slotA, hslot);
+        }
+    };
+
+    if constexpr (LA == 1) {
+        for (int y = y0; y < y1; y += 2) {
+            body(y, pA, nbA, pB, nbB);
+            if (y + 1 < y1) body(y + 1, pB, nbB, pA, nbA);
+        }
+    } else {
+        for (int y = y0; y < y1; y += 3) {
+            body(y, pA, nbA, pC, nbC);
+            if (y + 1 < y1) body(y + 1, pB, nbB, pA, nbA);
+            if (y + 2 < y1) body(y + 2, pC, nbC, pB, nbB);
         }
     }
 }

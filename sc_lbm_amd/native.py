@@ -24,7 +24,7 @@ EXPORTS = [
     "scl_compute_f_equilibrium_geom", "scl_compute_f_equilibrium", "scl_solid_surface_force",
     "scl_fluid_repulsion", "scl_equilibrium_velocities", "scl_collide1", "scl_collide2",
     "scl_volume_force", "scl_stream1", "scl_stream2", "scl_step1", "scl_step2", "scl_moments",
-    "scl_set_tile", "scl_get_tile", "scl_timing",
+    "scl_set_tile", "scl_set_lookahead", "scl_get_tile", "scl_autotune", "scl_timing", "scl_probe_copy",
 ]
 
 
@@ -78,8 +78,11 @@ def load():
         "scl_step2": [_vp, _vp, _vp, _dp, _dp, ctypes.c_int],
         "scl_moments": [_vp, _vp, _dp],
         "scl_set_tile": [_vp, ctypes.c_int, ctypes.c_int],
-        "scl_get_tile": [_vp, _ip, _ip],
+        "scl_set_lookahead": [_vp, ctypes.c_int],
+        "scl_get_tile": [_vp, ctypes.c_int, ctypes.c_int, _ip, _ip, _ip],
+        "scl_autotune": [_vp, ctypes.c_int, ctypes.c_int, _ip, _ip, _ip],
         "scl_timing": [ctypes.c_int, _dp, _ip],
+        "scl_probe_copy": [ctypes.c_size_t, ctypes.c_int, _dp],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -131,10 +134,24 @@ class Geometry:
     def set_tile(self, cols, rows):
         check(load().scl_set_tile(self.h, cols, rows))
 
-    def tile(self):
-        c, r = ctypes.c_int(), ctypes.c_int()
-        check(load().scl_get_tile(self.h, ctypes.byref(c), ctypes.byref(r)))
-        return c.value, r.value
+    def set_lookahead(self, rows_ahead):
+        check(load().scl_set_lookahead(self.h, int(rows_ahead)))
+
+    def tile(self, ncomp=2, precision=FP64):
+        """(cols, rows, lookahead) the fused kernel uses on this geometry."""
+        if isinstance(precision, str):
+            precision = PRECISION[precision]
+        c, r, la = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(load().scl_get_tile(self.h, ncomp, precision, ctypes.byref(c), ctypes.byref(r), ctypes.byref(la)))
+        return c.value, r.value, la.value
+
+    def autotune(self, ncomp=2, precision=FP64):
+        """Time every launch shape and keep the fastest; returns it."""
+        if isinstance(precision, str):
+            precision = PRECISION[precision]
+        c, r, la = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(load().scl_autotune(self.h, ncomp, precision, ctypes.byref(c), ctypes.byref(r), ctypes.byref(la)))
+        return c.value, r.value, la.value
 
     def close(self):
         if getattr(self, "h", None):
@@ -269,6 +286,13 @@ def timing(reset=0):
     avg, cnt = ctypes.c_double(), ctypes.c_int()
     check(load().scl_timing(int(reset), ctypes.byref(avg), ctypes.byref(cnt)))
     return avg.value, cnt.value
+
+
+def probe_copy(nbytes, reps=20):
+    """Mean milliseconds of one device copy of nbytes (k_probe_copy)."""
+    ms = ctypes.c_double()
+    check(load().scl_probe_copy(int(nbytes), int(reps), ctypes.byref(ms)))
+    return ms.value
 
 
 def gpu_available():

@@ -1,0 +1,68 @@
+"""Lean drivers for profiler runs (rocprofv3 puts the program after `--`).
+
+    python -m sc_lbm_amd.profiling step --form fp32dev --n 4096 --steps 20
+        builds the 4096^2 two-component spinodal case, runs `steps` fused
+        steps of one storage form and prints the mean kernel time;
+    python -m sc_lbm_amd.profiling copy --bytes 2147483648 --reps 10
+        runs the plain copy kernel k_probe_copy (reads and writes `bytes`
+        each launch) so the HBM byte counters can be calibrated.
+"""
+import argparse
+import json
+import os
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def main(argv=None):
+    p = argparse.ArgumentParser()
+    sub = p.add_subparsers(dest="cmd", required=True)
+    s = sub.add_parser("step")
+    s.add_argument("--form", default="fp32dev")
+    s.add_argument("--n", type=int, default=4096)
+    s.add_argument("--steps", type=int, default=20)
+    s.add_argument("--warmup", type=int, default=2)
+    s.add_argument("--tile", default="")
+    c = sub.add_parser("copy")
+    c.add_argument("--bytes", type=float, default=2 ** 31)
+    c.add_argument("--reps", type=int, default=10)
+    a = p.parse_args(argv)
+
+    from sc_lbm_amd import native as nat
+    nat.init(0)
+    if a.cmd == "copy":
+        ms = nat.probe_copy(int(a.bytes), a.reps)
+        print(json.dumps({"kernel": "k_probe_copy", "bytes_read": int(a.bytes) // 16 * 16,
+                          "bytes_written": int(a.bytes) // 16 * 16, "avg_ms": ms,
+                          "gbs": 2 * a.bytes / (ms * 1e-3) / 1e9}))
+        return
+    from sc_lbm_amd import workloads as wl
+    case = wl.spinodal(a.n, G=1.2, seed=777)
+    g = nat.Geometry(case["mask"])
+    if a.tile:
+        v = [int(t) for t in a.tile.split(",")]
+        g.set_tile(v[0], v[1])
+        if len(v) > 2:
+            g.set_lookahead(v[2])
+    else:
+        g.autotune(2, nat.PRECISION[a.form])
+    fl = [nat.Field(g, case["tau"][k], nat.PRECISION[a.form], 1.0, g_fluid=case["g_fluid"][k])
+          for k in range(2)]
+    for k in range(2):
+        fl[k].set_f(case["f"][k])
+    nat.solid_surface_force(g, fl[0], fl[1])
+    nat.step2(g, fl[0], fl[1], None, None, a.warmup)
+    nat.synchronize()
+    nat.timing(1)
+    nat.step2(g, fl[0], fl[1], None, None, a.steps)
+    avg, cnt = nat.timing(0)
+    print(json.dumps({"form": a.form, "n": a.n, "steps": a.steps, "kernel_avg_ms": avg, "launches": cnt,
+                      "tile": list(g.tile(2, nat.PRECISION[a.form])),
+                      "mlups": a.n * a.n / (avg * 1e-3) / 1e6 if avg else None}))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,76 @@
+"""Case files for tests/drivers/sc_driver (binary, little-endian) and the
+reader of its output."""
+import os
+import struct
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+BIN = os.path.join(HERE, "bin", "sc_driver")
+MAGIC = 0x424C4353
+
+OUT_1 = ["f", "rho", "ux", "uy", "feq"]
+OUT_2 = OUT_1 + ["ueqx", "ueqy", "frx", "fry", "fsx", "fsy"]
+
+
+def write_case(path, case, steps, mode=0):
+    mask = np.ascontiguousarray(case["mask"], dtype=np.int32)
+    ny, nx = mask.shape
+    nc = len(case["tau"])
+    forces = list(case.get("force", [None, None])) + [None, None]
+    hdr = [MAGIC, nx, ny, nc, steps, mode, int(forces[0] is not None), int(forces[1] is not None)]
+
+    def two(v):
+        v = list(v) + [0.0] * (2 - len(v))
+        return np.array(v[:2], dtype=np.float64)
+
+    with open(path, "wb") as fh:
+        fh.write(struct.pack("<8i", *hdr))
+        fh.write(two(case["tau"]).tobytes())
+        fh.write(two(case.get("g_fluid", [0.0, 0.0])).tobytes())
+        fh.write(two(case.get("g_solid", [0.0, 0.0])).tobytes())
+        for k in range(2):
+            fv = forces[k] if forces[k] is not None else np.zeros(9)
+            fh.write(np.ascontiguousarray(fv, dtype=np.float64).tobytes())
+        fh.write(mask.tobytes())
+        f0 = case["f0"] if "f0" in case else case["f"]
+        for c in range(nc):
+            fh.write(np.ascontiguousarray(f0[c], dtype=np.float64).tobytes())
+
+
+def read_output(path, ncomp):
+    names = OUT_1 if ncomp == 1 else OUT_2
+    out = {}
+    with open(path, "rb") as fh:
+        for c in range(ncomp):
+            for nm in names:
+                (n,) = struct.unpack("<Q", fh.read(8))
+                out[(nm, c)] = np.frombuffer(fh.read(8 * n), dtype=np.float64).copy()
+    return out
+
+
+def run(case, steps, mode, workdir):
+    cpath = os.path.join(workdir, "case.bin")
+    opath = os.path.join(workdir, "out.bin")
+    write_case(cpath, case, steps, mode)
+    p = subprocess.run([BIN, cpath, opath], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                       timeout=300)
+    if mode == 3:
+        return p.returncode, p.stdout
+    if p.returncode != 0:
+        raise RuntimeError("sc_driver failed (%d): %s" % (p.returncode, p.stdout[-2000:]))
+    return read_output(opath, len(case["tau"]))
+
+
+def build():
+    """Compile the driver against include/ and libsclbm.so (rpath to the
+    in-tree library)."""
+    os.makedirs(os.path.dirname(BIN), exist_ok=True)
+    lib = os.path.join(ROOT, "sc_lbm_amd")
+    cmd = ["g++", "-std=c++14", "-O2", "-Wall", "-I" + os.path.join(ROOT, "include"),
+           os.path.join(HERE, "sc_driver.cpp"), "-o", BIN, "-L" + lib, "-lsclbm",
+           "-Wl,-rpath,$ORIGIN/../../../sc_lbm_amd"]
+    subprocess.check_call(cmd)
+    return BIN

@@ -10,7 +10,9 @@ NAMES = {"f": nat.F, "feq": nat.FEQ, "rho": nat.RHO, "ux": nat.UX, "uy": nat.UY,
 def make(case, precision=nat.FP64, rho0=(1.0, 1.0), tile=None):
     g = nat.Geometry(case["mask"])
     if tile:
-        g.set_tile(*tile)
+        g.set_tile(tile[0], tile[1])
+        if len(tile) > 2:
+            g.set_lookahead(tile[2])
     nc = len(case["tau"])
     fl = []
     for c in range(nc):

@@ -84,7 +84,7 @@ def test_fused_fp64_matches_oracle_1000_steps(cfg):
 def test_tile_shape_does_not_change_results():
     case = wl.porous(200, 96, porosity=0.75, seed=8, rmin=4, rmax=7)
     ref = None
-    for tile in [(64, 4), (128, 7), (256, 16), (512, 3), (64, 96)]:
+    for tile in [(64, 4, 1), (128, 7, 2), (256, 16, 1), (256, 16, 2), (512, 3, 2), (64, 96, 1)]:
         _, fl = gh.run_fused(case, 60, tile=tile)
         got = [f.get_f() for f in fl]
         if ref is None:
@@ -242,3 +242,15 @@ def test_destroy_geometry_before_fields():
     fl[0].close()
     for f in fl2:
         f.close()
+
+
+def test_autotuned_shape_gives_same_results():
+    case = wl.porous(300, 140, porosity=0.8, seed=12, rmin=4, rmax=8)
+    _, ref = gh.run_fused(case, 30)
+    g, fl = gh.make(case)
+    shape = g.autotune(2, nat.FP64)
+    assert shape == g.tile(2, nat.FP64)
+    assert shape[0] in (64, 128, 256, 512) and shape[2] in (1, 2)
+    nat.step2(g, fl[0], fl[1], case["force"][0], None, 30)
+    for c in range(2):
+        assert np.array_equal(fl[c].get_f(), ref[c].get_f())
