@@ -9,6 +9,7 @@
 // reference would hold).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -109,6 +110,7 @@ struct scl_geometry {
     int tile_cols = 0, tile_rows = 0;
     int lookahead = 0;
     int tuned[2][3][3] = {};            // [ncomp-1][precision] -> {cols, rows, lookahead}
+    bool open = false;                  // no solid site: the fused kernel skips the mask
     double* part = nullptr;             // moment partials (3 per row + 3)
     // Fields keep their geometry alive: destroying a geometry that fields
     // still use only marks it, and the last field destroyed frees it.
@@ -218,35 +220,41 @@ bool needs_wide(const scl_geometry* g, size_t esize) {
     return (unsigned long long)g->plane * Q * esize >= (1ull << 32);
 }
 
-template <int NC, class ST, int NW, bool GATHER, bool REPLAY, class OffT, int LA>
+template <int NC, class ST, int NW, bool GATHER, bool REPLAY, class OffT, int LA, bool OPEN>
 void launch_step_t(const StepArgs<typename ST::T>& a, int nblk) {
-    hipLaunchKernelGGL((k_step<NC, ST, NW, GATHER, REPLAY, OffT, LA>), dim3(nblk), dim3(64 * NW), 0, g_lib.stream,
-                       a);
+    hipLaunchKernelGGL((k_step<NC, ST, NW, GATHER, REPLAY, OffT, LA, OPEN>), dim3(nblk), dim3(64 * NW), 0,
+                       g_lib.stream, a);
 }
 
-template <int NC, class ST, bool GATHER, int LA>
+template <int NC, class ST, bool GATHER, int LA, bool OPEN>
 int launch_nw_la(int nw, const StepArgs<typename ST::T>& a, int nblk) {
     switch (nw) {
-        case 1: launch_step_t<NC, ST, 1, GATHER, false, uint32_t, LA>(a, nblk); break;
-        case 2: launch_step_t<NC, ST, 2, GATHER, false, uint32_t, LA>(a, nblk); break;
-        case 4: launch_step_t<NC, ST, 4, GATHER, false, uint32_t, LA>(a, nblk); break;
-        case 8: launch_step_t<NC, ST, 8, GATHER, false, uint32_t, LA>(a, nblk); break;
+        case 1: launch_step_t<NC, ST, 1, GATHER, false, uint32_t, LA, OPEN>(a, nblk); break;
+        case 2: launch_step_t<NC, ST, 2, GATHER, false, uint32_t, LA, OPEN>(a, nblk); break;
+        case 4: launch_step_t<NC, ST, 4, GATHER, false, uint32_t, LA, OPEN>(a, nblk); break;
+        case 8: launch_step_t<NC, ST, 8, GATHER, false, uint32_t, LA, OPEN>(a, nblk); break;
         default: return fail(SCL_ERR_ARG, "unsupported tile width %d", 64 * nw);
     }
     return SCL_OK;
 }
 
 // The first step after an upload (no gather) runs once, so it has one
-// shape per width; the gathering step also chooses the lookahead.
+// shape per width; the gathering step also chooses the lookahead and the
+// solid-free variant.
 template <int NC, class ST, bool GATHER>
-int launch_nw(int nw, int la, bool wide, const StepArgs<typename ST::T>& a, int nblk) {
+int launch_nw(int nw, int la, bool open, bool wide, const StepArgs<typename ST::T>& a, int nblk) {
     if (wide) {
         if (nw != 4) return fail(SCL_ERR_ARG, "lattices above 4 GiB per component use 256-column tiles");
-        launch_step_t<NC, ST, 4, GATHER, false, uint64_t, 1>(a, nblk);
+        launch_step_t<NC, ST, 4, GATHER, false, uint64_t, 1, false>(a, nblk);
         return SCL_OK;
     }
-    if (GATHER && la == 2) return launch_nw_la<NC, ST, GATHER, 2>(nw, a, nblk);
-    return launch_nw_la<NC, ST, GATHER, 1>(nw, a, nblk);
+    if (!GATHER) return launch_nw_la<NC, ST, GATHER, 1, false>(nw, a, nblk);
+    if (open) {
+        if (la == 2) return launch_nw_la<NC, ST, GATHER, 2, true>(nw, a, nblk);
+        return launch_nw_la<NC, ST, GATHER, 1, true>(nw, a, nblk);
+    }
+    if (la == 2) return launch_nw_la<NC, ST, GATHER, 2, false>(nw, a, nblk);
+    return launch_nw_la<NC, ST, GATHER, 1, false>(nw, a, nblk);
 }
 
 template <class T>
@@ -313,7 +321,8 @@ int launch_fused(const scl_geometry* g, scl_field* const* fl, const double* cons
     }
     const int nblk = nblocks(g, nw, rows);
     const int la = sh.la;
-    int rc = gather ? launch_nw<NC, ST, true>(nw, la, wide, a, nblk) : launch_nw<NC, ST, false>(nw, la, wide, a, nblk);
+    int rc = gather ? launch_nw<NC, ST, true>(nw, la, g->open, wide, a, nblk)
+                    : launch_nw<NC, ST, false>(nw, la, g->open, wide, a, nblk);
     if (rc != SCL_OK) return rc;
     HIPCHK(hipGetLastError());
     if (e0) {
@@ -340,11 +349,11 @@ int launch_fused(const scl_geometry* g, scl_field* const* fl, const double* cons
 template <int NC, bool GATHER>
 void replay_t(const Replay& r, int nblk, bool wide) {
     if (wide)
-        hipLaunchKernelGGL((k_step<NC, StoreF64, 4, GATHER, true, uint64_t, 1>), dim3(nblk), dim3(256), 0, g_lib.stream,
-                           r.args);
+        hipLaunchKernelGGL((k_step<NC, StoreF64, 4, GATHER, true, uint64_t, 1, false>), dim3(nblk), dim3(256), 0,
+                           g_lib.stream, r.args);
     else
-        hipLaunchKernelGGL((k_step<NC, StoreF64, 4, GATHER, true, uint32_t, 1>), dim3(nblk), dim3(256), 0, g_lib.stream,
-                           r.args);
+        hipLaunchKernelGGL((k_step<NC, StoreF64, 4, GATHER, true, uint32_t, 1, false>), dim3(nblk), dim3(256), 0,
+                           g_lib.stream, r.args);
 }
 
 int run_replay(Replay& r) {
@@ -517,6 +526,7 @@ int scl_geometry_create(int nx, int ny, const int* mask, scl_geometry** out) {
     g->wpr = g->pitch / 64;
     g->plane = (long long)g->pitch * ny;
     g->host_mask.assign(mask, mask + (size_t)nx * ny);
+    g->open = std::all_of(g->host_mask.begin(), g->host_mask.end(), [](int v) { return v == 1; });
     std::vector<uint64_t> bits((size_t)g->wpr * ny, 0ull);
     for (int y = 0; y < ny; ++y)
         for (int x = 0; x < nx; ++x)

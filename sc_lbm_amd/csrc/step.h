@@ -148,19 +148,40 @@ __device__ inline void feq_ref(double rho, double ux, double uy, double (&e)[Q])
     e[8] = r2 * (1.0 + 3.0 * a8 + 4.5 * a8 * a8 - 1.5 * usq);
 }
 
-// Equilibrium deviation e_i = feq_i - w_i*rho0 written with the density
-// deviation drho = rho - rho0 so that small perturbations keep their digits:
-// e_i = w_i*drho + w_i*rho*(3 cu + 4.5 cu^2 - 1.5 u^2).
-__device__ inline void feq_dev(float rho, float drho, float ux, float uy, float (&e)[Q]) {
+// BGK relaxation of one component in deviation form, FP32:
+//   d_i' = (1 - omega) d_i + omega (feq_i - w_i rho0) [+ F_i],
+//   feq_i - w_i rho0 = w_i drho + w_i rho (3 cu + 4.5 cu^2 - 1.5 u^2),
+// with drho = rho - rho0 so small perturbations keep their digits.  Opposite
+// directions share cu^2 (their polynomials differ only in the sign of the
+// 3 cu term) and omega is folded into the weights, so each direction costs
+// two fused multiply-adds.
+__device__ inline void relax_dev(const float (&p)[Q], float rho, float drho, float ux, float uy, float omega,
+                                 float omc, const float* force, bool has_force, float (&out)[Q]) {
+#pragma clang fp contract(fast)
     const float usq15 = 1.5f * (ux * ux + uy * uy);
+    const float ow0 = omega * (4.0f / 9.0f), ow1 = omega * (1.0f / 9.0f), ow2 = omega * (1.0f / 36.0f);
+    const float wd0 = ow0 * drho, wr0 = ow0 * rho;
+    const float wd1 = ow1 * drho, wr1 = ow1 * rho;
+    const float wd2 = ow2 * drho, wr2 = ow2 * rho;
+    out[0] = omc * p[0] + (wd0 - wr0 * usq15);
+    auto pair = [&](int i, int j, float cu, float wd, float wr) {
+        const float t = (4.5f * cu) * cu - usq15;
+        const float c3 = 3.0f * cu;
+        out[i] = omc * p[i] + (wd + wr * (t + c3));
+        out[j] = omc * p[j] + (wd + wr * (t - c3));
+    };
+    pair(1, 3, ux, wd1, wr1);
+    pair(2, 4, uy, wd1, wr1);
+    pair(5, 7, ux + uy, wd2, wr2);
+    pair(6, 8, uy - ux, wd2, wr2);
+    if (has_force) {
 #pragma unroll
-    for (int i = 0; i < Q; ++i) {
-        const float w = (float)weq(i);
-        const float cu = (float)cx(i) * ux + (float)cy(i) * uy;
-        const float poly = cu * (3.0f + 4.5f * cu) - usq15;
-        e[i] = w * drho + (w * rho) * poly;
+        for (int i = 0; i < Q; ++i) out[i] += force[i];
     }
 }
+
+// Reciprocal for the deviation forms (hardware approximation, 1 ulp).
+__device__ inline float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
 __device__ inline bool is_dry(double r) { return fabs(r) <= kZeroTol * fmax(1.0, fabs(r)); }
 
@@ -203,16 +224,11 @@ __device__ inline void finish1(const CompArgs<typename ST::T>& ca, long long pla
             const float rho = ca.rho0 + drho;
             float jx, jy;
             momentum(p, jx, jy);
-            const float inv = 1.0f / rho;
-            const float ux = jx * inv, uy = jy * inv;
-            float e[Q];
-            feq_dev(rho, drho, ux, uy, e);
+            const float inv = rcp(rho);
+            float out[Q];
+            relax_dev(p, rho, drho, jx * inv, jy * inv, ca.omega, ca.omc, ca.force, ca.has_force, out);
 #pragma unroll
-            for (int i = 0; i < Q; ++i) {
-                float v = ca.omc * p[i] + ca.omega * e[i];
-                if (ca.has_force) v += ca.force[i];
-                st_at<ST>(ca.dst, (OffT)i * plane_b + home, v);
-            }
+            for (int i = 0; i < Q; ++i) st_at<ST>(ca.dst, (OffT)i * plane_b + home, out[i]);
         }
     }
 }
@@ -328,6 +344,7 @@ __device__ inline void finish2(const StepArgs<typename ST::T>& a, OffT plane_b, 
     } else {
         // deviation forms, FP32 arithmetic
         if (!REPLAY && fl) {
+#pragma clang fp contract(fast)
             const auto& A = a.c[0];
             const auto& B = a.c[1];
             float sd[2][2] = {{0.f, 0.f}, {0.f, 0.f}};   // sums over fluid nbrs of w c drho_other
@@ -351,22 +368,23 @@ __device__ inline void finish2(const StepArgs<typename ST::T>& a, OffT plane_b, 
             const float da0 = rc[0], db0 = rc[1];
             const float ra = A.rho0 + da0, rb = B.rho0 + db0;
             // psi_other over fluid neighbours = sum w c drho + rho0_other * sum w c
-            const float fa_x = A.gneg * ra * (sd[0][0] + B.rho0 * sfx);
-            const float fa_y = A.gneg * ra * (sd[0][1] + B.rho0 * sfy);
-            const float fb_x = B.gneg * rb * (sd[1][0] + A.rho0 * sfx);
-            const float fb_y = B.gneg * rb * (sd[1][1] + A.rho0 * sfy);
+            const float ga = A.gneg * ra, gb = B.gneg * rb;
+            const float fa_x = ga * (sd[0][0] + B.rho0 * sfx);
+            const float fa_y = ga * (sd[0][1] + B.rho0 * sfy);
+            const float fb_x = gb * (sd[1][0] + A.rho0 * sfx);
+            const float fb_y = gb * (sd[1][1] + A.rho0 * sfy);
             // adhesion: sum over solid nbrs of w c = -(sum over fluid nbrs)
             const float fs_ax = A.gsneg * -sfx, fs_ay = A.gsneg * -sfy;
             const float fs_bx = B.gsneg * -sfx, fs_by = B.gsneg * -sfy;
             float jax, jay, jbx, jby;
             momentum(p[0], jax, jay);
             momentum(p[1], jbx, jby);
-            const float inv_den = 1.0f / (ra * A.omega + rb * B.omega);
+            const float inv_den = rcp(ra * A.omega + rb * B.omega);
             const float ucx = (jax * A.omega + jbx * B.omega) * inv_den;
             const float ucy = (jay * A.omega + jby * B.omega) * inv_den;
             float uea[2], ueb[2];
             if (!is_dry(ra)) {
-                const float ir = 1.0f / ra;
+                const float ir = rcp(ra);
                 uea[0] = ucx + A.inv_omega * (fa_x * ir + fs_ax);
                 uea[1] = ucy + A.inv_omega * (fa_y * ir + fs_ay);
                 if (fabsf(ra) < (float)kUeqWatch) {
@@ -378,7 +396,7 @@ __device__ inline void finish2(const StepArgs<typename ST::T>& a, OffT plane_b, 
                 uea[1] = (float)A.ueqy[s_idx];
             }
             if (!is_dry(rb)) {
-                const float ir = 1.0f / rb;
+                const float ir = rcp(rb);
                 ueb[0] = ucx + B.inv_omega * (fb_x * ir + fs_bx);
                 ueb[1] = ucy + B.inv_omega * (fb_y * ir + fs_by);
                 if (fabsf(rb) < (float)kUeqWatch) {
@@ -389,17 +407,13 @@ __device__ inline void finish2(const StepArgs<typename ST::T>& a, OffT plane_b, 
                 ueb[0] = (float)B.ueqx[s_idx];
                 ueb[1] = (float)B.ueqy[s_idx];
             }
-            float ea[Q], eb[Q];
-            feq_dev(ra, da0, uea[0], uea[1], ea);
-            feq_dev(rb, db0, ueb[0], ueb[1], eb);
+            float oa[Q], ob[Q];
+            relax_dev(p[0], ra, da0, uea[0], uea[1], A.omega, A.omc, A.force, A.has_force, oa);
+            relax_dev(p[1], rb, db0, ueb[0], ueb[1], B.omega, B.omc, B.force, B.has_force, ob);
 #pragma unroll
             for (int i = 0; i < Q; ++i) {
-                float va = A.omc * p[0][i] + A.omega * ea[i];
-                float vb = B.omc * p[1][i] + B.omega * eb[i];
-                if (A.has_force) va += A.force[i];
-                if (B.has_force) vb += B.force[i];
-                st_at<ST>(A.dst, (OffT)i * plane_b + home, va);
-                st_at<ST>(B.dst, (OffT)i * plane_b + home, vb);
+                st_at<ST>(A.dst, (OffT)i * plane_b + home, oa[i]);
+                st_at<ST>(B.dst, (OffT)i * plane_b + home, ob[i]);
             }
         }
     }
@@ -411,9 +425,11 @@ __device__ inline void finish2(const StepArgs<typename ST::T>& a, OffT plane_b, 
 // reference's streamed state and are read in place), REPLAY (recompute the
 // step and write the reference's intermediate arrays instead of P_n), OffT
 // the byte-offset type (32-bit while one component's buffer is < 4 GiB), LA
-// the row lookahead of the software pipeline (1 or 2).
+// the row lookahead of the software pipeline (1 or 2), OPEN a lattice with
+// no solid site (the mask is not read and every neighbourhood is fluid, so
+// the bounce-back selects and the adhesion sums fold away at compile time).
 // ---------------------------------------------------------------------------
-template <int NC, class ST, int NW, bool GATHER, bool REPLAY, class OffT, int LA>
+template <int NC, class ST, int NW, bool GATHER, bool REPLAY, class OffT, int LA, bool OPEN>
 __global__ __launch_bounds__(64 * NW) void k_step(const StepArgs<typename ST::T> a) {
     using T = typename ST::T;
     using S = typename ST::S;
@@ -426,6 +442,11 @@ __global__ __launch_bounds__(64 * NW) void k_step(const StepArgs<typename ST::T>
     const OffT plane_b = (OffT)a.plane * ES;
     const OffT pitch_b = (OffT)a.pitch * ES;
     const uint64_t* __restrict__ mask = a.mask;
+    // fluid bits (west, centre, east) of one row around a column
+    auto tri = [&](int row, int cm, int c, int cp) -> unsigned {
+        if constexpr (OPEN) return 7u;
+        else return triple(mask, wpr, row, cm, c, cp);
+    };
 
     // Workgroup -> strip.  Workgroups are dealt round-robin to the 8 XCDs;
     // consecutive ids of one XCD are given adjacent strips so the halo rows
@@ -505,20 +526,20 @@ __global__ __launch_bounds__(64 * NW) void k_step(const StepArgs<typename ST::T>
         unsigned tm2 = 0, tm1 = 0, t0 = 0, tp1 = 0, tp2 = 0, tp3 = 0;
         unsigned hm2 = 0, hm1 = 0, h0 = 0, hp1 = 0, hp2 = 0, hp3 = 0;
         if (xin) {
-            tm2 = triple(mask, wpr, r_m2, xm, x, xp);
-            tm1 = triple(mask, wpr, r_m1, xm, x, xp);
-            t0 = triple(mask, wpr, y0, xm, x, xp);
-            tp1 = triple(mask, wpr, r_p1, xm, x, xp);
-            tp2 = triple(mask, wpr, r_p2, xm, x, xp);
-            if (LA == 2) tp3 = triple(mask, wpr, r_p3, xm, x, xp);
+            tm2 = tri(r_m2, xm, x, xp);
+            tm1 = tri(r_m1, xm, x, xp);
+            t0 = tri(y0, xm, x, xp);
+            tp1 = tri(r_p1, xm, x, xp);
+            tp2 = tri(r_p2, xm, x, xp);
+            if (LA == 2) tp3 = tri(r_p3, xm, x, xp);
         }
         if (hrole) {
-            hm2 = triple(mask, wpr, r_m2, hcm, hcol, hcp);
-            hm1 = triple(mask, wpr, r_m1, hcm, hcol, hcp);
-            h0 = triple(mask, wpr, y0, hcm, hcol, hcp);
-            hp1 = triple(mask, wpr, r_p1, hcm, hcol, hcp);
-            hp2 = triple(mask, wpr, r_p2, hcm, hcol, hcp);
-            if (LA == 2) hp3 = triple(mask, wpr, r_p3, hcm, hcol, hcp);
+            hm2 = tri(r_m2, hcm, hcol, hcp);
+            hm1 = tri(r_m1, hcm, hcol, hcp);
+            h0 = tri(y0, hcm, hcol, hcp);
+            hp1 = tri(r_p1, hcm, hcol, hcp);
+            hp2 = tri(r_p2, hcm, hcol, hcp);
+            if (LA == 2) hp3 = tri(r_p3, hcm, hcol, hcp);
         }
         if constexpr (NC == 2) {
             if (xin) {
@@ -576,12 +597,12 @@ __global__ __launch_bounds__(64 * NW) void k_step(const StepArgs<typename ST::T>
             if (xin) {
                 t1 = t2;
                 t2 = t3;
-                t3 = triple(mask, wpr, rnext, xm, x, xp);
+                t3 = tri(rnext, xm, x, xp);
             }
             if (hrole) {
                 h1 = h2;
                 h2 = h3;
-                h3 = triple(mask, wpr, rnext, hcm, hcol, hcp);
+                h3 = tri(rnext, hcm, hcol, hcp);
             }
             if (LA == 1) {
                 if (xin) file_rho(pn, slotA, tid + 1);
