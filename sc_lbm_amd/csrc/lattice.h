@@ -46,12 +46,27 @@ constexpr float kHalfInv = 1.0f / 256.0f;
 // Storage forms.  T is the arithmetic type, S the stored type.  The
 // deviation forms store d_i = f_i - w_i*rho0.
 // ---------------------------------------------------------------------------
+// Population stores are streamed (non-temporal): a step's output is read
+// back only by the next step, after the whole lattice has passed through the
+// caches.
+#ifndef SCL_STREAM_STORES
+#define SCL_STREAM_STORES 1
+#endif
+template <class S>
+__device__ inline void stream_store(S* p, S v) {
+#if SCL_STREAM_STORES
+    __builtin_nontemporal_store(v, p);
+#else
+    *p = v;
+#endif
+}
+
 struct StoreF64 {
     using S = double;
     using T = double;
     static constexpr bool kDev = false;
     __device__ static inline T ld(const S* p) { return *p; }
-    __device__ static inline void st(S* p, T v) { *p = v; }
+    __device__ static inline void st(S* p, T v) { stream_store<S>(p, v); }
 };
 
 struct StoreF32 {
@@ -59,7 +74,7 @@ struct StoreF32 {
     using T = float;
     static constexpr bool kDev = true;
     __device__ static inline T ld(const S* p) { return *p; }
-    __device__ static inline void st(S* p, T v) { *p = v; }
+    __device__ static inline void st(S* p, T v) { stream_store<S>(p, v); }
 };
 
 struct StoreF16 {
@@ -67,7 +82,7 @@ struct StoreF16 {
     using T = float;
     static constexpr bool kDev = true;
     __device__ static inline T ld(const S* p) { return (float)(*p) * kHalfInv; }
-    __device__ static inline void st(S* p, T v) { *p = (_Float16)(v * kHalfScale); }
+    __device__ static inline void st(S* p, T v) { stream_store<S>(p, (_Float16)(v * kHalfScale)); }
 };
 
 // Wrap a column / row index that is at most one step outside [0, n).

@@ -319,11 +319,25 @@ __global__ void k_fill_rest(Grid2 g, typename ST::S* __restrict__ dst, double rh
     }
 }
 
-// Plain copy used to calibrate the HBM byte counters (16 B per lane).
-__global__ void k_probe_copy(const float4* __restrict__ src, float4* __restrict__ dst, size_t n) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (; i < n; i += stride) dst[i] = src[i];
+// Plain copy used to calibrate the HBM byte counters and to measure the
+// practical read+write bandwidth: each lane moves four 16-byte vectors that
+// are all in flight together; the grid covers the buffer once.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_probe_copy(const f32x4* __restrict__ src, f32x4* __restrict__ dst,
+                                                    size_t n) {
+    const size_t base = (size_t)blockIdx.x * 1024 + threadIdx.x;
+    f32x4 v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const size_t i = base + 256 * k;
+        if (i < n) v[k] = __builtin_nontemporal_load(src + i);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const size_t i = base + 256 * k;
+        if (i < n) __builtin_nontemporal_store(v[k], dst + i);
+    }
 }
 
 }  // namespace scl

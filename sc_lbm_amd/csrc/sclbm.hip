@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <type_traits>
@@ -1094,7 +1095,7 @@ int scl_probe_copy(size_t bytes, int reps, double* avg_ms) {
     if (!avg_ms || bytes < 16 || reps <= 0) return fail(SCL_ERR_ARG, "bad probe arguments");
     CHK(lib_ready());
     const size_t n = bytes / 16;
-    float4 *a = nullptr, *b = nullptr;
+    f32x4 *a = nullptr, *b = nullptr;
     CHK(dalloc(&a, n));
     int rc = dalloc(&b, n);
     if (rc != SCL_OK) {
@@ -1105,10 +1106,11 @@ int scl_probe_copy(size_t bytes, int reps, double* avg_ms) {
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipMemsetAsync(a, 0, n * 16, g_lib.stream));
-    hipLaunchKernelGGL(k_probe_copy, dim3(4096), dim3(256), 0, g_lib.stream, a, b, n);
+    const unsigned nblk = (unsigned)((n + 1023) / 1024);
+    hipLaunchKernelGGL(k_probe_copy, dim3(nblk), dim3(256), 0, g_lib.stream, a, b, n);
     HIPCHK(hipEventRecord(e0, g_lib.stream));
     for (int r = 0; r < reps; ++r)
-        hipLaunchKernelGGL(k_probe_copy, dim3(4096), dim3(256), 0, g_lib.stream, a, b, n);
+        hipLaunchKernelGGL(k_probe_copy, dim3(nblk), dim3(256), 0, g_lib.stream, a, b, n);
     HIPCHK(hipEventRecord(e1, g_lib.stream));
     HIPCHK(hipEventSynchronize(e1));
     float ms = 0.f;

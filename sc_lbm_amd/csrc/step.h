@@ -429,8 +429,16 @@ __device__ inline void finish2(const StepArgs<typename ST::T>& a, OffT plane_b, 
 // no solid site (the mask is not read and every neighbourhood is fluid, so
 // the bounce-back selects and the adhesion sums fold away at compile time).
 // ---------------------------------------------------------------------------
+// Occupancy request: the solid-free FP32-arithmetic kernels fit in 128
+// registers, i.e. four waves per SIMD.
+template <class ST, bool OPEN, int LA>
+constexpr int step_waves_per_eu() {
+    return (OPEN && LA == 1 && sizeof(typename ST::T) == 4) ? 4 : 1;
+}
+
 template <int NC, class ST, int NW, bool GATHER, bool REPLAY, class OffT, int LA, bool OPEN>
-__global__ __launch_bounds__(64 * NW) void k_step(const StepArgs<typename ST::T> a) {
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(step_waves_per_eu<ST, OPEN, LA>()))) void
+k_step(const StepArgs<typename ST::T> a) {
     using T = typename ST::T;
     using S = typename ST::S;
     constexpr int BW = 64 * NW;
@@ -442,10 +450,15 @@ __global__ __launch_bounds__(64 * NW) void k_step(const StepArgs<typename ST::T>
     const OffT plane_b = (OffT)a.plane * ES;
     const OffT pitch_b = (OffT)a.pitch * ES;
     const uint64_t* __restrict__ mask = a.mask;
-    // fluid bits (west, centre, east) of one row around a column
+    // fluid bits (west, centre, east) of one row around a column, and the
+    // neighbourhood of a site from three of them (all fluid in OPEN)
     auto tri = [&](int row, int cm, int c, int cp) -> unsigned {
         if constexpr (OPEN) return 7u;
         else return triple(mask, wpr, row, cm, c, cp);
+    };
+    auto neigh = [&](unsigned s, unsigned c, unsigned n) -> unsigned {
+        if constexpr (OPEN) return 0x1FFu;
+        else return neigh_bits(s, c, n);
     };
 
     // Workgroup -> strip.  Workgroups are dealt round-robin to the 8 XCDs;
@@ -543,26 +556,26 @@ __global__ __launch_bounds__(64 * NW) void k_step(const StepArgs<typename ST::T>
         }
         if constexpr (NC == 2) {
             if (xin) {
-                issue(x, xm, xp, r_m1, r_m2, y0, neigh_bits(tm2, tm1, t0), pC);
+                issue(x, xm, xp, r_m1, r_m2, y0, neigh(tm2, tm1, t0), pC);
                 file_rho(pC, 0, tid + 1);
             }
             if (hrole) {
-                issue(hcol, hcm, hcp, r_m1, r_m2, y0, neigh_bits(hm2, hm1, h0), ph);
+                issue(hcol, hcm, hcp, r_m1, r_m2, y0, neigh(hm2, hm1, h0), ph);
                 file_rho(ph, 0, hslot);
-                issue(hcol, hcm, hcp, y0, r_m1, r_p1, neigh_bits(hm1, h0, hp1), ph);
+                issue(hcol, hcm, hcp, y0, r_m1, r_p1, neigh(hm1, h0, hp1), ph);
                 file_rho(ph, 1, hslot);
                 if (LA == 2) {
-                    issue(hcol, hcm, hcp, r_p1, y0, r_p2, neigh_bits(h0, hp1, hp2), ph);
+                    issue(hcol, hcm, hcp, r_p1, y0, r_p2, neigh(h0, hp1, hp2), ph);
                     file_rho(ph, 2, hslot);
                 }
             }
         }
         if (xin) {
-            nbA = neigh_bits(tm1, t0, tp1);
+            nbA = neigh(tm1, t0, tp1);
             issue(x, xm, xp, y0, r_m1, r_p1, nbA, pA);
             file_rho(pA, 1, tid + 1);
             if (LA == 2 && (NC == 2 || y0 + 1 < y1)) {
-                nbB = neigh_bits(t0, tp1, tp2);
+                nbB = neigh(t0, tp1, tp2);
                 issue(x, xm, xp, r_p1, y0, r_p2, nbB, pB);
                 file_rho(pB, 2, tid + 1);
             }
@@ -585,11 +598,11 @@ __global__ __launch_bounds__(64 * NW) void k_step(const StepArgs<typename ST::T>
         const int slotA = (k + LA + 1) & 3;
         if (more) {
             if (xin) {
-                nbn = neigh_bits(t1, t2, t3);
+                nbn = neigh(t1, t2, t3);
                 issue(x, xm, xp, rA, rAm, rAp, nbn, pn);
             }
             if (hrole) {
-                hnb = neigh_bits(h1, h2, h3);
+                hnb = neigh(h1, h2, h3);
                 issue(hcol, hcm, hcp, rA, rAm, rAp, hnb, ph);
             }
             // mask row of the next issue
@@ -614,7 +627,8 @@ __global__ __launch_bounds__(64 * NW) void k_step(const StepArgs<typename ST::T>
             const int s_idx = y * a.pitch + x;
             const OffT home = (OffT)y * pitch_b + (OffT)x * ES;
             if constexpr (NC == 1) {
-                finish1<ST, REPLAY, OffT>(a.c[0], a.plane, plane_b, s_idx, home, p0[0], dens(p0[0]), nb0 & 1u);
+                finish1<ST, REPLAY, OffT>(a.c[0], a.plane, plane_b, s_idx, home, p0[0], dens(p0[0]),
+                                          OPEN || (nb0 & 1u));
             } else {
                 const int sS = k & 3, sC = (k + 1) & 3, sN = (k + 2) & 3;
                 T nr[2][Q], r0[2];
@@ -629,7 +643,7 @@ __global__ __launch_bounds__(64 * NW) void k_step(const StepArgs<typename ST::T>
                     r0[c] = lds[sC][c][tid + 1];
                     nr[c][0] = r0[c];
                 }
-                finish2<ST, REPLAY, OffT>(a, plane_b, s_idx, home, p0, r0, nr, nb0);
+                finish2<ST, REPLAY, OffT>(a, plane_b, s_idx, home, p0, r0, nr, OPEN ? 0x1FFu : nb0);
             }
         }
         if (LA == 2 && more) {

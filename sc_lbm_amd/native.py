@@ -9,7 +9,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libsclbm.so")
+# SCLBM_LIBRARY selects another build of the library (e.g. to compare builds)
+LIB_PATH = os.environ.get("SCLBM_LIBRARY") or os.path.join(HERE, "libsclbm.so")
 
 SCL_OK, SCL_ERR_ARG, SCL_ERR_HIP, SCL_ERR_RANGE, SCL_ERR_UNSUPPORTED = 0, 1, 2, 3, 4
 FP64, FP32_DEV, FP16_DEV = 0, 1, 2
