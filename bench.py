@@ -34,6 +34,13 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU baseline sample")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--tile", default="", help="cols,rows[,lookahead] override (default: autotuned)")
+    p.add_argument("--no-serve", action="store_true", help="skip the served job stream")
+    p.add_argument("--serve-jobs", type=int, default=2000)
+    p.add_argument("--serve-step-scale", type=float, default=0.05,
+                   help="multiplies the per-job step counts drawn from [2000, 50000]")
+    p.add_argument("--serve-load", type=float, default=0.8, help="offered load / measured server capacity")
+    p.add_argument("--serve-budget-gib", type=float, default=64.0)
+    p.add_argument("--serve-sample", type=int, default=20, help="every k-th job is rerun in FP64 (5%%)")
     return p.parse_args()
 
 
@@ -77,6 +84,74 @@ def time_form(nat, wl, case, form, steps, warmup, tile):
                 kernel_avg_ms=avg_ms, kernel_launches=cnt, tile=list(tile_used),
                 achieved_gbs=n_sites * BYTES_PER_LU[form] / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else None,
                 pct_hbm=100.0 * mlups * 1e6 * BYTES_PER_LU[form] / (HBM_PEAK_GBS * 1e9), mass=mass)
+
+
+def served_stream(nat, a):
+    """The served job stream in FP32 deviations: capacity probe, Poisson
+    arrivals at `serve_load` of it, p50/p99 completion latency, aggregate
+    MLUPS, and the max density error of a 5% sample against an FP64 rerun."""
+    from sc_lbm_amd import serve
+    from sc_lbm_amd import workloads as wl
+    jobs = wl.served_stream(a.serve_jobs, seed=2025, step_scale=a.serve_step_scale)
+    cfg = dict(precision=nat.FP32_DEV, budget_bytes=int(a.serve_budget_gib * (1 << 30)), max_jobs=256,
+               max_pending=max(4096, 2 * a.serve_jobs))
+    with serve.Server(**cfg) as srv:
+        st0 = srv.stats()
+        log("  server tile %d x %d, lookahead %d, %d workgroups per step"
+            % (st0["tile_cols"], st0["tile_rows"], st0["lookahead"], st0["workgroups"]))
+        # capacity: eight large porous jobs submitted together
+        probe = [dict(jobs[0], nx=2048, ny=2048, steps=200, geom_seed=91 + k, ic_seed=92 + k) for k in range(8)]
+        ids = [srv.submit(**p) for p in probe]
+        for i in ids:
+            srv.wait(i)
+        st1 = srv.stats()
+        cap = (st1["lattice_updates"] - st0["lattice_updates"]) / (st1["busy_ms"] - st0["busy_ms"]) / 1e3
+        for i in ids:
+            srv.release(i)
+        lus = np.array([j["nx"] * j["ny"] * j["steps"] for j in jobs], dtype=np.float64)
+        rate = a.serve_load * cap * 1e6 / lus.mean()
+        arr = wl.poisson_arrivals(len(jobs), rate, seed=11)
+        log("  capacity %.0f MLUPS; %d jobs at %.1f/s (offered load %.2f), mean %.3g LU/job"
+            % (cap, len(jobs), rate, a.serve_load, lus.mean()))
+        t0 = srv.stats()["clock_ms"] + 50.0
+        sample = set(range(0, len(jobs), a.serve_sample))
+        ids = []
+        for k, j in enumerate(jobs):
+            ids.append(srv.submit(release_ms=t0 + arr[k], deliver=serve.DELIVER_RHO if k in sample else 0, **j))
+        sts = []
+        for k, i in enumerate(ids):
+            sts.append(srv.wait(i))
+            if (k + 1) % 500 == 0:
+                log("  %d jobs finished" % (k + 1))
+        rho32 = {k: [srv.result(ids[k], serve.RHO, c) for c in range(2)]
+                 for k in sample if sts[k]["state"] == "done"}
+        st2 = srv.stats()
+    done = [k for k, s in enumerate(sts) if s["state"] == "done"]
+    lat = np.array([sts[k]["done_ms"] - sts[k]["release_ms"] for k in done])
+    first = min(s["release_ms"] for s in sts)
+    last = max(sts[k]["done_ms"] for k in done)
+    lu_done = float(lus[done].sum())
+    # FP64 rerun of the sample
+    errs = []
+    with serve.Server(precision=nat.FP64, budget_bytes=int(a.serve_budget_gib * (1 << 30)), max_jobs=256) as s64:
+        ids64 = {k: s64.submit(deliver=serve.DELIVER_RHO, **jobs[k]) for k in rho32}
+        for k, i in ids64.items():
+            st = s64.wait(i)
+            if st["state"] != "done":
+                continue
+            errs.append(max(float(np.max(np.abs(s64.result(i, serve.RHO, c) - rho32[k][c]))) for c in range(2)))
+    return dict(jobs=len(jobs), done=len(done), failed=sum(s["state"] == "failed" for s in sts),
+                step_scale=a.serve_step_scale, steps_range=[int(round(2000 * a.serve_step_scale)),
+                                                            int(round(50000 * a.serve_step_scale))],
+                offered_load=a.serve_load, capacity_mlups=cap, arrival_rate_per_s=rate,
+                p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
+                mean_ms=float(lat.mean()), max_ms=float(lat.max()),
+                aggregate_mlups=lu_done / (last - first) / 1e3, busy_mlups=st2["lattice_updates"] / st2["busy_ms"] / 1e3,
+                makespan_s=(last - first) / 1e3, max_resident=st2["max_resident"],
+                arena_peak_gib=st2["arena_peak"] / (1 << 30),
+                tile=[st2["tile_cols"], st2["tile_rows"], st2["lookahead"]], round_steps=st2["round_steps"],
+                sample_jobs=len(errs), sample_max_abs_rho_err=max(errs) if errs else None,
+                sample_median_abs_rho_err=float(np.median(errs)) if errs else None)
 
 
 def cpu_baseline(seconds):
@@ -160,6 +235,12 @@ def main():
     }
     if "fp64" in res and "fp32dev" in res:
         out["fp32dev_over_fp64"] = res["fp32dev"]["mlups"] / res["fp64"]["mlups"]
+    if not a.no_serve:
+        log("served job stream: %d jobs, step scale %g" % (a.serve_jobs, a.serve_step_scale))
+        out["served"] = served_stream(nat, a)
+        sv = out["served"]
+        log("  p50 %.1f ms, p99 %.1f ms, aggregate %.0f MLUPS, %d failed, sample max |drho| %s"
+            % (sv["p50_ms"], sv["p99_ms"], sv["aggregate_mlups"], sv["failed"], sv["sample_max_abs_rho_err"]))
     if not a.no_cpu:
         log("timing the CPU baseline (about %.0f s)" % a.cpu_seconds)
         out["cpu_baseline"] = cpu_baseline(a.cpu_seconds)

@@ -39,6 +39,8 @@ extern "C" {
 #define SCL_ERR_HIP 2        /* HIP runtime failure (no device, launch, memory)  */
 #define SCL_ERR_RANGE 3      /* the reference would throw std::out_of_range      */
 #define SCL_ERR_UNSUPPORTED 4/* operation not available for this precision       */
+#define SCL_ERR_BUSY 5       /* server queue full: back-pressure, retry later     */
+#define SCL_ERR_TIMEOUT 6    /* a wait ended before the job or server finished    */
 
 /* Population storage forms. */
 #define SCL_FP64 0           /* plain FP64, bit-exact with the reference          */
@@ -184,6 +186,125 @@ int scl_timing(int reset, double* avg_ms, int* count);
  * repeated `reps` times; avg_ms is the mean duration of one copy.  Used to
  * calibrate the HBM byte counters against a kernel of known traffic. */
 int scl_probe_copy(size_t bytes, int reps, double* avg_ms);
+
+/* ===========================================================================
+ * Serving layer (no reference counterpart; the reference runs one lattice
+ * per process).  A server keeps independent two-component Shan-Chen jobs
+ * resident in device memory and advances all of them together: each step of
+ * a round is one launch of a persistent kernel whose workgroups take tiles
+ * (job, strip of columns, band of rows) from a work-list, and the steps of a
+ * round are replayed from a captured HIP graph.  Requests are admitted
+ * continuously, between rounds, as job slots and arena memory free up.
+ * Per-site arithmetic is that of scl_step2 in the same precision, so a job's
+ * result does not depend on the other jobs it shares the device with.
+ * ======================================================================== */
+typedef struct scl_server scl_server;
+
+/* Generated initial conditions (densities; populations are rho/9 in every
+ * direction on fluid sites and 0 on solid sites, as the reference's
+ * initialisers, src/lbm.cpp:11-48).  Component B mirrors A (rho_hi <-> rho_lo). */
+#define SCL_IC_UNIFORM 0     /* A = rho_hi everywhere                                   */
+#define SCL_IC_DROPLET 1     /* A = rho_hi where (x-nx/2)^2+(y-ny/2)^2 <= radius^2      */
+#define SCL_IC_STRATIFIED 2  /* A = rho_hi on rows y < ny/2                             */
+
+/* Deliverables of a finished job (bit mask). */
+#define SCL_DELIVER_RHO 1    /* final densities, nx*ny per component                  */
+#define SCL_DELIVER_F 2      /* final populations (streamed state), 9*nx*ny per comp. */
+
+/* Job states. */
+#define SCL_JOB_QUEUED 0
+#define SCL_JOB_RUNNING 1
+#define SCL_JOB_DONE 2
+#define SCL_JOB_FAILED 3     /* setup failed, or the mass watchdog tripped            */
+#define SCL_JOB_CANCELLED 4
+#define SCL_JOB_TIMED_OUT 5
+
+typedef struct scl_job {
+    int nx, ny;                 /* lattice, 1..32767 each                             */
+    /* geometry: caller's mask (nx*ny, 1 = fluid), or NULL for seeded disks */
+    const int* mask;
+    double porosity;            /* target fluid fraction of the generated mask (1 = open) */
+    int rmin, rmax;             /* disk radii                                          */
+    int walls;                  /* 1: solid rows y = 0 and y = ny-1                    */
+    uint64_t geom_seed;
+    /* initial state: caller's populations (9*nx*ny per component, reference
+     * layout), or NULL for a generated initial condition */
+    const double* f[2];
+    int ic;                     /* SCL_IC_*                                            */
+    double rho_hi, rho_lo, radius;
+    double noise;               /* rho *= 1 + U[-noise, noise), seeded per site        */
+    uint64_t ic_seed;           /* component A uses ic_seed, B ic_seed + 1             */
+    /* physics (reference Fluid parameters, include/fluid.h:60-70,110-114) */
+    double tau[2], g_fluid[2], g_solid[2];
+    const double* force[2];     /* 9-vectors added on fluid sites each step, or NULL  */
+    long long steps;            /* reference time steps to run, >= 1                   */
+    double release_ms;          /* earliest admission, on the server clock            */
+    double timeout_ms;          /* > 0: dropped (TIMED_OUT) if not done this long after release */
+    int deliver;                /* SCL_DELIVER_* bits                                  */
+} scl_job;
+
+typedef struct scl_job_status {
+    int state;                  /* SCL_JOB_*                                           */
+    long long steps_done;
+    double release_ms, admit_ms, done_ms;   /* server clock; -1 if not reached        */
+    double mass[2];             /* FP64 mass per component at the last round          */
+    double mass0[2];            /* ... at the first round (watchdog reference)        */
+    size_t bytes;               /* device memory held while resident                  */
+    char error[160];
+} scl_job_status;
+
+typedef struct scl_server_config {
+    int precision;              /* SCL_FP64, SCL_FP32_DEV or SCL_FP16_DEV              */
+    size_t budget_bytes;        /* device arena for resident jobs (allocated once)    */
+    int max_jobs;               /* resident job slots                                  */
+    int max_pending;            /* queued requests before submit returns SCL_ERR_BUSY */
+    int round_steps;            /* steps per round (graph length), 1..1024            */
+    int tile_cols, tile_rows, lookahead;    /* persistent-kernel tile; 0 = autotuned  */
+    double mass_tol;            /* watchdog: |mass - mass0| > mass_tol*|mass0| fails a job;
+                                   0 = by precision (1e-6 FP64, 1e-3 FP32, 0.1 FP16) */
+    int max_bypass;             /* later jobs admitted past a waiting one before it blocks admission */
+} scl_server_config;
+
+typedef struct scl_server_stats {
+    long long submitted, admitted, done, failed, cancelled, timed_out;
+    long long rejected_busy;    /* submits refused by back-pressure                    */
+    long long rejected;         /* submits refused as invalid or larger than the arena */
+    long long rounds, job_steps, lattice_updates;
+    double busy_ms;             /* wall time spent in rounds                           */
+    double clock_ms;            /* server clock now                                    */
+    int resident, queued, max_resident;
+    size_t arena_bytes, arena_used, arena_peak;
+    int tile_cols, tile_rows, lookahead, round_steps, workgroups;
+} scl_server_stats;
+
+/* Defaults: FP32 deviations, 8 GiB arena, 64 slots, 4096 pending, rounds of
+ * 32 steps, autotuned tile, mass_tol by precision, max_bypass 16. */
+int scl_server_default_config(scl_server_config* cfg);
+/* Creates the server on the library's device and starts its request loop. */
+int scl_server_create(const scl_server_config* cfg, scl_server** out);
+/* Queues a job (copies everything it points to).  SCL_ERR_BUSY when
+ * max_pending requests are queued; SCL_ERR_RANGE when the job can never fit
+ * the arena; SCL_ERR_ARG for an invalid job. */
+int scl_server_submit(scl_server* s, const scl_job* job, long long* id);
+int scl_server_status(scl_server* s, long long id, scl_job_status* st);
+/* Waits until the job has finished (any final state); SCL_ERR_TIMEOUT if
+ * timeout_ms (< 0: forever) passes first. */
+int scl_server_wait(scl_server* s, long long id, double timeout_ms, scl_job_status* st);
+/* Queued jobs are dropped at once, resident ones at the next round boundary. */
+int scl_server_cancel(scl_server* s, long long id);
+/* Copies a deliverable of a finished job: which = SCL_RHO (nx*ny) or SCL_F
+ * (9*nx*ny), component 0 or 1. */
+int scl_server_result(scl_server* s, long long id, int which, int comp, double* out, size_t n);
+/* Forgets a finished job and its deliverables. */
+int scl_server_release(scl_server* s, long long id);
+/* Waits until no job is queued or resident. */
+int scl_server_drain(scl_server* s, double timeout_ms);
+int scl_server_get_stats(scl_server* s, scl_server_stats* st);
+/* Stops the loop (queued and resident jobs end CANCELLED) and frees the arena. */
+int scl_server_destroy(scl_server* s);
+
+/* Host helper (no device needed): the mask a job would use, nx*ny ints. */
+int scl_job_mask(const scl_job* job, int* mask, size_t n);
 
 #ifdef __cplusplus
 }

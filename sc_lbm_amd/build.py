@@ -30,7 +30,7 @@ def _run(cmd, cwd=None):
 
 HOST = os.path.join(HERE, "host")
 INCLUDE = os.path.join(ROOT, "include")
-CXX_FLAGS = ["-std=c++14", "-O2", "-fPIC", "-Wall", "-I" + INCLUDE, "-I" + HOST]
+CXX_FLAGS = ["-std=c++14", "-O2", "-fPIC", "-Wall", "-pthread", "-I" + INCLUDE, "-I" + HOST]
 
 
 def build_library():
@@ -38,8 +38,12 @@ def build_library():
     classes Geometry / Fluid / LBM (g++), linked into one shared library."""
     bdir = os.path.join(HERE, "_build")
     os.makedirs(bdir, exist_ok=True)
-    objs = [os.path.join(bdir, "sclbm.o")]
-    _run([HIPCC] + HIP_FLAGS + ["-c", "-o", objs[0], os.path.join(CSRC, "sclbm.hip")])
+    objs = []
+    for src in sorted(os.listdir(CSRC)):
+        if src.endswith(".hip"):
+            o = os.path.join(bdir, src[:-4] + ".o")
+            _run([HIPCC] + HIP_FLAGS + ["-c", "-o", o, os.path.join(CSRC, src)])
+            objs.append(o)
     for src in sorted(os.listdir(HOST)):
         if src.endswith(".cpp"):
             o = os.path.join(bdir, src[:-4] + ".o")
@@ -47,7 +51,7 @@ def build_library():
             objs.append(o)
     out = os.path.join(HERE, "libsclbm.so")
     tmp = out + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread"])
     os.replace(tmp, out)
     return out
 

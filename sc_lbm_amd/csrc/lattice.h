@@ -42,6 +42,36 @@ constexpr double kUeqWatch = 1e-8;
 constexpr float kHalfScale = 256.0f;
 constexpr float kHalfInv = 1.0f / 256.0f;
 
+// Global-memory view of a pointer.  A pointer that reaches a kernel through
+// memory (a job table) rather than as a kernel argument has no known address
+// space, and the compiler would access it with flat instructions, which also
+// wait on LDS traffic; every population, mask and per-site array access goes
+// through this cast so it is a global one.
+// (The qualifiers apply to the device compilation; the host pass only
+// parses device code.)
+#if defined(__HIP_DEVICE_COMPILE__)
+#define SCL_AS_GLOBAL __attribute__((address_space(1)))
+#define SCL_AS_CONST __attribute__((address_space(4)))
+#else
+#define SCL_AS_GLOBAL
+#define SCL_AS_CONST
+#endif
+template <class T>
+__device__ inline SCL_AS_GLOBAL T* G(T* p) {
+    return (SCL_AS_GLOBAL T*)p;
+}
+template <class T>
+__device__ inline const SCL_AS_GLOBAL T* G(const T* p) {
+    return (const SCL_AS_GLOBAL T*)p;
+}
+
+// Constant-memory view of a table that no thread writes during the kernel
+// (uniform reads of it become scalar loads).
+template <class T>
+__device__ inline const SCL_AS_CONST T* K(const T* p) {
+    return (const SCL_AS_CONST T*)p;
+}
+
 // ---------------------------------------------------------------------------
 // Storage forms.  T is the arithmetic type, S the stored type.  The
 // deviation forms store d_i = f_i - w_i*rho0.
@@ -55,9 +85,9 @@ constexpr float kHalfInv = 1.0f / 256.0f;
 template <class S>
 __device__ inline void stream_store(S* p, S v) {
 #if SCL_STREAM_STORES
-    __builtin_nontemporal_store(v, p);
+    __builtin_nontemporal_store(v, G(p));
 #else
-    *p = v;
+    *G(p) = v;
 #endif
 }
 
@@ -65,7 +95,7 @@ struct StoreF64 {
     using S = double;
     using T = double;
     static constexpr bool kDev = false;
-    __device__ static inline T ld(const S* p) { return *p; }
+    __device__ static inline T ld(const S* p) { return *G(p); }
     __device__ static inline void st(S* p, T v) { stream_store<S>(p, v); }
 };
 
@@ -73,7 +103,7 @@ struct StoreF32 {
     using S = float;
     using T = float;
     static constexpr bool kDev = true;
-    __device__ static inline T ld(const S* p) { return *p; }
+    __device__ static inline T ld(const S* p) { return *G(p); }
     __device__ static inline void st(S* p, T v) { stream_store<S>(p, v); }
 };
 
@@ -81,7 +111,7 @@ struct StoreF16 {
     using S = _Float16;
     using T = float;
     static constexpr bool kDev = true;
-    __device__ static inline T ld(const S* p) { return (float)(*p) * kHalfInv; }
+    __device__ static inline T ld(const S* p) { return (float)(*G(p)) * kHalfInv; }
     __device__ static inline void st(S* p, T v) { stream_store<S>(p, (_Float16)(v * kHalfScale)); }
 };
 
@@ -89,7 +119,7 @@ struct StoreF16 {
 __device__ inline int wrap1(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
 
 __device__ inline int fluid_bit(const uint64_t* __restrict__ mask, int wpr, int row, int col) {
-    return (int)((mask[(size_t)row * wpr + (col >> 6)] >> (col & 63)) & 1ull);
+    return (int)((G(mask)[(size_t)row * wpr + (col >> 6)] >> (col & 63)) & 1ull);
 }
 
 // Bits (col-1, col, col+1) of one row: bit0 west, bit1 centre, bit2 east.

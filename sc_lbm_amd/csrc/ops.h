@@ -7,30 +7,11 @@
 // step sequence.  They work on the reference's streamed state S in FP64 and
 // reproduce each method's arithmetic as written (-ffp-contract=off).
 #pragma once
+#include "convert.h"
 #include "lattice.h"
 #include "step.h"
 
 namespace scl {
-
-struct Grid2 {
-    int nx, ny, pitch, wpr;
-    long long plane;
-    const uint64_t* mask;
-};
-
-__device__ inline bool site_of(const Grid2& g, int& x, int& y, int& s) {
-    x = blockIdx.x * blockDim.x + threadIdx.x;
-    y = blockIdx.y;
-    s = y * g.pitch + x;
-    return x < g.nx;
-}
-
-__device__ inline unsigned nbits_at(const Grid2& g, int x, int y) {
-    const int xm = wrap1(x - 1, g.nx), xp = wrap1(x + 1, g.nx);
-    const int ym = wrap1(y - 1, g.ny), yp = wrap1(y + 1, g.ny);
-    return neigh_bits(triple(g.mask, g.wpr, ym, xm, x, xp), triple(g.mask, g.wpr, y, xm, x, xp),
-                      triple(g.mask, g.wpr, yp, xm, x, xp));
-}
 
 // Fluid::compute_density, src/fluid.cpp:175-183 (every site).
 __global__ void k_density(Grid2 g, const double* __restrict__ f, double* __restrict__ rho) {
@@ -190,63 +171,6 @@ __global__ void k_volume_force(Grid2 g, double* __restrict__ f, Force9 F) {
     if (!fluid_bit(g.mask, g.wpr, y, x)) return;
 #pragma unroll
     for (int i = 0; i < Q; ++i) f[i * g.plane + s] += F.v[i];
-}
-
-// Stream with link bounce-back as a pull (src/lbm.cpp:343-435 seen from the
-// receiving site); solid sites become 0.  Works for any storage form (the
-// deviation forms store 0 at solid sites, which are never read).
-template <class ST>
-__global__ void k_gather(Grid2 g, const typename ST::S* __restrict__ src, typename ST::S* __restrict__ dst) {
-    int x, y, s;
-    if (!site_of(g, x, y, s)) return;
-    const unsigned nb = nbits_at(g, x, y);
-    typename ST::T p[Q];
-    if (nb & 1u) {
-        load_site<ST, true>(src, g.plane, g.pitch, x, wrap1(x - 1, g.nx), wrap1(x + 1, g.nx), y,
-                            wrap1(y - 1, g.ny), wrap1(y + 1, g.ny), nb, p);
-    } else {
-#pragma unroll
-        for (int i = 0; i < Q; ++i) p[i] = typename ST::T(0);
-    }
-#pragma unroll
-    for (int i = 0; i < Q; ++i) ST::st(dst + i * g.plane + s, p[i]);
-}
-
-// Storage -> FP64 populations (pitched), with an optional gather (phase P
-// -> S).  Deviation forms add w_i*rho0 back; solid sites come out as 0 when
-// gathering (the reference zero-fills them in stream).
-template <class ST, bool GATHER>
-__global__ void k_to_f64(Grid2 g, const typename ST::S* __restrict__ src, double rho0, double* __restrict__ out) {
-    int x, y, s;
-    if (!site_of(g, x, y, s)) return;
-    const unsigned nb = nbits_at(g, x, y);
-    typename ST::T p[Q];
-    const bool fl = nb & 1u;
-    if (!GATHER || fl) {
-        load_site<ST, GATHER>(src, g.plane, g.pitch, x, wrap1(x - 1, g.nx), wrap1(x + 1, g.nx), y,
-                              wrap1(y - 1, g.ny), wrap1(y + 1, g.ny), nb, p);
-    }
-#pragma unroll
-    for (int i = 0; i < Q; ++i) {
-        double v;
-        if (GATHER && !fl) v = 0.0;
-        else if (ST::kDev) v = (double)p[i] + weq(i) * rho0;
-        else v = (double)p[i];
-        out[i * g.plane + s] = v;
-    }
-}
-
-// FP64 populations -> storage form.
-template <class ST>
-__global__ void k_from_f64(Grid2 g, const double* __restrict__ in, double rho0, typename ST::S* __restrict__ dst) {
-    int x, y, s;
-    if (!site_of(g, x, y, s)) return;
-#pragma unroll
-    for (int i = 0; i < Q; ++i) {
-        const double v = in[i * g.plane + s];
-        if (ST::kDev) ST::st(dst + i * g.plane + s, (typename ST::T)(v - weq(i) * rho0));
-        else ST::st(dst + i * g.plane + s, (typename ST::T)v);
-    }
 }
 
 // FP64 mass and momentum of FP64 populations (pitched): one partial triple

@@ -20,8 +20,10 @@
 #include <vector>
 
 #include "../../include/sc_lbm_amd.h"
+#include "../host/serve_core.h"
 #include "lattice.h"
 #include "ops.h"
+#include "runtime.h"
 #include "step.h"
 
 using namespace scl;
@@ -493,6 +495,17 @@ int fused_steps(const scl_geometry* g, scl_field** fl, const double* const* forc
 }
 
 }  // namespace
+
+namespace scl {
+int runtime_ready() { return lib_ready(); }
+int runtime_device() { return g_lib.device; }
+namespace serve {
+int report(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+}  // namespace serve
+}  // namespace scl
 
 // ===========================================================================
 // C-ABI

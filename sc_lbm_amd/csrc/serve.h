@@ -1,0 +1,242 @@
+// serve.h -- device side of the serving layer: the persistent multi-job step
+// kernel over a tile work-list, the per-job mass reductions of the watchdog,
+// and the per-job set-up and delivery kernels.
+//
+// Work-list.  A tile is (job, strip of 64*NW columns, band of rows [y0, y1)).
+// Every step of a round is one launch of k_serve with a fixed grid (the
+// workgroups that fit on the device at once); each workgroup takes tiles in
+// order from a ticket counter until the list is exhausted, so small and
+// large jobs share the device without per-job launches.  A tile is computed
+// by step_tile, the body of the single-lattice fused kernel, so a job's
+// populations are the same bits whichever tiles, workgroups or other jobs
+// share its steps.
+//
+// The ticket counter resets itself: after its last fetch each workgroup
+// increments a second counter, and the workgroup that completes it zeroes
+// both, so consecutive launches (and replays of a captured graph) start from
+// zero without a memset.
+#pragma once
+#include "../host/serve_ic.h"
+#include "convert.h"
+#include "step.h"
+
+namespace scl {
+
+// One resident job as the kernels see it.  a.c[c].src / a.c[c].dst are
+// buffers 0 / 1 of component c; `cur` names the buffer that holds the job's
+// post-collision populations at the start of the round, so step s reads
+// buffer cur ^ (s & 1).
+template <class T>
+struct JobArgs {
+    StepArgs<T> a;
+    int cur;
+    int steps;                  // steps of this job in the current round
+    int item_begin, item_end;   // its tiles in the work-list
+};
+
+struct ServeHdr {
+    const void* jobs;           // JobArgs<T>[n_jobs]
+    const uint64_t* items;      // work-list
+    double* partial;            // 2 per tile (mass per component)
+    double* mass;               // 2 per job
+    int n_items, n_jobs;
+    int bw;                     // tile width in columns
+};
+
+// Tile encoding, 16 bits each from the low end: job, strip, y0, y1.
+__host__ __device__ inline uint64_t pack_item(int job, int bx, int y0, int y1) {
+    return (uint64_t)(unsigned)job | ((uint64_t)(unsigned)bx << 16) | ((uint64_t)(unsigned)y0 << 32) |
+           ((uint64_t)(unsigned)y1 << 48);
+}
+
+__device__ inline void unpack_item(uint64_t it, int& job, int& bx, int& y0, int& y1) {
+    job = (int)(it & 0xFFFFu);
+    bx = (int)((it >> 16) & 0xFFFFu);
+    y0 = (int)((it >> 32) & 0xFFFFu);
+    y1 = (int)(it >> 48);
+}
+
+template <class ST, int NW, int LA>
+__global__ __launch_bounds__(64 * NW) void k_serve(const ServeHdr* __restrict__ hdr, int s,
+                                                   unsigned* __restrict__ ctr) {
+    using T = typename ST::T;
+    __shared__ DensRing<2, T, NW> lds;
+    __shared__ int ticket;
+    const int n = hdr->n_items;
+    const SCL_AS_CONST JobArgs<T>* jobs = K((const JobArgs<T>*)hdr->jobs);
+    const SCL_AS_CONST uint64_t* items = K(hdr->items);
+    for (;;) {
+        __syncthreads();   // the previous tile is done with `ticket` and the ring
+        if (threadIdx.x == 0)
+            ticket = (int)__hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        const int t = __builtin_amdgcn_readfirstlane(ticket);
+        if (t >= n) break;
+        int j, bx, y0, y1;
+        unpack_item(items[t], j, bx, y0, y1);
+        j = __builtin_amdgcn_readfirstlane(j);
+        bx = __builtin_amdgcn_readfirstlane(bx);
+        y0 = __builtin_amdgcn_readfirstlane(y0);
+        y1 = __builtin_amdgcn_readfirstlane(y1);
+        const SCL_AS_CONST JobArgs<T>& J = jobs[j];
+        if (s >= J.steps) continue;
+        const StepArgs<T> a = J.a;
+        Bufs b;
+        const int sw = (J.cur ^ s) & 1;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            b.src[c] = sw ? a.c[c].dst : a.c[c].src;
+            b.dst[c] = sw ? const_cast<void*>(a.c[c].src) : a.c[c].dst;
+        }
+        step_tile<2, ST, NW, true, false, uint32_t, LA, false>(a, b, bx, y0, y1, lds);
+    }
+    if (threadIdx.x == 0) {
+        // every fetch of this workgroup precedes this release
+        const unsigned d = __hip_atomic_fetch_add(ctr + 1, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (d == gridDim.x - 1) {
+            __hip_atomic_store(ctr, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(ctr + 1, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+// Sum of two doubles per thread over a 256-thread workgroup, fixed order.
+__device__ inline void block_sum2(double& a, double& b, double (&red)[2][256]) {
+    const int t = threadIdx.x;
+    red[0][t] = a;
+    red[1][t] = b;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+            red[0][t] += red[0][t + w];
+            red[1][t] += red[1][t + w];
+        }
+        __syncthreads();
+    }
+    a = red[0][0];
+    b = red[1][0];
+}
+
+// Watchdog mass: FP64 sum over the fluid sites of each tile of the
+// post-collision populations the round ended with (collision and the
+// force vectors conserve each site's mass and the gather permutes slots, so
+// this equals the mass of the streamed state up to rounding).
+template <class ST>
+__global__ __launch_bounds__(256) void k_serve_mass(const ServeHdr* __restrict__ hdr) {
+    using T = typename ST::T;
+    __shared__ double red[2][256];
+    const int n = hdr->n_items, bw = hdr->bw;
+    const JobArgs<T>* __restrict__ jobs = (const JobArgs<T>*)hdr->jobs;
+    for (int t = blockIdx.x; t < n; t += gridDim.x) {
+        int j, bx, y0, y1;
+        unpack_item(hdr->items[t], j, bx, y0, y1);
+        const JobArgs<T>& J = jobs[j];
+        const int fin = J.cur ^ (J.steps & 1);
+        const typename ST::S* P[2];
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+            P[c] = (const typename ST::S*)(fin ? (const void*)J.a.c[c].dst : J.a.c[c].src);
+        const StepArgs<T>& a = J.a;
+        const int x0 = bx * bw, w = min(bw, a.nx - x0);
+        double m[2] = {0.0, 0.0};
+        for (int k = threadIdx.x; k < w * (y1 - y0); k += 256) {
+            const int x = x0 + k % w, y = y0 + k / w;
+            if (!fluid_bit(a.mask, a.wpr, y, x)) continue;
+            const long long s_idx = (long long)y * a.pitch + x;
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                double r = ST::kDev ? (double)a.c[c].rho0 : 0.0;
+#pragma unroll
+                for (int i = 0; i < Q; ++i) r += (double)ST::ld(P[c] + i * a.plane + s_idx);
+                m[c] += r;
+            }
+        }
+        block_sum2(m[0], m[1], red);
+        if (threadIdx.x == 0) {
+            hdr->partial[2 * t] = m[0];
+            hdr->partial[2 * t + 1] = m[1];
+        }
+        __syncthreads();
+    }
+}
+
+// Per-job sums of the tile partials, in tile order.
+template <class T>
+__global__ __launch_bounds__(256) void k_serve_mass_final(const ServeHdr* __restrict__ hdr) {
+    __shared__ double red[2][256];
+    const int j = blockIdx.x;
+    if (j >= hdr->n_jobs) return;
+    const JobArgs<T>& J = ((const JobArgs<T>*)hdr->jobs)[j];
+    double m0 = 0.0, m1 = 0.0;
+    for (int t = J.item_begin + threadIdx.x; t < J.item_end; t += 256) {
+        m0 += hdr->partial[2 * t];
+        m1 += hdr->partial[2 * t + 1];
+    }
+    block_sum2(m0, m1, red);
+    if (threadIdx.x == 0) {
+        hdr->mass[2 * j] = m0;
+        hdr->mass[2 * j + 1] = m1;
+    }
+}
+
+// Generated initial state S_0 of one component in the storage form:
+// f_i = rho/9 on fluid sites (the reference initialisers, src/lbm.cpp:33-46,
+// src/fluid.cpp:96-107), 0 on solid sites; deviation forms store
+// f_i - w_i*rho0 converted as k_from_f64 does.
+template <class ST>
+__global__ void k_init_ic(Grid2 g, serve::IcParams p, int c, double rho0, typename ST::S* __restrict__ dst) {
+    int x, y, s;
+    if (!site_of(g, x, y, s)) return;
+    const bool fl = fluid_bit(g.mask, g.wpr, y, x);
+    const double fi = fl ? serve::ic_density_at(p, c, x, y) / 9 : 0.0;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        if (ST::kDev) ST::st(dst + i * g.plane + s, (typename ST::T)(fi - weq(i) * rho0));
+        else ST::st(dst + i * g.plane + s, (typename ST::T)fi);
+    }
+}
+
+// Inverse of the gather: post-collision populations P whose gather is the
+// streamed state S.  Slot i of fluid site z is read by site z + c_i when that
+// site is fluid, and by z itself as direction opp(i) otherwise, so
+//   P_i(z) = S_i(z + c_i)   if z + c_i is fluid,
+//          = S_opp(i)(z)    otherwise.
+// Pure copies: Gather(Scatter(S)) == S bit for bit on fluid sites.
+template <class ST>
+__global__ void k_scatter(Grid2 g, const typename ST::S* __restrict__ src, typename ST::S* __restrict__ dst) {
+    int x, y, s;
+    if (!site_of(g, x, y, s)) return;
+    const unsigned nb = nbits_at(g, x, y);
+    if (!(nb & 1u)) return;
+    const int xm = wrap1(x - 1, g.nx), xp = wrap1(x + 1, g.nx);
+    const int ym = wrap1(y - 1, g.ny), yp = wrap1(y + 1, g.ny);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        const int sx = cx(i) == 1 ? xp : (cx(i) == -1 ? xm : x);
+        const int sy = cy(i) == 1 ? yp : (cy(i) == -1 ? ym : y);
+        const long long from = ((nb >> i) & 1u) ? (long long)i * g.plane + (long long)sy * g.pitch + sx
+                                                : (long long)opp(i) * g.plane + s;
+        dst[i * g.plane + s] = src[from];
+    }
+}
+
+// Density of the streamed state Gather(P) in FP64, reference order
+// (src/fluid.cpp:177-182) over the FP64 values of the populations; 0 on
+// solid sites.
+template <class ST>
+__global__ void k_job_density(Grid2 g, const typename ST::S* __restrict__ P, double rho0, double* __restrict__ rho) {
+    int x, y, s;
+    if (!site_of(g, x, y, s)) return;
+    const unsigned nb = nbits_at(g, x, y);
+    double r = 0.0;
+    if (nb & 1u) {
+        typename ST::T p[Q];
+        load_site<ST, true>(P, g.plane, g.pitch, x, wrap1(x - 1, g.nx), wrap1(x + 1, g.nx), y, wrap1(y - 1, g.ny),
+                            wrap1(y + 1, g.ny), nb, p);
+#pragma unroll
+        for (int i = 0; i < Q; ++i) r += ST::kDev ? (double)p[i] + weq(i) * rho0 : (double)p[i];
+    }
+    rho[s] = r;
+}
+
+}  // namespace scl

@@ -1,0 +1,604 @@
+// serve.hip -- gfx950 backend of the serving layer (the scheduler is in
+// host/serve_core.cpp).
+//
+// Memory.  One device arena of budget_bytes is allocated when the server is
+// created; each resident job owns one block of it (see Layout): its bit mask,
+// two population buffers per component in the server's storage form, the
+// FP64 equilibrium-velocity arrays of the zero-density rule, FP64 density
+// outputs, and an FP64 staging area when the job brings its own populations
+// or asks for them back.
+//
+// Admission builds the job's streamed initial state S_0 in buffer 0 and
+// scatters it into buffer 1 (P with Gather(P) = S_0), so every step of every
+// job is the same gathering kernel.
+//
+// A round of R steps is a captured graph: R launches of k_serve over the
+// work-list of all resident jobs, then the two mass reductions.  The graph
+// reads everything that changes between rounds (job table, work-list, their
+// sizes) through a device header that is refreshed before each launch, so a
+// graph is captured once per round length and replayed for the rest of the
+// server's life.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../host/serve_core.h"
+#include "convert.h"
+#include "runtime.h"
+#include "serve.h"
+#include "step.h"
+
+using namespace scl;
+using serve::Job;
+using serve::JobSpec;
+
+namespace {
+
+const char* hip_msg(hipError_t e) { return hipGetErrorString(e); }
+
+#define SHIP(expr)                                                                     \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess) {                                                        \
+            *err = std::string(#expr) + " failed: " + hip_msg(e_);                     \
+            return SCL_ERR_HIP;                                                        \
+        }                                                                              \
+    } while (0)
+
+size_t align256(size_t n) { return (n + 255) / 256 * 256; }
+
+// Placement of one job inside its arena block (byte offsets).
+struct Layout {
+    int pitch = 0, wpr = 0;
+    long long plane = 0;
+    size_t mask = 0, f[2][2] = {}, ueq[2][2] = {}, rho[2] = {}, stage = 0, total = 0;
+    bool has_stage = false;
+};
+
+Layout layout_of(const JobSpec& s, size_t es) {
+    Layout L;
+    L.pitch = (s.nx + 63) / 64 * 64;
+    L.wpr = L.pitch / 64;
+    L.plane = (long long)L.pitch * s.ny;
+    size_t off = 0;
+    auto take = [&](size_t n) {
+        const size_t o = off;
+        off += align256(n);
+        return o;
+    };
+    L.mask = take((size_t)s.ny * L.wpr * 8);
+    for (int c = 0; c < 2; ++c)
+        for (int b = 0; b < 2; ++b) L.f[c][b] = take((size_t)Q * L.plane * es);
+    for (int c = 0; c < 2; ++c)
+        for (int d = 0; d < 2; ++d) L.ueq[c][d] = take((size_t)L.plane * 8);
+    for (int c = 0; c < 2; ++c) L.rho[c] = take((size_t)L.plane * 8);
+    L.has_stage = !s.f[0].empty() || (s.deliver & SCL_DELIVER_F);
+    if (L.has_stage) L.stage = take((size_t)Q * L.plane * 8);
+    L.total = off;
+    return L;
+}
+
+// Device state of one resident job.
+struct GJob {
+    Layout L;
+    char* base = nullptr;
+    int cur = 1;   // buffer holding the post-collision populations
+    Grid2 grid() const {
+        return Grid2{nx, ny, L.pitch, L.wpr, L.plane, (const uint64_t*)(base + L.mask)};
+    }
+    void* f(int c, int b) const { return base + L.f[c][b]; }
+    double* ueq(int c, int d) const { return (double*)(base + L.ueq[c][d]); }
+    double* rho(int c) const { return (double*)(base + L.rho[c]); }
+    double* stage() const { return (double*)(base + L.stage); }
+    int nx = 0, ny = 0;
+};
+
+dim3 site_grid(int nx, int ny) { return dim3((nx + 255) / 256, ny); }
+
+template <class ST>
+class GpuServe final : public serve::Backend {
+    using T = typename ST::T;
+    using S = typename ST::S;
+
+  public:
+    explicit GpuServe(const scl_server_config& cfg) : cfg_(cfg) {}
+    ~GpuServe() override {
+        (void)hipSetDevice(dev_);
+        if (stream_) (void)hipStreamSynchronize(stream_);
+        for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
+        for (void* p : {(void*)arena_, (void*)hdr_d_, (void*)ctr_d_, table_d_, (void*)items_d_, (void*)partial_d_,
+                        (void*)mass_d_})
+            if (p) (void)hipFree(p);
+        for (void* p : {(void*)hdr_h_, table_h_, (void*)items_h_, (void*)mass_h_})
+            if (p) (void)hipHostFree(p);
+        if (stream_) (void)hipStreamDestroy(stream_);
+    }
+
+    int init(std::string* err) {
+        dev_ = runtime_device();
+        SHIP(hipSetDevice(dev_));
+        hipDeviceProp_t prop;
+        SHIP(hipGetDeviceProperties(&prop, dev_));
+        cus_ = prop.multiProcessorCount;
+        SHIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));   // default priority
+        if (cfg_.budget_bytes) {
+            hipError_t e = hipMalloc(&arena_, cfg_.budget_bytes);
+            if (e != hipSuccess) {
+                *err = "cannot allocate a device arena of " + std::to_string(cfg_.budget_bytes) +
+                       " bytes: " + hip_msg(e);
+                return SCL_ERR_HIP;
+            }
+        }
+        SHIP(hipMalloc(&hdr_d_, sizeof(ServeHdr)));
+        SHIP(hipMalloc(&ctr_d_, 2 * sizeof(unsigned)));
+        SHIP(hipMemsetAsync(ctr_d_, 0, 2 * sizeof(unsigned), stream_));
+        SHIP(hipMalloc(&table_d_, (size_t)cfg_.max_jobs * sizeof(JobArgs<T>)));
+        SHIP(hipMalloc(&mass_d_, (size_t)cfg_.max_jobs * 2 * sizeof(double)));
+        SHIP(hipHostMalloc(&hdr_h_, sizeof(ServeHdr), hipHostMallocDefault));
+        SHIP(hipHostMalloc(&table_h_, (size_t)cfg_.max_jobs * sizeof(JobArgs<T>), hipHostMallocDefault));
+        SHIP(hipHostMalloc(&mass_h_, (size_t)cfg_.max_jobs * 2 * sizeof(double), hipHostMallocDefault));
+        int rc = grow_items(4096, err);
+        if (rc != SCL_OK) return rc;
+        mass_grid_ = 4 * cus_;
+        if (cfg_.tile_cols) {
+            nw_ = cfg_.tile_cols / 64;
+            rows_ = cfg_.tile_rows ? cfg_.tile_rows : 32;
+            la_ = cfg_.lookahead ? cfg_.lookahead : 1;
+            if ((nw_ != 2 && nw_ != 4) || nw_ * 64 != cfg_.tile_cols) {
+                *err = "server tiles are 128 or 256 columns wide";
+                return SCL_ERR_ARG;
+            }
+            if (la_ != 1 && la_ != 2) {
+                *err = "lookahead is 1 or 2";
+                return SCL_ERR_ARG;
+            }
+            if (rows_ < 1 || rows_ > 4096) {
+                *err = "tile rows must be in 1..4096";
+                return SCL_ERR_ARG;
+            }
+        } else {
+            rc = autotune(err);
+            if (rc != SCL_OK) return rc;
+        }
+        rc = occupancy(nw_, la_, &grid_, err);
+        if (rc != SCL_OK) return rc;
+        SHIP(hipStreamSynchronize(stream_));
+        return SCL_OK;
+    }
+
+    int validate(const JobSpec& s, std::string* err) const override {
+        const Layout L = layout_of(s, sizeof(S));
+        if ((unsigned long long)Q * L.plane * sizeof(S) >= (1ull << 32)) {
+            *err = "lattice too large for the server (one component's populations must stay below 4 GiB)";
+            return SCL_ERR_RANGE;
+        }
+        return SCL_OK;
+    }
+
+    size_t footprint(const JobSpec& s) const override { return layout_of(s, sizeof(S)).total; }
+
+    int setup(Job& j, std::string* err) override {
+        SHIP(hipSetDevice(dev_));
+        return build(j, arena_ + j.off, err);
+    }
+
+    int run_round(const std::vector<Job*>& jobs, const std::vector<int>& steps, int rsteps,
+                  std::string* err) override {
+        SHIP(hipSetDevice(dev_));
+        std::vector<int> order(jobs.size());
+        for (size_t k = 0; k < order.size(); ++k) order[k] = (int)k;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int a, int b) { return jobs[a]->sites() > jobs[b]->sites(); });
+        size_t need = 0;
+        for (Job* j : jobs) need += tiles_of(j->spec.nx, j->spec.ny, nw_, rows_);
+        int rc = grow_items(need, err);
+        if (rc != SCL_OK) return rc;
+        JobArgs<T>* table = (JobArgs<T>*)table_h_;
+        int n = 0;
+        for (int k : order) {
+            const GJob* g = (const GJob*)jobs[k]->dev;
+            JobArgs<T>& ja = table[k];
+            ja.a = args_of(jobs[k]->spec, *g);
+            ja.cur = g->cur;
+            ja.steps = steps[k];
+            ja.item_begin = n;
+            n = emit_tiles(k, g->nx, g->ny, nw_, rows_, items_h_, n);
+            ja.item_end = n;
+        }
+        *hdr_h_ = ServeHdr{table_d_, items_d_, partial_d_, mass_d_, n, (int)jobs.size(), 64 * nw_};
+        SHIP(hipMemcpyAsync(table_d_, table_h_, jobs.size() * sizeof(JobArgs<T>), hipMemcpyHostToDevice, stream_));
+        SHIP(hipMemcpyAsync(items_d_, items_h_, (size_t)n * sizeof(uint64_t), hipMemcpyHostToDevice, stream_));
+        SHIP(hipMemcpyAsync(hdr_d_, hdr_h_, sizeof(ServeHdr), hipMemcpyHostToDevice, stream_));
+        hipGraphExec_t ge = nullptr;
+        rc = graph(rsteps, &ge, err);
+        if (rc != SCL_OK) return rc;
+        SHIP(hipGraphLaunch(ge, stream_));
+        SHIP(hipMemcpyAsync(mass_h_, mass_d_, jobs.size() * 2 * sizeof(double), hipMemcpyDeviceToHost, stream_));
+        SHIP(hipStreamSynchronize(stream_));
+        for (size_t k = 0; k < jobs.size(); ++k) {
+            GJob* g = (GJob*)jobs[k]->dev;
+            jobs[k]->mass[0] = mass_h_[2 * k];
+            jobs[k]->mass[1] = mass_h_[2 * k + 1];
+            g->cur ^= steps[k] & 1;
+        }
+        return SCL_OK;
+    }
+
+    int finish(Job& j, std::string* err) override {
+        SHIP(hipSetDevice(dev_));
+        const GJob* g = (const GJob*)j.dev;
+        const Grid2 gr = g->grid();
+        const double rho0 = ST::kDev ? 1.0 : 0.0;
+        const size_t n = (size_t)g->nx * g->ny;
+        for (int c = 0; c < 2; ++c) {
+            const S* P = (const S*)g->f(c, g->cur);
+            if (j.spec.deliver & SCL_DELIVER_RHO) {
+                hipLaunchKernelGGL(k_job_density<ST>, site_grid(g->nx, g->ny), dim3(256), 0, stream_, gr, P, rho0,
+                                   g->rho(c));
+                SHIP(hipGetLastError());
+                j.rho[c].resize(n);
+                SHIP(hipMemcpy2DAsync(j.rho[c].data(), g->nx * 8, g->rho(c), (size_t)g->L.pitch * 8, g->nx * 8,
+                                      g->ny, hipMemcpyDeviceToHost, stream_));
+            }
+            if (j.spec.deliver & SCL_DELIVER_F) {
+                hipLaunchKernelGGL((k_to_f64<ST, true>), site_grid(g->nx, g->ny), dim3(256), 0, stream_, gr, P, rho0,
+                                   g->stage());
+                SHIP(hipGetLastError());
+                j.f[c].resize(9 * n);
+                SHIP(hipMemcpy2DAsync(j.f[c].data(), g->nx * 8, g->stage(), (size_t)g->L.pitch * 8, g->nx * 8,
+                                      (size_t)9 * g->ny, hipMemcpyDeviceToHost, stream_));
+            }
+            SHIP(hipStreamSynchronize(stream_));
+        }
+        return SCL_OK;
+    }
+
+    void drop(Job& j) override {
+        delete (GJob*)j.dev;
+        j.dev = nullptr;
+    }
+
+    void shape(int* cols, int* rows, int* la) const override {
+        *cols = 64 * nw_;
+        *rows = rows_;
+        *la = la_;
+    }
+    int workgroups() const override { return grid_; }
+
+  private:
+    static size_t tiles_of(int nx, int ny, int nw, int rows) {
+        return (size_t)((nx + 64 * nw - 1) / (64 * nw)) * (size_t)((ny + rows - 1) / rows);
+    }
+
+    // Tiles of job k, band by band, strips left to right within a band.
+    static int emit_tiles(int k, int nx, int ny, int nw, int rows, uint64_t* out, int n) {
+        const int gx = (nx + 64 * nw - 1) / (64 * nw);
+        for (int y0 = 0; y0 < ny; y0 += rows)
+            for (int bx = 0; bx < gx; ++bx) out[n++] = pack_item(k, bx, y0, std::min(y0 + rows, ny));
+        return n;
+    }
+
+    int grow_items(size_t need, std::string* err) {
+        if (need <= items_cap_) return SCL_OK;
+        size_t cap = std::max<size_t>(items_cap_ * 2, need);
+        if (items_d_) SHIP(hipFree(items_d_));
+        if (partial_d_) SHIP(hipFree(partial_d_));
+        if (items_h_) SHIP(hipHostFree(items_h_));
+        items_d_ = nullptr;
+        partial_d_ = nullptr;
+        items_h_ = nullptr;
+        SHIP(hipMalloc(&items_d_, cap * sizeof(uint64_t)));
+        SHIP(hipMalloc(&partial_d_, cap * 2 * sizeof(double)));
+        SHIP(hipHostMalloc(&items_h_, cap * sizeof(uint64_t), hipHostMallocDefault));
+        items_cap_ = cap;
+        return SCL_OK;
+    }
+
+    // Per-component parameters exactly as the single-lattice path forms them
+    // (omega = 1/tau, include/fluid.h:62; 1 - omega and 1/omega,
+    // src/lbm.cpp:213-226, 284, 308; -1.0 * G, src/lbm.cpp:163-170 and
+    // src/fluid.cpp:154-160).
+    StepArgs<T> args_of(const JobSpec& s, const GJob& g) const {
+        StepArgs<T> a{};
+        a.nx = g.nx;
+        a.ny = g.ny;
+        a.pitch = g.L.pitch;
+        a.wpr = g.L.wpr;
+        a.plane = g.L.plane;
+        a.mask = (const uint64_t*)(g.base + g.L.mask);
+        a.rows = rows_;
+        a.gx = (g.nx + 64 * nw_ - 1) / (64 * nw_);
+        for (int c = 0; c < 2; ++c) {
+            CompArgs<T>& ca = a.c[c];
+            const double omega = 1 / s.tau[c];
+            ca.src = g.f(c, 0);
+            ca.dst = g.f(c, 1);
+            ca.omega = (T)omega;
+            ca.omc = (T)(1.0 - omega);
+            ca.inv_omega = (T)(1.0 / omega);
+            ca.gneg = (T)(-1.0 * s.g_fluid[c]);
+            ca.gsneg = (T)(-1.0 * s.g_solid[c]);
+            ca.rho0 = (T)(ST::kDev ? 1.0 : 0.0);
+            ca.has_force = s.has_force[c] ? 1 : 0;
+            for (int i = 0; i < Q; ++i) ca.force[i] = (T)s.force[c][i];
+            ca.ueqx = g.ueq(c, 0);
+            ca.ueqy = g.ueq(c, 1);
+        }
+        return a;
+    }
+
+    // The job's lattice at `base`: zeroed block, mask, S_0 in buffer 0 and
+    // its scatter in buffer 1.
+    int build(Job& j, char* base, std::string* err) {
+        const JobSpec& s = j.spec;
+        auto g = std::unique_ptr<GJob>(new GJob);
+        g->L = layout_of(s, sizeof(S));
+        g->base = base;
+        g->nx = s.nx;
+        g->ny = s.ny;
+        const Layout& L = g->L;
+        if (!j.prepared) {   // jobs built outside the scheduler (the autotuner's)
+            std::vector<int> m;
+            serve::job_mask(s, &m);
+            serve::pack_mask(m, s.nx, s.ny, &j.bits);
+            j.prepared = true;
+        }
+        if ((int)(j.bits.size() / s.ny) != L.wpr || j.bits.size() != (size_t)s.ny * L.wpr) {
+            *err = "packed mask does not match the lattice";
+            return SCL_ERR_ARG;
+        }
+        SHIP(hipMemsetAsync(base, 0, L.total, stream_));
+        SHIP(hipMemcpyAsync(base + L.mask, j.bits.data(), j.bits.size() * 8, hipMemcpyHostToDevice, stream_));
+        const Grid2 gr = g->grid();
+        const dim3 sg = site_grid(s.nx, s.ny);
+        const double rho0 = ST::kDev ? 1.0 : 0.0;
+        for (int c = 0; c < 2; ++c) {
+            S* s0 = (S*)g->f(c, 0);
+            if (!s.f[0].empty()) {
+                SHIP(hipMemcpy2DAsync(g->stage(), (size_t)L.pitch * 8, s.f[c].data(), (size_t)s.nx * 8,
+                                      (size_t)s.nx * 8, (size_t)9 * s.ny, hipMemcpyHostToDevice, stream_));
+                hipLaunchKernelGGL(k_from_f64<ST>, sg, dim3(256), 0, stream_, gr, g->stage(), rho0, s0);
+            } else {
+                const serve::IcParams p{s.nx, s.ny, s.ic, s.rho_hi, s.rho_lo, s.radius, s.noise, s.ic_seed};
+                hipLaunchKernelGGL(k_init_ic<ST>, sg, dim3(256), 0, stream_, gr, p, c, rho0, s0);
+            }
+            SHIP(hipGetLastError());
+            hipLaunchKernelGGL(k_scatter<ST>, sg, dim3(256), 0, stream_, gr, (const S*)s0, (S*)g->f(c, 1));
+            SHIP(hipGetLastError());
+        }
+        SHIP(hipStreamSynchronize(stream_));   // host vectors above must outlive the copies
+        g->cur = 1;
+        j.dev = g.release();
+        return SCL_OK;
+    }
+
+    template <int NW, int LA>
+    void launch_one(int grid, int s) {
+        hipLaunchKernelGGL((k_serve<ST, NW, LA>), dim3(grid), dim3(64 * NW), 0, stream_, hdr_d_, s, ctr_d_);
+    }
+
+    void launch_step(int nw, int la, int grid, int s) {
+        if (nw == 2) {
+            if (la == 2) launch_one<2, 2>(grid, s);
+            else launch_one<2, 1>(grid, s);
+        } else {
+            if (la == 2) launch_one<4, 2>(grid, s);
+            else launch_one<4, 1>(grid, s);
+        }
+    }
+
+    static const void* kernel_of(int nw, int la) {
+        if (nw == 2)
+            return la == 2 ? reinterpret_cast<const void*>(&k_serve<ST, 2, 2>)
+                           : reinterpret_cast<const void*>(&k_serve<ST, 2, 1>);
+        return la == 2 ? reinterpret_cast<const void*>(&k_serve<ST, 4, 2>)
+                       : reinterpret_cast<const void*>(&k_serve<ST, 4, 1>);
+    }
+
+    // Workgroups of the persistent grid: as many as are resident at once.
+    int occupancy(int nw, int la, int* grid, std::string* err) {
+        int per_cu = 0;
+        SHIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel_of(nw, la), 64 * nw, 0));
+        *grid = std::max(1, per_cu) * cus_;
+        return SCL_OK;
+    }
+
+    int graph(int rsteps, hipGraphExec_t* out, std::string* err) {
+        auto it = graphs_.find(rsteps);
+        if (it != graphs_.end()) {
+            *out = it->second;
+            return SCL_OK;
+        }
+        SHIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+        for (int s = 0; s < rsteps; ++s) launch_step(nw_, la_, grid_, s);
+        hipLaunchKernelGGL(k_serve_mass<ST>, dim3(mass_grid_), dim3(256), 0, stream_, hdr_d_);
+        hipLaunchKernelGGL(k_serve_mass_final<T>, dim3(cfg_.max_jobs), dim3(256), 0, stream_, hdr_d_);
+        hipGraph_t gr = nullptr;
+        hipError_t e = hipStreamEndCapture(stream_, &gr);
+        if (e != hipSuccess) {
+            *err = std::string("graph capture failed: ") + hip_msg(e);
+            return SCL_ERR_HIP;
+        }
+        hipGraphExec_t ge = nullptr;
+        e = hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        if (e != hipSuccess) {
+            *err = std::string("graph instantiation failed: ") + hip_msg(e);
+            return SCL_ERR_HIP;
+        }
+        graphs_[rsteps] = ge;
+        *out = ge;
+        return SCL_OK;
+    }
+
+    // Tile shape by measurement: two resident 2048 x 2048 porous jobs
+    // (porosity 0.8, disks of radius 8..20, droplet initial state) in
+    // scratch memory; every (width 128/256, lookahead 1/2, rows 16..128)
+    // runs 2 + 8 steps and the fastest mean step wins.
+    int autotune(std::string* err) {
+        JobSpec s;
+        s.nx = s.ny = 2048;
+        s.porosity = 0.8;
+        s.rmin = 8;
+        s.rmax = 20;
+        s.geom_seed = 99;
+        s.ic = SCL_IC_DROPLET;
+        s.rho_hi = 1.0;
+        s.rho_lo = 0.06;
+        s.radius = 400;
+        s.noise = 0.01;
+        s.ic_seed = 7;
+        s.tau[0] = s.tau[1] = 1.0;
+        s.g_fluid[0] = s.g_fluid[1] = 0.9;
+        const size_t bytes = layout_of(s, sizeof(S)).total;
+        char* scratch = nullptr;
+        hipError_t e = hipMalloc(&scratch, 2 * bytes);
+        if (e != hipSuccess) {
+            *err = std::string("autotune scratch allocation failed: ") + hip_msg(e);
+            return SCL_ERR_HIP;
+        }
+        Job ja, jb;
+        ja.spec = s;
+        jb.spec = s;
+        ja.spec.ic_seed = 11;
+        int rc = build(ja, scratch, err);
+        if (rc == SCL_OK) rc = build(jb, scratch + bytes, err);
+        std::vector<Job*> jobs = {&ja, &jb};
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (rc == SCL_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) {
+            *err = "event creation failed";
+            rc = SCL_ERR_HIP;
+        }
+        float best = 1e30f;
+        int bnw = 4, brows = 32, bla = 1;
+        const int nws[] = {2, 4}, las[] = {1, 2}, rowss[] = {16, 32, 64, 128};
+        for (int nw : nws)
+            for (int la : las)
+                for (int rows : rowss) {
+                    if (rc != SCL_OK) break;
+                    int grid = 0;
+                    rc = occupancy(nw, la, &grid, err);
+                    if (rc != SCL_OK) break;
+                    nw_ = nw;
+                    rows_ = rows;
+                    la_ = la;
+                    const std::vector<int> steps = {10, 10};
+                    size_t need = 2 * tiles_of(s.nx, s.ny, nw, rows);
+                    rc = grow_items(need, err);
+                    if (rc != SCL_OK) break;
+                    JobArgs<T>* table = (JobArgs<T>*)table_h_;
+                    int n = 0;
+                    for (int k = 0; k < 2; ++k) {
+                        const GJob* g = (const GJob*)jobs[k]->dev;
+                        table[k].a = args_of(jobs[k]->spec, *g);
+                        table[k].cur = g->cur;
+                        table[k].steps = 10;
+                        table[k].item_begin = n;
+                        n = emit_tiles(k, g->nx, g->ny, nw, rows, items_h_, n);
+                        table[k].item_end = n;
+                    }
+                    *hdr_h_ = ServeHdr{table_d_, items_d_, partial_d_, mass_d_, n, 2, 64 * nw};
+                    if (hipMemcpyAsync(table_d_, table_h_, 2 * sizeof(JobArgs<T>), hipMemcpyHostToDevice,
+                                       stream_) != hipSuccess ||
+                        hipMemcpyAsync(items_d_, items_h_, (size_t)n * sizeof(uint64_t), hipMemcpyHostToDevice,
+                                       stream_) != hipSuccess ||
+                        hipMemcpyAsync(hdr_d_, hdr_h_, sizeof(ServeHdr), hipMemcpyHostToDevice, stream_) !=
+                            hipSuccess) {
+                        *err = "autotune upload failed";
+                        rc = SCL_ERR_HIP;
+                        break;
+                    }
+                    launch_step(nw, la, grid, 0);
+                    launch_step(nw, la, grid, 1);
+                    (void)hipEventRecord(e0, stream_);
+                    for (int t = 2; t < 10; ++t) launch_step(nw, la, grid, t);
+                    (void)hipEventRecord(e1, stream_);
+                    if (hipStreamSynchronize(stream_) != hipSuccess || hipGetLastError() != hipSuccess) {
+                        *err = "autotune launch failed";
+                        rc = SCL_ERR_HIP;
+                        break;
+                    }
+                    float ms = 0;
+                    (void)hipEventElapsedTime(&ms, e0, e1);
+                    // the two scratch jobs took 10 steps: both buffers swap back
+                    if (ms < best) {
+                        best = ms;
+                        bnw = nw;
+                        brows = rows;
+                        bla = la;
+                    }
+                }
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        drop(ja);
+        drop(jb);
+        (void)hipStreamSynchronize(stream_);
+        (void)hipFree(scratch);
+        nw_ = bnw;
+        rows_ = brows;
+        la_ = bla;
+        return rc;
+    }
+
+    scl_server_config cfg_;
+    int dev_ = 0, cus_ = 256;
+    hipStream_t stream_ = nullptr;
+    char* arena_ = nullptr;
+    ServeHdr* hdr_d_ = nullptr;
+    ServeHdr* hdr_h_ = nullptr;
+    unsigned* ctr_d_ = nullptr;
+    void* table_d_ = nullptr;
+    void* table_h_ = nullptr;
+    uint64_t* items_d_ = nullptr;
+    uint64_t* items_h_ = nullptr;
+    size_t items_cap_ = 0;
+    double* partial_d_ = nullptr;
+    double* mass_d_ = nullptr;
+    double* mass_h_ = nullptr;
+    int nw_ = 4, rows_ = 32, la_ = 1, grid_ = 0, mass_grid_ = 1024;
+    std::map<int, hipGraphExec_t> graphs_;
+};
+
+template <class ST>
+int make_backend(const scl_server_config& cfg, std::unique_ptr<serve::Backend>* out, std::string* err) {
+    auto b = std::unique_ptr<GpuServe<ST>>(new GpuServe<ST>(cfg));
+    const int rc = b->init(err);
+    if (rc != SCL_OK) return rc;
+    *out = std::move(b);
+    return SCL_OK;
+}
+
+}  // namespace
+
+extern "C" int scl_server_create(const scl_server_config* cfg, scl_server** out) {
+    if (!out) return serve::report(SCL_ERR_ARG, "null output");
+    *out = nullptr;
+    if (!cfg) return serve::report(SCL_ERR_ARG, "null config");
+    const scl_server_config& c = *cfg;
+    if (c.precision < SCL_FP64 || c.precision > SCL_FP16_DEV)
+        return serve::report(SCL_ERR_ARG, "bad precision " + std::to_string(c.precision));
+    if (c.max_jobs < 1 || c.max_jobs > 65535) return serve::report(SCL_ERR_ARG, "max_jobs must be in 1..65535");
+    if (c.max_pending < 1) return serve::report(SCL_ERR_ARG, "max_pending must be >= 1");
+    if (c.round_steps < 1 || c.round_steps > 1024) return serve::report(SCL_ERR_ARG, "round_steps must be in 1..1024");
+    if (!(c.mass_tol >= 0)) return serve::report(SCL_ERR_ARG, "mass_tol must be >= 0 (0: by precision)");
+    if (c.max_bypass < 0) return serve::report(SCL_ERR_ARG, "max_bypass must be >= 0");
+    int rc = runtime_ready();
+    if (rc != SCL_OK) return rc;
+    std::unique_ptr<serve::Backend> be;
+    std::string err;
+    switch (c.precision) {
+        case SCL_FP64: rc = make_backend<StoreF64>(c, &be, &err); break;
+        case SCL_FP32_DEV: rc = make_backend<StoreF32>(c, &be, &err); break;
+        default: rc = make_backend<StoreF16>(c, &be, &err); break;
+    }
+    if (rc != SCL_OK) return serve::report(rc, err);
+    auto s = std::unique_ptr<scl_server>(new scl_server);
+    s->sched.reset(new serve::Scheduler(std::move(be), c));
+    *out = s.release();
+    return SCL_OK;
+}

@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 # SCLBM_LIBRARY selects another build of the library (e.g. to compare builds)
 LIB_PATH = os.environ.get("SCLBM_LIBRARY") or os.path.join(HERE, "libsclbm.so")
 
-SCL_OK, SCL_ERR_ARG, SCL_ERR_HIP, SCL_ERR_RANGE, SCL_ERR_UNSUPPORTED = 0, 1, 2, 3, 4
+SCL_OK, SCL_ERR_ARG, SCL_ERR_HIP, SCL_ERR_RANGE, SCL_ERR_UNSUPPORTED, SCL_ERR_BUSY, SCL_ERR_TIMEOUT = range(7)
 FP64, FP32_DEV, FP16_DEV = 0, 1, 2
 F, FEQ, RHO, UX, UY, UEQX, UEQY, FRX, FRY, FSX, FSY = range(11)
 PRECISION = {"fp64": FP64, "fp32dev": FP32_DEV, "fp16dev": FP16_DEV}
@@ -26,6 +26,10 @@ EXPORTS = [
     "scl_fluid_repulsion", "scl_equilibrium_velocities", "scl_collide1", "scl_collide2",
     "scl_volume_force", "scl_stream1", "scl_stream2", "scl_step1", "scl_step2", "scl_moments",
     "scl_set_tile", "scl_set_lookahead", "scl_get_tile", "scl_autotune", "scl_timing", "scl_probe_copy",
+    # serving layer (bound in serve.py)
+    "scl_server_default_config", "scl_server_create", "scl_server_submit", "scl_server_status",
+    "scl_server_wait", "scl_server_cancel", "scl_server_result", "scl_server_release", "scl_server_drain",
+    "scl_server_get_stats", "scl_server_destroy", "scl_job_mask",
 ]
 
 

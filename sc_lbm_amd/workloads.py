@@ -198,3 +198,78 @@ def poiseuille_profile(ny, g, tau, rho=1.0):
     u = g / (2.0 * rho * nu) * (h * h - (y - yc) ** 2)
     u[0] = u[-1] = 0.0
     return u
+
+
+# --------------------------------------------------------------------------
+# Served jobs (the serving layer generates these on the device; the same
+# expressions evaluated here give the inputs of the checks)
+# --------------------------------------------------------------------------
+
+def served_density(nx, ny, ic=0, rho_hi=1.0, rho_lo=1.0, radius=0.0, noise=0.0, seed=0):
+    """Initial densities (rho_A, rho_B) of a generated served job: ic 0
+    uniform, 1 a disk of `radius` about (nx//2, ny//2), 2 rows y < ny//2;
+    A takes rho_hi in the region and rho_lo elsewhere, B the mirror; then
+    rho * (1 + U[-noise, noise)) with seed (A) and seed + 1 (B)."""
+    y, x = np.mgrid[0:ny, 0:nx]
+    if ic == 1:
+        dx, dy = x - nx // 2, y - ny // 2
+        a_hi = (dx * dx + dy * dy).astype(np.float64) <= radius * radius
+    elif ic == 2:
+        a_hi = y < ny // 2
+    else:
+        a_hi = np.ones((ny, nx), dtype=bool)
+    ra = np.where(a_hi, rho_hi, rho_lo).astype(np.float64)
+    rb = np.where(a_hi, rho_lo, rho_hi).astype(np.float64)
+    if noise != 0.0:
+        ra = noisy(ra, seed, noise)
+        rb = noisy(rb, seed + 1, noise)
+    return ra, rb
+
+
+def served_case(nx, ny, porosity=1.0, rmin=8, rmax=20, walls=False, geom_seed=0, ic=0, rho_hi=1.0, rho_lo=1.0,
+                radius=0.0, noise=0.0, ic_seed=0, tau=(1.0, 1.0), g_fluid=(0.0, 0.0), g_solid=(0.0, 0.0),
+                force=(None, None), **_):
+    """Mask and initial populations of a generated served job, in the case
+    dict form of the standard cases above."""
+    if porosity < 1.0:
+        m, _d = porous_mask(nx, ny, porosity, geom_seed, rmin, rmax, walls=walls)
+    else:
+        m = walls_x(nx, ny) if walls else open_mask(nx, ny)
+    ra, rb = served_density(nx, ny, ic, rho_hi, rho_lo, radius, noise, ic_seed)
+    return dict(name="served_%dx%d" % (nx, ny), mask=m, tau=list(tau), g_fluid=list(g_fluid),
+                g_solid=list(g_solid), f=[f_from_rho(ra, m), f_from_rho(rb, m)], force=list(force))
+
+
+SERVED_SIDES = (128, 256, 512, 1024, 2048)
+
+
+def served_stream(n_jobs=2000, seed=2025, steps=(2000, 50000), step_scale=1.0):
+    """The served job mix: lattice sides drawn independently from
+    {128, 256, 512, 1024, 2048}; seeded disk obstacles (radii 8..20, no
+    walls) to a porosity uniform in [0.6, 1.0]; G uniform in [0.8, 1.1] and
+    tau uniform in [0.7, 1.3] (both components); a droplet (radius a quarter
+    of the shorter side) or stratified initial state, densities 1.0 / 0.06
+    with 1% noise; a step count uniform in `steps`, times step_scale.
+    Returns keyword dicts for serve.make_job (without release times)."""
+    st = Stream(seed)
+    jobs = []
+    for k in range(n_jobs):
+        nx = SERVED_SIDES[min(4, int(st.next() * 5))]
+        ny = SERVED_SIDES[min(4, int(st.next() * 5))]
+        porosity = 0.6 + 0.4 * st.next()
+        G = 0.8 + 0.3 * st.next()
+        tau = 0.7 + 0.6 * st.next()
+        ic = 1 if st.next() < 0.5 else 2
+        n = steps[0] + int(st.next() * (steps[1] - steps[0] + 1))
+        jobs.append(dict(nx=nx, ny=ny, steps=max(1, int(round(n * step_scale))), porosity=porosity, rmin=8, rmax=20,
+                         geom_seed=seed * 1000003 + 2 * k, ic=ic, rho_hi=1.0, rho_lo=0.06,
+                         radius=min(nx, ny) / 4.0, noise=0.01, ic_seed=seed * 1000003 + 2 * k + 1,
+                         tau=(tau, tau), g_fluid=(G, G)))
+    return jobs
+
+
+def poisson_arrivals(n, rate_per_s, seed=7):
+    """Arrival times in ms of a Poisson process of the given rate."""
+    u = uniform01(seed, n)
+    gaps = -np.log1p(-u) / rate_per_s * 1e3
+    return np.cumsum(gaps)

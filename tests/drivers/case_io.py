@@ -73,4 +73,19 @@ def build():
            os.path.join(HERE, "sc_driver.cpp"), "-o", BIN, "-L" + lib, "-lsclbm",
            "-Wl,-rpath,$ORIGIN/../../../sc_lbm_amd"]
     subprocess.check_call(cmd)
+    build_serve_cpu()
     return BIN
+
+
+SERVE_CPU = os.path.join(HERE, "bin", "libserve_cpu.so")
+
+
+def build_serve_cpu():
+    """The serving scheduler with the oracle as its backend (CPU tests)."""
+    os.makedirs(os.path.dirname(SERVE_CPU), exist_ok=True)
+    cmd = ["g++", "-std=c++14", "-O2", "-fPIC", "-shared", "-pthread", "-ffp-contract=off", "-Wall",
+           "-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(ROOT, "sc_lbm_amd", "host"),
+           os.path.join(HERE, "serve_cpu.cpp"), os.path.join(ROOT, "sc_lbm_amd", "host", "serve_core.cpp"),
+           os.path.join(ROOT, "oracle", "sc_oracle.cpp"), "-o", SERVE_CPU]
+    subprocess.check_call(cmd)
+    return SERVE_CPU

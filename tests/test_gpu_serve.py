@@ -1,0 +1,204 @@
+"""GPU tests of the serving layer (scl_server_*): served FP64 jobs are bit
+for bit the oracle's run of the same case; every job's result is the same
+alone, in a mix of jobs of other sizes admitted while it runs, under
+memory back-pressure, with other tile shapes, and on the single-lattice
+path; the watchdog isolates a failing job; FP32 deviations stay within
+1e-5 of FP64 on a served job."""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from sc_lbm_amd import native as nat
+from sc_lbm_amd import serve
+from sc_lbm_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+TILE = dict(tile_cols=256, tile_rows=32, lookahead=1)
+DROP = dict(ic=serve.IC_DROPLET, rho_hi=1.0, rho_lo=0.06, noise=0.01, tau=(1.0, 0.8), g_fluid=(0.9, 0.9))
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _init():
+    nat.init(0)
+    yield
+
+
+def gpu_server(precision="fp32dev", **cfg):
+    base = dict(budget_bytes=2 << 30, round_steps=16, precision=nat.PRECISION[precision], **TILE)
+    base.update(cfg)
+    return serve.Server(**base)
+
+
+def oracle_run(case, steps):
+    s = po.System(case["mask"], case["tau"], case["g_fluid"], case["g_solid"])
+    for c in range(2):
+        s.set_f(c, case["f"][c])
+    s.solid_surface_force()
+    s.run(steps, case["force"][0], case["force"][1])
+    return [s.get_f(c) for c in range(2)]
+
+
+def dens_ref(f):
+    p = f.reshape(9, -1)
+    r = np.zeros(p.shape[1])
+    for i in range(9):
+        r = r + p[i]
+    return r
+
+
+def footprint(nx, ny, es, stage):
+    """Device bytes of one resident job (the backend's block layout: mask,
+    2 x 2 population buffers, 4 equilibrium-velocity and 2 density planes
+    in FP64, optional FP64 staging; each region 256-byte aligned)."""
+    a = lambda n: (n + 255) // 256 * 256
+    pitch = (nx + 63) // 64 * 64
+    plane = pitch * ny
+    total = a(ny * pitch // 64 * 8) + 4 * a(9 * plane * es) + 6 * a(plane * 8)
+    return total + (a(9 * plane * 8) if stage else 0)
+
+
+def run_alone(job, precision="fp32dev", **cfg):
+    with gpu_server(precision, **cfg) as srv:
+        jid = srv.submit(**job)
+        st = srv.wait(jid, 60000)
+        assert st["state"] == "done", st
+        assert st["bytes"] == footprint(job["nx"], job["ny"], {"fp64": 8, "fp32dev": 4, "fp16dev": 2}[precision],
+                                        "f" in job or job.get("deliver", 0) & serve.DELIVER_F)
+        return [srv.result(jid, serve.F, c) for c in range(2)]
+
+
+POROUS = dict(nx=150, ny=96, steps=300, porosity=0.8, rmin=4, rmax=9, walls=True, geom_seed=3, radius=20,
+              ic_seed=21, g_solid=(0.1, -0.1), force=(wl.body_force(1e-5), None), **DROP)
+
+
+def test_fp64_served_job_is_the_oracle_run():
+    """Generated mask (host sampler) and initial state (device kernel), then
+    300 steps in rounds of 16 with other jobs resident: bit-exact."""
+    others = [dict(nx=64 + 32 * k, ny=80, steps=100 + 50 * k, ic_seed=k, radius=10, **DROP) for k in range(3)]
+    with gpu_server("fp64") as srv:
+        ids = [srv.submit(**o) for o in others]
+        jid = srv.submit(deliver=serve.DELIVER_F | serve.DELIVER_RHO, **POROUS)
+        st = srv.wait(jid, 60000)
+        assert st["state"] == "done" and st["steps_done"] == 300
+        got = [srv.result(jid, serve.F, c) for c in range(2)]
+        rho = [srv.result(jid, serve.RHO, c) for c in range(2)]
+        for i in ids:
+            assert srv.wait(i, 60000)["state"] == "done"
+    ref = oracle_run(wl.served_case(**POROUS), 300)
+    for c in range(2):
+        assert np.array_equal(got[c].reshape(-1), ref[c])
+        assert np.array_equal(rho[c].reshape(-1), dens_ref(ref[c]))
+
+
+def test_fp64_job_with_caller_arrays_is_the_oracle_run():
+    case = wl.porous(nx=128, ny=72, porosity=0.8, seed=9, rmin=4, rmax=8)
+    job = dict(nx=128, ny=72, steps=200, mask=case["mask"], f=case["f"], tau=case["tau"], g_fluid=case["g_fluid"],
+               g_solid=case["g_solid"], force=case["force"], deliver=serve.DELIVER_F)
+    got = run_alone(job, "fp64", round_steps=32)
+    ref = oracle_run(case, 200)
+    for c in range(2):
+        assert np.array_equal(got[c].reshape(-1), ref[c])
+
+
+@pytest.mark.parametrize("precision", ["fp32dev", "fp16dev"])
+def test_alone_equals_mix_and_single_lattice_path(precision):
+    """Continuous batching: the target runs alone, then in a mix whose other
+    jobs are of other sizes and arrive while it runs; both results and the
+    single-lattice fused path agree bit for bit."""
+    target = dict(deliver=serve.DELIVER_F, **POROUS)
+    alone = run_alone(target, precision)
+    mix = [dict(nx=300, ny=200, steps=500, porosity=0.9, geom_seed=5, ic=serve.IC_STRATIFIED, rho_hi=1.0,
+                rho_lo=0.1, noise=0.01, ic_seed=8, tau=(0.9, 1.2), g_fluid=(1.0, 1.0)),
+           dict(nx=64, ny=64, steps=40, ic_seed=1, radius=12, **DROP),
+           dict(nx=517, ny=33, steps=120, ic_seed=2, radius=10, release_ms=5.0, **DROP),
+           dict(nx=128, ny=256, steps=260, porosity=0.75, rmin=5, rmax=12, geom_seed=7, ic_seed=3, radius=30,
+                release_ms=20.0, **DROP)]
+    with gpu_server(precision) as srv:
+        t0 = srv.stats()["clock_ms"]
+        ids = [srv.submit(**dict(m, release_ms=t0 + m.get("release_ms", 0.0))) for m in mix[:2]]
+        jid = srv.submit(**dict(target, release_ms=t0 + 2.0))
+        ids += [srv.submit(**dict(m, release_ms=t0 + m["release_ms"])) for m in mix[2:]]
+        st = srv.wait(jid, 60000)
+        assert st["state"] == "done"
+        mixed = [srv.result(jid, serve.F, c) for c in range(2)]
+        for i in ids:
+            assert srv.wait(i, 60000)["state"] == "done"
+        stats = srv.stats()
+    assert stats["max_resident"] >= 3
+    for c in range(2):
+        assert np.array_equal(alone[c], mixed[c])
+    # the single-lattice path from the same initial state
+    case = wl.served_case(**POROUS)
+    g = nat.Geometry(case["mask"])
+    fl = [nat.Field(g, case["tau"][c], nat.PRECISION[precision], 1.0, g_solid=case["g_solid"][c],
+                    g_fluid=case["g_fluid"][c]) for c in range(2)]
+    for c in range(2):
+        fl[c].set_f(case["f"][c])
+    nat.solid_surface_force(g, fl[0], fl[1])
+    nat.step2(g, fl[0], fl[1], case["force"][0], case["force"][1], POROUS["steps"])
+    for c in range(2):
+        assert np.array_equal(fl[c].get_f().reshape(-1), alone[c].reshape(-1))
+
+
+def test_tile_shape_and_round_length_do_not_change_results():
+    job = dict(deliver=serve.DELIVER_F, **dict(POROUS, steps=77))
+    a = run_alone(job, "fp32dev", tile_cols=128, tile_rows=16, lookahead=2, round_steps=5)
+    b = run_alone(job, "fp32dev", tile_cols=256, tile_rows=64, lookahead=1, round_steps=64)
+    for c in range(2):
+        assert np.array_equal(a[c], b[c])
+
+
+def test_memory_back_pressure_keeps_results():
+    spec = [dict(nx=256, ny=256, steps=60 + 10 * k, ic_seed=40 + k, radius=50, deliver=serve.DELIVER_F, **DROP)
+            for k in range(5)]
+    per_job = footprint(256, 256, 4, stage=True)
+    with gpu_server("fp32dev", budget_bytes=2 * per_job + 4096) as srv:
+        ids = [srv.submit(**s) for s in spec]
+        sts = [srv.wait(i, 60000) for i in ids]
+        stats = srv.stats()
+        mixed = [[srv.result(i, serve.F, c) for c in range(2)] for i in ids]
+    assert all(s["state"] == "done" for s in sts)
+    assert stats["max_resident"] <= 2 and stats["done"] == 5
+    for k in (0, 4):
+        alone = run_alone(spec[k])
+        for c in range(2):
+            assert np.array_equal(alone[c], mixed[k][c])
+
+
+def test_watchdog_fails_only_the_bad_job():
+    case = wl.droplet(n=64, radius=12, G=0.9, seed=4)
+    bad = [case["f"][0].copy(), case["f"][1]]
+    bad[0][5 * 64 * 64 + 700] = np.inf
+    with gpu_server("fp32dev") as srv:
+        good = [srv.submit(nx=96, ny=64, steps=50, ic_seed=k, radius=12, **DROP) for k in range(3)]
+        jbad = srv.submit(nx=64, ny=64, steps=50, f=bad, tau=case["tau"], g_fluid=case["g_fluid"])
+        st = srv.wait(jbad, 60000)
+        assert st["state"] == "failed" and "non-finite mass" in st["error"], st
+        assert all(srv.wait(g, 60000)["state"] == "done" for g in good)
+        assert srv.stats()["arena_used"] == 0
+
+
+def test_fp32dev_served_job_within_1e5_of_fp64():
+    job = dict(nx=256, ny=256, steps=1000, ic_seed=5, radius=60, deliver=serve.DELIVER_RHO, **DROP)
+    with gpu_server("fp64") as s64, gpu_server("fp32dev") as s32:
+        a, b = s64.submit(**job), s32.submit(**job)
+        assert s64.wait(a, 60000)["state"] == "done" and s32.wait(b, 60000)["state"] == "done"
+        err = max(float(np.max(np.abs(s64.result(a, serve.RHO, c) - s32.result(b, serve.RHO, c))))
+                  for c in range(2))
+    assert err <= 1e-5, err
+
+
+def test_autotuned_server_runs_and_reports_its_shape():
+    with serve.Server(precision=nat.FP32_DEV, budget_bytes=1 << 30) as srv:
+        st = srv.stats()
+        assert st["tile_cols"] in (128, 256) and st["tile_rows"] in (16, 32, 64, 128)
+        assert st["lookahead"] in (1, 2) and st["workgroups"] >= 256
+        jid = srv.submit(nx=200, ny=120, steps=33, ic_seed=9, radius=30, deliver=serve.DELIVER_F, **DROP)
+        assert srv.wait(jid, 60000)["state"] == "done"
+        tuned = [srv.result(jid, serve.F, c) for c in range(2)]
+        st = srv.stats()
+        assert st["job_steps"] == 33 and st["lattice_updates"] == 200 * 120 * 33
+    fixed = run_alone(dict(nx=200, ny=120, steps=33, ic_seed=9, radius=30, deliver=serve.DELIVER_F, **DROP))
+    for c in range(2):
+        assert np.array_equal(tuned[c], fixed[c])
