@@ -57,8 +57,8 @@ __device__ inline void unpack_item(uint64_t it, int& job, int& bx, int& y0, int&
 }
 
 template <class ST, int NW, int LA>
-__global__ __launch_bounds__(64 * NW) void k_serve(const ServeHdr* __restrict__ hdr, int s,
-                                                   unsigned* __restrict__ ctr) {
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(step_waves_per_eu<ST, false, LA>()))) void
+k_serve(const ServeHdr* __restrict__ hdr, int s, unsigned* __restrict__ ctr) {
     using T = typename ST::T;
     __shared__ DensRing<2, T, NW> lds;
     __shared__ int ticket;

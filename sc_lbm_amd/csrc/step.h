@@ -441,8 +441,12 @@ __device__ inline void finish2(const StepArgs<typename ST::T>& a, void* const (&
 // Occupancy request: the solid-free FP32-arithmetic kernels fit in 128
 // registers, i.e. four waves per SIMD.
 template <class ST, bool OPEN, int LA>
+#ifndef SCL_GATHER_WAVES
+#define SCL_GATHER_WAVES 1
+#endif
 constexpr int step_waves_per_eu() {
-    return (OPEN && LA == 1 && sizeof(typename ST::T) == 4) ? 4 : 1;
+    return (OPEN && LA == 1 && sizeof(typename ST::T) == 4) ? 4
+                                                            : ((!OPEN && sizeof(typename ST::T) == 4) ? SCL_GATHER_WAVES : 1);
 }
 
 // Source and destination population buffers of one launch, per component.

@@ -279,6 +279,16 @@ double ic_density(const JobSpec& s, int c, int x, int y) {
 // ---------------------------------------------------------------------------
 // Scheduler
 // ---------------------------------------------------------------------------
+namespace {
+// Timed waits use a system-clock deadline (pthread_cond_timedwait, which
+// thread sanitizers see); every wait re-checks its condition, so a step of
+// the system clock only moves a wake-up.
+std::chrono::system_clock::time_point deadline(double ms) {
+    return std::chrono::system_clock::now() +
+           std::chrono::duration_cast<std::chrono::system_clock::duration>(std::chrono::duration<double, std::milli>(ms));
+}
+}  // namespace
+
 // Watchdog tolerance by storage form.  FP64 and the FP32 deviation form
 // conserve mass to their rounding; FP16 storage rounds each population to 11
 // bits of (f_i - w_i rho0) * 2^8, which has moved the mass of a droplet
@@ -428,7 +438,7 @@ int Scheduler::wait(long long id, double timeout_ms, scl_job_status* st) {
         cv_done_.wait(lk, done);
         ok = true;
     } else {
-        ok = cv_done_.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), done);
+        ok = cv_done_.wait_until(lk, deadline(timeout_ms), done);
     }
     if (st) fill_status(*j, st);
     if (!ok) return report(SCL_ERR_TIMEOUT, "job " + std::to_string(id) + " not finished within the wait");
@@ -490,7 +500,7 @@ int Scheduler::drain(double timeout_ms) {
         cv_done_.wait(lk, idle);
         ok = true;
     } else {
-        ok = cv_done_.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), idle);
+        ok = cv_done_.wait_until(lk, deadline(timeout_ms), idle);
     }
     return ok ? SCL_OK : report(SCL_ERR_TIMEOUT, "jobs still queued or resident at the end of the wait");
 }
@@ -612,7 +622,7 @@ void Scheduler::loop() {
                     if (next < 0 || dl < next) next = dl;
                 }
             if (next < 0) cv_work_.wait(lk);
-            else cv_work_.wait_for(lk, std::chrono::duration<double, std::milli>(next - now));
+            else cv_work_.wait_until(lk, deadline(next - now));
             continue;
         }
         lk.unlock();
