@@ -36,7 +36,7 @@ def parse():
     p.add_argument("--tile", default="", help="cols,rows[,lookahead] override (default: autotuned)")
     p.add_argument("--no-serve", action="store_true", help="skip the served job stream")
     p.add_argument("--serve-jobs", type=int, default=2000)
-    p.add_argument("--serve-step-scale", type=float, default=0.05,
+    p.add_argument("--serve-step-scale", type=float, default=0.025,
                    help="multiplies the per-job step counts drawn from [2000, 50000]")
     p.add_argument("--serve-load", type=float, default=0.8, help="offered load / measured server capacity")
     p.add_argument("--serve-budget-gib", type=float, default=64.0)
@@ -99,13 +99,13 @@ def served_stream(nat, a):
         st0 = srv.stats()
         log("  server tile %d x %d, lookahead %d, %d workgroups per step"
             % (st0["tile_cols"], st0["tile_rows"], st0["lookahead"], st0["workgroups"]))
-        # capacity: eight large porous jobs submitted together
-        probe = [dict(jobs[0], nx=2048, ny=2048, steps=200, geom_seed=91 + k, ic_seed=92 + k) for k in range(8)]
+        # capacity: 64 jobs of the same mix (another seed) submitted together
+        probe = wl.served_stream(64, seed=77, step_scale=a.serve_step_scale)
         ids = [srv.submit(**p) for p in probe]
         for i in ids:
             srv.wait(i)
         st1 = srv.stats()
-        cap = (st1["lattice_updates"] - st0["lattice_updates"]) / (st1["busy_ms"] - st0["busy_ms"]) / 1e3
+        cap = (st1["lattice_updates"] - st0["lattice_updates"]) / (st1["clock_ms"] - st0["clock_ms"]) / 1e3
         for i in ids:
             srv.release(i)
         lus = np.array([j["nx"] * j["ny"] * j["steps"] for j in jobs], dtype=np.float64)
@@ -132,14 +132,21 @@ def served_stream(nat, a):
     last = max(sts[k]["done_ms"] for k in done)
     lu_done = float(lus[done].sum())
     # FP64 rerun of the sample
-    errs = []
+    errs, worst = [], []
     with serve.Server(precision=nat.FP64, budget_bytes=int(a.serve_budget_gib * (1 << 30)), max_jobs=256) as s64:
         ids64 = {k: s64.submit(deliver=serve.DELIVER_RHO, **jobs[k]) for k in rho32}
         for k, i in ids64.items():
             st = s64.wait(i)
             if st["state"] != "done":
                 continue
-            errs.append(max(float(np.max(np.abs(s64.result(i, serve.RHO, c) - rho32[k][c]))) for c in range(2)))
+            e = max(float(np.max(np.abs(s64.result(i, serve.RHO, c) - rho32[k][c]))) for c in range(2))
+            errs.append(e)
+            j = jobs[k]
+            worst.append(dict(err=e, job=k, nx=j["nx"], ny=j["ny"], steps=j["steps"], tau=round(j["tau"][0], 4),
+                              G=round(j["g_fluid"][0], 4), porosity=round(j["porosity"], 4),
+                              ic="droplet" if j["ic"] == serve.IC_DROPLET else "stratified",
+                              rho_min_fp64=min(float(s64.result(i, serve.RHO, c).min()) for c in range(2))))
+    worst.sort(key=lambda w: -w["err"])
     return dict(jobs=len(jobs), done=len(done), failed=sum(s["state"] == "failed" for s in sts),
                 step_scale=a.serve_step_scale, steps_range=[int(round(2000 * a.serve_step_scale)),
                                                             int(round(50000 * a.serve_step_scale))],
@@ -147,11 +154,16 @@ def served_stream(nat, a):
                 p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
                 mean_ms=float(lat.mean()), max_ms=float(lat.max()),
                 aggregate_mlups=lu_done / (last - first) / 1e3, busy_mlups=st2["lattice_updates"] / st2["busy_ms"] / 1e3,
+                device_mlups=st2["lattice_updates"] / st2["device_ms"] / 1e3 if st2["device_ms"] else None,
+                busy_ms=st2["busy_ms"], device_ms=st2["device_ms"], setup_ms=st2["setup_ms"],
+                finish_ms=st2["finish_ms"], rounds=st2["rounds"],
                 makespan_s=(last - first) / 1e3, max_resident=st2["max_resident"],
                 arena_peak_gib=st2["arena_peak"] / (1 << 30),
                 tile=[st2["tile_cols"], st2["tile_rows"], st2["lookahead"]], round_steps=st2["round_steps"],
                 sample_jobs=len(errs), sample_max_abs_rho_err=max(errs) if errs else None,
-                sample_median_abs_rho_err=float(np.median(errs)) if errs else None)
+                sample_median_abs_rho_err=float(np.median(errs)) if errs else None,
+                sample_frac_within_1e5=float(np.mean(np.array(errs) <= 1e-5)) if errs else None,
+                sample_worst=worst[:5])
 
 
 def cpu_baseline(seconds):

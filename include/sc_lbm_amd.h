@@ -271,6 +271,9 @@ typedef struct scl_server_stats {
     long long rejected;         /* submits refused as invalid or larger than the arena */
     long long rounds, job_steps, lattice_updates;
     double busy_ms;             /* wall time spent in rounds                           */
+    double device_ms;           /* device time of the rounds' graphs (HIP events)     */
+    double setup_ms;            /* wall time spent building admitted jobs' lattices    */
+    double finish_ms;           /* wall time spent delivering finished jobs           */
     double clock_ms;            /* server clock now                                    */
     int resident, queued, max_resident;
     size_t arena_bytes, arena_used, arena_peak;

@@ -91,12 +91,14 @@ __device__ inline void stream_store(S* p, S v) {
 #endif
 }
 
+// st: streamed store; stc: cached store (for scattered partial-line writes).
 struct StoreF64 {
     using S = double;
     using T = double;
     static constexpr bool kDev = false;
     __device__ static inline T ld(const S* p) { return *G(p); }
     __device__ static inline void st(S* p, T v) { stream_store<S>(p, v); }
+    __device__ static inline void stc(S* p, T v) { *G(p) = v; }
 };
 
 struct StoreF32 {
@@ -105,6 +107,7 @@ struct StoreF32 {
     static constexpr bool kDev = true;
     __device__ static inline T ld(const S* p) { return *G(p); }
     __device__ static inline void st(S* p, T v) { stream_store<S>(p, v); }
+    __device__ static inline void stc(S* p, T v) { *G(p) = v; }
 };
 
 struct StoreF16 {
@@ -113,6 +116,7 @@ struct StoreF16 {
     static constexpr bool kDev = true;
     __device__ static inline T ld(const S* p) { return (float)(*G(p)) * kHalfInv; }
     __device__ static inline void st(S* p, T v) { stream_store<S>(p, (_Float16)(v * kHalfScale)); }
+    __device__ static inline void stc(S* p, T v) { *G(p) = (_Float16)(v * kHalfScale); }
 };
 
 // Wrap a column / row index that is at most one step outside [0, n).

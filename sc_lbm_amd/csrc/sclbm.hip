@@ -139,6 +139,9 @@ struct scl_field {
     void* f[2] = {nullptr, nullptr};
     int cur = 0;
     int phase = PH_S;
+    // The geometry whose mask produced the post-collision populations (phase
+    // P, with the ghost slots of its solid sites).  Held by reference.
+    scl_geometry* pg = nullptr;
     // reference per-site arrays (FP64, pitched)
     double *rho = nullptr, *ux = nullptr, *uy = nullptr, *ueqx = nullptr, *ueqy = nullptr;
     double *frx = nullptr, *fry = nullptr, *fsx = nullptr, *fsy = nullptr, *feq = nullptr;
@@ -155,6 +158,17 @@ struct Replay {
     scl_field* fl[2] = {nullptr, nullptr};
     StepArgs<double> args{};
 };
+
+void geometry_release(scl_geometry* g);
+
+// Record that f's post-collision populations were produced with g's mask.
+void set_pgeom(scl_field* f, scl_geometry* g) {
+    if (f->pg == g) return;
+    scl_geometry* old = f->pg;
+    f->pg = g;
+    g->refs++;
+    if (old) geometry_release(old);
+}
 
 size_t n_sites(const scl_geometry* g) { return (size_t)g->pitch * g->ny; }
 size_t n_pop(const scl_geometry* g) { return (size_t)g->plane * Q; }
@@ -388,7 +402,7 @@ int materialize(scl_field* f) {
 int ensure_s(scl_field* f) {
     if (f->phase == PH_S) return SCL_OK;
     CHK(materialize(f));
-    const Grid2 gr = f->g->grid();
+    const Grid2 gr = (f->pg ? f->pg : f->g)->grid();
     const int nxt = f->cur ^ 1;
     switch (f->precision) {
         case SCL_FP64:
@@ -469,6 +483,10 @@ int fused_steps(const scl_geometry* g, scl_field** fl, const double* const* forc
     }
     if (nsteps <= 0) return SCL_OK;
     CHK(settle_replays(fl, NC));
+    // post-collision populations from another mask: back to the streamed
+    // state with that mask, then this step starts from it
+    for (int c = 0; c < NC; ++c)
+        if (fl[c]->phase == PH_P && fl[c]->pg != g) CHK(ensure_s(fl[c]));
     // both components must be in the same phase
     if (NC == 2 && fl[0]->phase != fl[1]->phase) {
         CHK(ensure_s(fl[0]));
@@ -488,6 +506,7 @@ int fused_steps(const scl_geometry* g, scl_field** fl, const double* const* forc
         for (int c = 0; c < NC; ++c) {
             fl[c]->cur ^= 1;
             fl[c]->phase = PH_P;
+            set_pgeom(fl[c], const_cast<scl_geometry*>(g));
             if (rec) fl[c]->replay = rec;
         }
     }
@@ -558,6 +577,12 @@ static void geometry_free(scl_geometry* g) {
     dfree(g->part);
     delete g;
 }
+
+namespace {
+void geometry_release(scl_geometry* g) {
+    if (--g->refs == 0 && g->dead) geometry_free(g);
+}
+}  // namespace
 
 int scl_geometry_destroy(scl_geometry* g) {
     if (!g) return SCL_OK;
@@ -657,8 +682,10 @@ int scl_field_destroy(scl_field* f) {
     for (double* p : {f->rho, f->ux, f->uy, f->ueqx, f->ueqy, f->frx, f->fry, f->fsx, f->fsy, f->feq, f->stage})
         dfree(p);
     scl_geometry* g = f->g;
+    scl_geometry* pg = f->pg;
     delete f;
-    if (--g->refs == 0 && g->dead) geometry_free(g);
+    if (pg) geometry_release(pg);
+    geometry_release(g);
     return SCL_OK;
 }
 
@@ -751,8 +778,8 @@ int scl_field_download(scl_field* f, int which, double* dst, size_t n) {
             CHK(copy_planes(g, f->f[f->cur], dst, 9, false));
         } else {
             CHK(ensure_stage(f));
-            const Grid2 gr = g->grid();
             const bool gth = f->phase == PH_P;
+            const Grid2 gr = (gth && f->pg ? f->pg : g)->grid();
             const void* s = f->f[f->cur];
             switch (f->precision) {
                 case SCL_FP64:
@@ -951,9 +978,9 @@ int scl_moments(const scl_geometry* g, scl_field* f, double out[3]) {
     CHK(check_geom(g, f));
     if (!out) return fail(SCL_ERR_ARG, "null output");
     CHK(ensure_stage(f));
-    const Grid2 gr = g->grid();
     const void* s = f->f[f->cur];
     const bool gth = f->phase == PH_P;
+    const Grid2 gr = (gth && f->pg ? f->pg : g)->grid();
     switch (f->precision) {
         case SCL_FP64:
             if (gth) hipLaunchKernelGGL((k_to_f64<StoreF64, true>), site_grid(g), dim3(256), 0, g_lib.stream, gr, (const double*)s, 0.0, f->stage);

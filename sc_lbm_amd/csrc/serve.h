@@ -201,22 +201,29 @@ __global__ void k_init_ic(Grid2 g, serve::IcParams p, int c, double rho0, typena
 // site is fluid, and by z itself as direction opp(i) otherwise, so
 //   P_i(z) = S_i(z + c_i)   if z + c_i is fluid,
 //          = S_opp(i)(z)    otherwise.
+// The gathering kernels read the bounce-back value of a fluid site x from
+// slot j of its solid source s = x - c_j (the ghost slot the stepping kernel
+// writes), so the ghost slots of solid sites are filled too:
+//   P_j(s) = S_j(s + c_j)   for solid s and fluid s + c_j.
 // Pure copies: Gather(Scatter(S)) == S bit for bit on fluid sites.
 template <class ST>
 __global__ void k_scatter(Grid2 g, const typename ST::S* __restrict__ src, typename ST::S* __restrict__ dst) {
     int x, y, s;
     if (!site_of(g, x, y, s)) return;
     const unsigned nb = nbits_at(g, x, y);
-    if (!(nb & 1u)) return;
     const int xm = wrap1(x - 1, g.nx), xp = wrap1(x + 1, g.nx);
     const int ym = wrap1(y - 1, g.ny), yp = wrap1(y + 1, g.ny);
+    const bool fl = nb & 1u;
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
         const int sx = cx(i) == 1 ? xp : (cx(i) == -1 ? xm : x);
         const int sy = cy(i) == 1 ? yp : (cy(i) == -1 ? ym : y);
-        const long long from = ((nb >> i) & 1u) ? (long long)i * g.plane + (long long)sy * g.pitch + sx
-                                                : (long long)opp(i) * g.plane + s;
-        dst[i * g.plane + s] = src[from];
+        const long long nbr = (long long)i * g.plane + (long long)sy * g.pitch + sx;   // slot i of z + c_i
+        if (fl) {
+            dst[i * g.plane + s] = ((nb >> i) & 1u) ? src[nbr] : src[(long long)opp(i) * g.plane + s];
+        } else if (i > 0 && ((nb >> i) & 1u)) {
+            dst[i * g.plane + s] = src[nbr];
+        }
     }
 }
 

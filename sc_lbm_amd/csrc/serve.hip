@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -116,6 +117,8 @@ class GpuServe final : public serve::Backend {
             if (p) (void)hipFree(p);
         for (void* p : {(void*)hdr_h_, table_h_, (void*)items_h_, (void*)mass_h_})
             if (p) (void)hipHostFree(p);
+        for (hipEvent_t e : ev_)
+            if (e) (void)hipEventDestroy(e);
         if (stream_) (void)hipStreamDestroy(stream_);
     }
 
@@ -126,6 +129,8 @@ class GpuServe final : public serve::Backend {
         SHIP(hipGetDeviceProperties(&prop, dev_));
         cus_ = prop.multiProcessorCount;
         SHIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));   // default priority
+        SHIP(hipEventCreate(&ev_[0]));
+        SHIP(hipEventCreate(&ev_[1]));
         if (cfg_.budget_bytes) {
             hipError_t e = hipMalloc(&arena_, cfg_.budget_bytes);
             if (e != hipSuccess) {
@@ -217,9 +222,14 @@ class GpuServe final : public serve::Backend {
         hipGraphExec_t ge = nullptr;
         rc = graph(rsteps, &ge, err);
         if (rc != SCL_OK) return rc;
+        SHIP(hipEventRecord(ev_[0], stream_));
         SHIP(hipGraphLaunch(ge, stream_));
+        SHIP(hipEventRecord(ev_[1], stream_));
         SHIP(hipMemcpyAsync(mass_h_, mass_d_, jobs.size() * 2 * sizeof(double), hipMemcpyDeviceToHost, stream_));
         SHIP(hipStreamSynchronize(stream_));
+        float ms = 0;
+        SHIP(hipEventElapsedTime(&ms, ev_[0], ev_[1]));
+        device_ms_.store(device_ms_.load() + ms);
         for (size_t k = 0; k < jobs.size(); ++k) {
             GJob* g = (GJob*)jobs[k]->dev;
             jobs[k]->mass[0] = mass_h_[2 * k];
@@ -269,6 +279,7 @@ class GpuServe final : public serve::Backend {
         *la = la_;
     }
     int workgroups() const override { return grid_; }
+    double device_ms() const override { return device_ms_.load(); }
 
   private:
     static size_t tiles_of(int nx, int ny, int nw, int rows) {
@@ -562,6 +573,8 @@ class GpuServe final : public serve::Backend {
     double* mass_h_ = nullptr;
     int nw_ = 4, rows_ = 32, la_ = 1, grid_ = 0, mass_grid_ = 1024;
     std::map<int, hipGraphExec_t> graphs_;
+    hipEvent_t ev_[2] = {nullptr, nullptr};
+    std::atomic<double> device_ms_{0.0};   // written by the request loop, read by stats
 };
 
 template <class ST>

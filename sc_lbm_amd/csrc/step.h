@@ -67,8 +67,10 @@ struct StepArgs {
 // Byte offsets of the 9 populations that site (col, r) reads: the pull
 // sources, or its own opposite slot where the source is solid.  The choice
 // is made with a bit mask rather than a branch, so the nine loads of a site
-// issue back to back without divergent control flow between them.
-template <class ST, bool GATHER, class OffT>
+// issue back to back without divergent control flow between them.  GHOST:
+// the source buffer holds the bounce-back values in the solid sources' slots
+// (see ghost_stores), so every pull reads its source slot.
+template <class ST, bool GATHER, class OffT, bool GHOST = false>
 __device__ inline void pull_offsets(OffT plane_b, OffT pitch_b, int col, int colm, int colp, int r, int rm,
                                     int rp, unsigned nb, OffT (&off)[Q]) {
     constexpr OffT es = sizeof(typename ST::S);
@@ -83,9 +85,13 @@ __device__ inline void pull_offsets(OffT plane_b, OffT pitch_b, int col, int col
             const OffT row = cy(i) == 1 ? os : (cy(i) == -1 ? on : orr);
             const OffT col_b = cx(i) == 1 ? cm : (cx(i) == -1 ? cp : cc);
             const OffT pull = (OffT)i * plane_b + row + col_b;
-            const OffT back = (OffT)opp(i) * plane_b + home;
-            const OffT keep = (OffT)0 - (OffT)((nb >> opp(i)) & 1u);   // all ones where the source is fluid
-            off[i] = back ^ ((pull ^ back) & keep);
+            if constexpr (GHOST) {
+                off[i] = pull;
+            } else {
+                const OffT back = (OffT)opp(i) * plane_b + home;
+                const OffT keep = (OffT)0 - (OffT)((nb >> opp(i)) & 1u);   // all ones where the source is fluid
+                off[i] = back ^ ((pull ^ back) & keep);
+            }
         }
     }
 }
@@ -193,14 +199,53 @@ __device__ inline float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
 __device__ inline bool is_dry(double r) { return fabs(r) <= kZeroTol * fmax(1.0, fabs(r)); }
 
+// Bounce-back mode of the gathering kernels: 0 selects the pull source per
+// direction from the mask bits; 1 / 2 read every pull from its source slot
+// and keep the bounce-back values in the solid sites' slots (ghost stores,
+// streamed / cached).
+#ifndef SCL_GHOST
+#define SCL_GHOST 0
+#endif
+
+// Byte offsets of rows y-1, y, y+1 and columns x-1, x, x+1 around a site
+// (periodic), for the ghost stores.
+template <class OffT>
+struct Nbr {
+    OffT rs, rc, rn, cm, cc, cp;
+};
+
+// Ghost stores.  A fluid site x whose neighbour x + c_i is solid also writes
+// its post-collision population i into slot opp(i) of that neighbour.  The
+// next step's pull of direction opp(i) at x reads exactly that slot, and the
+// value there is the link bounce-back value P_i(x), so the gather needs no
+// per-direction select and its loads do not wait for mask bits.  Slot j of a
+// solid site s is written only by the fluid site s + c_j, so there is one
+// writer per slot.  Solid sites are never read as sources of their own.
+template <class ST, class OffT>
+__device__ inline void ghost_stores(void* dst, OffT plane_b, const Nbr<OffT>& n, unsigned nb,
+                                    const typename ST::T (&v)[Q]) {
+    if (nb == 0x1FFu) return;   // no solid neighbour
+#pragma unroll
+    for (int i = 1; i < Q; ++i) {
+        if (!((nb >> i) & 1u)) {
+            const OffT row = cy(i) == 1 ? n.rn : (cy(i) == -1 ? n.rs : n.rc);
+            const OffT col = cx(i) == 1 ? n.cp : (cx(i) == -1 ? n.cm : n.cc);
+            typename ST::S* q = (typename ST::S*)((char*)dst + ((OffT)opp(i) * plane_b + row + col));
+            if (SCL_GHOST == 2) ST::stc(q, v[i]);
+            else ST::st(q, v[i]);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Per-site completion: everything after the gather, for one site whose
 // (gathered) populations are p, densities rho, neighbourhood bits nb and
 // neighbour densities nr[c][i] (i = 1..8, valid where bit i of nb is set).
 // ---------------------------------------------------------------------------
-template <class ST, bool REPLAY, class OffT>
+template <class ST, bool REPLAY, class OffT, bool GW>
 __device__ inline void finish1(const CompArgs<typename ST::T>& ca, void* dst, long long plane, OffT plane_b,
-                               int s_idx, OffT home, const typename ST::T (&p)[Q], typename ST::T rho_in, bool fl) {
+                               int s_idx, OffT home, const typename ST::T (&p)[Q], typename ST::T rho_in, bool fl,
+                               unsigned nb, const Nbr<OffT>& nbr) {
     if constexpr (!ST::kDev) {
         double ux = 0.0, uy = 0.0;
         const double rho = rho_in;
@@ -219,12 +264,14 @@ __device__ inline void finish1(const CompArgs<typename ST::T>& ca, void* dst, lo
 #pragma unroll
             for (int i = 0; i < Q; ++i) G(ca.feq)[i * plane + s_idx] = e[i];
         } else if (fl) {
+            double v[Q];
 #pragma unroll
             for (int i = 0; i < Q; ++i) {
-                double v = ca.omc * p[i] + ca.omega * e[i];
-                if (ca.has_force) v += ca.force[i];
-                st_at<ST>(dst, (OffT)i * plane_b + home, v);
+                v[i] = ca.omc * p[i] + ca.omega * e[i];
+                if (ca.has_force) v[i] += ca.force[i];
+                st_at<ST>(dst, (OffT)i * plane_b + home, v[i]);
             }
+            if constexpr (GW) ghost_stores<ST, OffT>(dst, plane_b, nbr, nb, v);
         }
     } else {
         if (!REPLAY && fl) {
@@ -237,15 +284,16 @@ __device__ inline void finish1(const CompArgs<typename ST::T>& ca, void* dst, lo
             relax_dev(p, rho, drho, jx * inv, jy * inv, ca.omega, ca.omc, ca.force, ca.has_force, out);
 #pragma unroll
             for (int i = 0; i < Q; ++i) st_at<ST>(dst, (OffT)i * plane_b + home, out[i]);
+            if constexpr (GW) ghost_stores<ST, OffT>(dst, plane_b, nbr, nb, out);
         }
     }
 }
 
-template <class ST, bool REPLAY, class OffT>
+template <class ST, bool REPLAY, class OffT, bool GW>
 __device__ inline void finish2(const StepArgs<typename ST::T>& a, void* const (&dst)[2], OffT plane_b, int s_idx,
                                OffT home,
                                const typename ST::T (&p)[2][Q], const typename ST::T (&rc)[2],
-                               const typename ST::T (&nr)[2][Q], unsigned nb) {
+                               const typename ST::T (&nr)[2][Q], unsigned nb, const Nbr<OffT>& nbr) {
     const bool fl = nb & 1u;
     const long long plane = a.plane;
     if constexpr (!ST::kDev) {
@@ -342,12 +390,14 @@ __device__ inline void finish2(const StepArgs<typename ST::T>& a, void* const (&
 #pragma unroll
             for (int c = 0; c < 2; ++c) {
                 const auto& C = a.c[c];
+                double v[Q];
 #pragma unroll
                 for (int i = 0; i < Q; ++i) {
-                    double v = C.omc * p[c][i] + C.omega * e[c][i];
-                    if (C.has_force) v += C.force[i];
-                    st_at<ST>(dst[c], (OffT)i * plane_b + home, v);
+                    v[i] = C.omc * p[c][i] + C.omega * e[c][i];
+                    if (C.has_force) v[i] += C.force[i];
+                    st_at<ST>(dst[c], (OffT)i * plane_b + home, v[i]);
                 }
+                if constexpr (GW) ghost_stores<ST, OffT>(dst[c], plane_b, nbr, nb, v);
             }
         }
     } else {
@@ -423,6 +473,10 @@ __device__ inline void finish2(const StepArgs<typename ST::T>& a, void* const (&
             for (int i = 0; i < Q; ++i) {
                 st_at<ST>(dst[0], (OffT)i * plane_b + home, oa[i]);
                 st_at<ST>(dst[1], (OffT)i * plane_b + home, ob[i]);
+            }
+            if constexpr (GW) {
+                ghost_stores<ST, OffT>(dst[0], plane_b, nbr, nb, oa);
+                ghost_stores<ST, OffT>(dst[1], plane_b, nbr, nb, ob);
             }
         }
     }
@@ -519,7 +573,8 @@ __device__ __forceinline__ void step_tile(const StepArgs<typename ST::T>& a, con
     // there) and its lane is simply masked off for the whole block.
     auto issue = [&](int col, int colm, int colp, int r, int rm, int rp, unsigned nb, T (&p)[NC][Q]) {
         OffT off[Q];
-        pull_offsets<ST, GATHER, OffT>(plane_b, pitch_b, col, colm, colp, r, rm, rp, nb, off);
+        pull_offsets<ST, GATHER, OffT, !REPLAY && SCL_GHOST != 0>(plane_b, pitch_b, col, colm, colp, r, rm, rp, nb,
+                                                                 off);
         if (!GATHER || (nb & 1u)) {
 #pragma unroll
             for (int c = 0; c < NC; ++c) {
@@ -635,9 +690,19 @@ __device__ __forceinline__ void step_tile(const StepArgs<typename ST::T>& a, con
         if (xin) {
             const int s_idx = y * a.pitch + x;
             const OffT home = (OffT)y * pitch_b + (OffT)x * ES;
+            constexpr bool GW = !OPEN && !REPLAY && SCL_GHOST != 0;   // ghost stores (see ghost_stores)
+            Nbr<OffT> nbr{};
+            if constexpr (GW) {
+                nbr.rs = (OffT)wrap1(y - 1, ny) * pitch_b;
+                nbr.rc = (OffT)y * pitch_b;
+                nbr.rn = (OffT)wrap1(y + 1, ny) * pitch_b;
+                nbr.cm = (OffT)xm * ES;
+                nbr.cc = (OffT)x * ES;
+                nbr.cp = (OffT)xp * ES;
+            }
             if constexpr (NC == 1) {
-                finish1<ST, REPLAY, OffT>(a.c[0], b.dst[0], a.plane, plane_b, s_idx, home, p0[0], dens(p0[0]),
-                                          OPEN || (nb0 & 1u));
+                finish1<ST, REPLAY, OffT, GW>(a.c[0], b.dst[0], a.plane, plane_b, s_idx, home, p0[0], dens(p0[0]),
+                                              OPEN || (nb0 & 1u), OPEN ? 0x1FFu : nb0, nbr);
             } else {
                 const int sS = k & 3, sC = (k + 1) & 3, sN = (k + 2) & 3;
                 T nr[2][Q], r0[2];
@@ -652,7 +717,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<typename ST::T>& a, con
                     r0[c] = lds[sC][c][tid + 1];
                     nr[c][0] = r0[c];
                 }
-                finish2<ST, REPLAY, OffT>(a, b.dst, plane_b, s_idx, home, p0, r0, nr, OPEN ? 0x1FFu : nb0);
+                finish2<ST, REPLAY, OffT, GW>(a, b.dst, plane_b, s_idx, home, p0, r0, nr, OPEN ? 0x1FFu : nb0, nbr);
             }
         }
         if (LA == 2 && more) {

@@ -520,6 +520,9 @@ int Scheduler::stats(scl_server_stats* st) {
     st->job_steps = job_steps_;
     st->lattice_updates = lups_;
     st->busy_ms = busy_ms_;
+    st->device_ms = be_->device_ms();
+    st->setup_ms = setup_ms_;
+    st->finish_ms = finish_ms_;
     st->clock_ms = now_ms();
     st->resident = (int)active_.size() + setting_up_;
     st->queued = (int)pending_.size();
@@ -630,6 +633,7 @@ void Scheduler::loop() {
         // 3. new lattices
         std::vector<std::pair<Job*, std::string>> setup_failed;
         std::vector<Job*> ready;
+        const double ts0 = now_ms();
         for (Job* j : admitted) {
             std::string err;
             const int rc = be_->setup(*j, &err);
@@ -640,7 +644,9 @@ void Scheduler::loop() {
                 ready.push_back(j);
             }
         }
+        const double ts1 = now_ms();
         lk.lock();
+        setup_ms_ += ts1 - ts0;
         for (auto& e : evict) {
             active_.erase(std::find(active_.begin(), active_.end(), e.first));
             retire(e.first, e.second, e.second == SCL_JOB_CANCELLED ? "cancelled" : "timed out while resident");
@@ -676,6 +682,7 @@ void Scheduler::loop() {
         // 4. watchdog and completions
         std::vector<std::pair<Job*, std::string>> failed;
         std::vector<Job*> complete;
+        double t_finish = 0;
         if (rc != SCL_OK) {
             for (Job* j : jobs) failed.emplace_back(j, "device round failed: " + err);
         } else {
@@ -706,8 +713,10 @@ void Scheduler::loop() {
                 }
                 if (j->steps_done >= j->spec.steps) {
                     std::string ferr;
+                    const double tf0 = now_ms();
                     if (be_->finish(*j, &ferr) != SCL_OK) failed.emplace_back(j, "delivery failed: " + ferr);
                     else complete.push_back(j);
+                    t_finish += now_ms() - tf0;
                 }
             }
         }
@@ -716,6 +725,7 @@ void Scheduler::loop() {
         lk.lock();
         ++rounds_;
         busy_ms_ += t1 - t0;
+        finish_ms_ += t_finish;
         if (rc == SCL_OK) {
             for (size_t k = 0; k < jobs.size(); ++k) {
                 job_steps_ += steps[k];

@@ -147,6 +147,8 @@ class Backend {
     // Launch shape in use (cols, rows, lookahead) and workgroups per step.
     virtual void shape(int* cols, int* rows, int* la) const = 0;
     virtual int workgroups() const { return 0; }
+    // Device time of the rounds so far, in milliseconds (0 if not measured).
+    virtual double device_ms() const { return 0.0; }
 };
 
 // ---------------------------------------------------------------------------
@@ -195,7 +197,7 @@ class Scheduler {
     // statistics
     long long n_submitted_ = 0, n_admitted_ = 0, n_done_ = 0, n_failed_ = 0, n_cancelled_ = 0,
               n_timed_out_ = 0, n_busy_ = 0, n_rejected_ = 0, rounds_ = 0, job_steps_ = 0, lups_ = 0;
-    double busy_ms_ = 0;
+    double busy_ms_ = 0, setup_ms_ = 0, finish_ms_ = 0;
     int max_resident_ = 0;
 };
 

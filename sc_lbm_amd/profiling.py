@@ -1,8 +1,9 @@
 """Lean drivers for profiler runs (rocprofv3 puts the program after `--`).
 
-    python -m sc_lbm_amd.profiling step --form fp32dev --n 4096 --steps 20
-        builds the 4096^2 two-component spinodal case, runs `steps` fused
-        steps of one storage form and prints the mean kernel time;
+    python -m sc_lbm_amd.profiling step --form fp32dev --n 4096 --steps 20 [--porosity 0.8]
+        builds the 4096^2 two-component spinodal case (optionally with seeded
+        disk obstacles), runs `steps` fused steps of one storage form and
+        prints the mean kernel time;
     python -m sc_lbm_amd.profiling copy --bytes 2147483648 --reps 10
         runs the plain copy kernel k_probe_copy (reads and writes `bytes`
         each launch) so the HBM byte counters can be calibrated.
@@ -11,6 +12,8 @@ import argparse
 import json
 import os
 import sys
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
@@ -26,6 +29,8 @@ def main(argv=None):
     s.add_argument("--steps", type=int, default=20)
     s.add_argument("--warmup", type=int, default=2)
     s.add_argument("--tile", default="")
+    s.add_argument("--porosity", type=float, default=1.0,
+                   help="below 1: seeded disk obstacles (radii 8..20) to this fluid fraction")
     c = sub.add_parser("copy")
     c.add_argument("--bytes", type=float, default=2 ** 31)
     c.add_argument("--reps", type=int, default=10)
@@ -41,6 +46,11 @@ def main(argv=None):
         return
     from sc_lbm_amd import workloads as wl
     case = wl.spinodal(a.n, G=1.2, seed=777)
+    if a.porosity < 1.0:
+        from sc_lbm_amd import serve
+        m = serve.job_mask(nx=a.n, ny=a.n, steps=1, porosity=a.porosity, rmin=8, rmax=20, geom_seed=1)
+        case["mask"] = m
+        case["f"] = [np.where(np.tile(m.ravel(), 9) == 1, f, 0.0) for f in case["f"]]
     g = nat.Geometry(case["mask"])
     if a.tile:
         v = [int(t) for t in a.tile.split(",")]
@@ -59,7 +69,7 @@ def main(argv=None):
     nat.timing(1)
     nat.step2(g, fl[0], fl[1], None, None, a.steps)
     avg, cnt = nat.timing(0)
-    print(json.dumps({"form": a.form, "n": a.n, "steps": a.steps, "kernel_avg_ms": avg, "launches": cnt,
+    print(json.dumps({"form": a.form, "n": a.n, "porosity": a.porosity, "steps": a.steps, "kernel_avg_ms": avg, "launches": cnt,
                       "tile": list(g.tile(2, nat.PRECISION[a.form])),
                       "mlups": a.n * a.n / (avg * 1e-3) / 1e6 if avg else None}))
 

@@ -69,6 +69,7 @@ class Stats(ctypes.Structure):
         ("failed", ctypes.c_longlong), ("cancelled", ctypes.c_longlong), ("timed_out", ctypes.c_longlong),
         ("rejected_busy", ctypes.c_longlong), ("rejected", ctypes.c_longlong), ("rounds", ctypes.c_longlong),
         ("job_steps", ctypes.c_longlong), ("lattice_updates", ctypes.c_longlong), ("busy_ms", ctypes.c_double),
+        ("device_ms", ctypes.c_double), ("setup_ms", ctypes.c_double), ("finish_ms", ctypes.c_double),
         ("clock_ms", ctypes.c_double), ("resident", ctypes.c_int), ("queued", ctypes.c_int),
         ("max_resident", ctypes.c_int), ("arena_bytes", ctypes.c_size_t), ("arena_used", ctypes.c_size_t),
         ("arena_peak", ctypes.c_size_t), ("tile_cols", ctypes.c_int), ("tile_rows", ctypes.c_int),
