@@ -5,7 +5,9 @@
 //   1. drops queued jobs that were cancelled or whose time-out passed, and
 //      marks resident ones for eviction;
 //   2. admits released jobs in (release, id) order while a slot is free and
-//      the arena can hold the job.  A job that does not fit is passed over,
+//      the arena can hold the job (a job whose mask is still being built
+//      holds the later ones back, so the order does not depend on thread
+//      timing).  A job that does not fit is passed over,
 //      until `max_bypass` later jobs have been admitted ahead of it; then
 //      admission waits for it (no starvation of large jobs);
 //   3. builds the new jobs' lattices, then runs one round: every resident
@@ -562,10 +564,7 @@ void Scheduler::admit_locked(double now, std::vector<Job*>* admitted) {
     for (auto it = pending_.begin(); it != pending_.end() && slots > 0;) {
         Job* j = *it;
         if (j->spec.release_ms > now) break;
-        if (!j->prepared) {   // its mask is still being built
-            ++it;
-            continue;
-        }
+        if (!j->prepared) break;   // its mask is being built: keep release order
         size_t off;
         if (arena_.alloc(j->bytes, &off)) {
             j->off = off;
