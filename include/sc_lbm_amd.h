@@ -165,14 +165,27 @@ int scl_set_tile(scl_geometry* g, int cols, int rows);
  * (1 or 2; 0 = automatic: 1 for FP64, 2 for the deviation forms).  Results
  * do not depend on it. */
 int scl_set_lookahead(scl_geometry* g, int rows_ahead);
+/* Time steps per launch on g (no reference counterpart).  Two-component
+ * deviation-form fields on lattices at least 256 columns wide can advance two
+ * steps per launch (the intermediate populations stay on chip; results are
+ * bit for bit those of single steps).  0 (the default): two per launch unless
+ * scl_autotune measured single steps faster; 2: two per launch wherever
+ * supported; 1: one per launch.  rows: rows per band of the two-step kernel
+ * (0 = autotuned, else 16 for FP32 and 32 for FP16 storage). */
+int scl_set_step_fusion(scl_geometry* g, int steps_per_launch, int rows);
+/* What two-component steps of `precision` on g use: steps per launch (1 or
+ * 2) and the band rows of the two-step kernel (0 when one per launch). */
+int scl_get_step_fusion(const scl_geometry* g, int precision, int* steps_per_launch, int* rows);
 /* The shape the fused kernel uses for `ncomp` components of `precision` on
  * g: the caller's overrides, else the autotuned one, else the default. */
 int scl_get_tile(const scl_geometry* g, int ncomp, int precision, int* cols, int* rows, int* lookahead);
 
-/* Autotuner (no reference counterpart): times the fused step of `ncomp`
- * components of `precision` on g for every launch shape (columns 64..512,
- * rows 4..128, lookahead 1..2) on scratch fields at rest, keeps the fastest
- * for later steps on g and returns it (outputs may be NULL).  Takes about
+/* Autotuner (no reference counterpart): times the fused single step of
+ * `ncomp` components of `precision` on g for every launch shape (columns
+ * 64..512, rows 4..128, lookahead 1..2) on scratch fields at rest, keeps the
+ * fastest for later steps on g and returns it (outputs may be NULL); for two
+ * components in a deviation form it then times the two-step kernel over band
+ * heights 8..64 and keeps two steps per launch only if faster.  Takes about
  * 3 ms of GPU time per shape; needs memory for two scratch fields. */
 int scl_autotune(scl_geometry* g, int ncomp, int precision, int* cols, int* rows, int* lookahead);
 

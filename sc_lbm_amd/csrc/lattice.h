@@ -99,6 +99,8 @@ struct StoreF64 {
     __device__ static inline T ld(const S* p) { return *G(p); }
     __device__ static inline void st(S* p, T v) { stream_store<S>(p, v); }
     __device__ static inline void stc(S* p, T v) { *G(p) = v; }
+    __device__ static inline S enc(T v) { return v; }   // value as stored
+    __device__ static inline T dec(S s) { return s; }   // stored value as loaded
 };
 
 struct StoreF32 {
@@ -108,6 +110,8 @@ struct StoreF32 {
     __device__ static inline T ld(const S* p) { return *G(p); }
     __device__ static inline void st(S* p, T v) { stream_store<S>(p, v); }
     __device__ static inline void stc(S* p, T v) { *G(p) = v; }
+    __device__ static inline S enc(T v) { return v; }
+    __device__ static inline T dec(S s) { return s; }
 };
 
 struct StoreF16 {
@@ -117,6 +121,8 @@ struct StoreF16 {
     __device__ static inline T ld(const S* p) { return (float)(*G(p)) * kHalfInv; }
     __device__ static inline void st(S* p, T v) { stream_store<S>(p, (_Float16)(v * kHalfScale)); }
     __device__ static inline void stc(S* p, T v) { *G(p) = (_Float16)(v * kHalfScale); }
+    __device__ static inline S enc(T v) { return (_Float16)(v * kHalfScale); }
+    __device__ static inline T dec(S s) { return (float)s * kHalfInv; }
 };
 
 // Wrap a column / row index that is at most one step outside [0, n).

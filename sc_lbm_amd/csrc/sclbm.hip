@@ -25,6 +25,7 @@
 #include "ops.h"
 #include "runtime.h"
 #include "step.h"
+#include "step2x.h"
 
 using namespace scl;
 
@@ -68,6 +69,7 @@ struct Lib {
     hipStream_t stream = nullptr;
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    std::vector<int> ev_steps;   // time steps covered by each recorded launch
 };
 Lib g_lib;
 const size_t kMaxTimed = 1 << 15;
@@ -114,6 +116,9 @@ struct scl_geometry {
     int lookahead = 0;
     int tuned[2][3][3] = {};            // [ncomp-1][precision] -> {cols, rows, lookahead}
     bool open = false;                  // no solid site: the fused kernel skips the mask
+    int fusion = 0;                     // time steps per launch: 0 automatic, 1, or 2 where supported
+    int x2_rows = 0;                    // rows per band of the two-step kernel (0 = automatic)
+    int x2_tuned[3] = {};               // [precision]: autotuned band rows; -1: single steps were faster
     double* part = nullptr;             // moment partials (3 per row + 3)
     // Fields keep their geometry alive: destroying a geometry that fields
     // still use only marks it, and the last field destroyed frees it.
@@ -140,8 +145,14 @@ struct scl_field {
     int cur = 0;
     int phase = PH_S;
     // The geometry whose mask produced the post-collision populations (phase
-    // P, with the ghost slots of its solid sites).  Held by reference.
+    // P).  Held by reference.
     scl_geometry* pg = nullptr;
+    // Deviation forms: equilibrium velocities of the zero-density rule in
+    // three rotating buffers (x, y); a step reads buffer ucur and writes the
+    // next, so steps fused into one launch never read what the same launch
+    // writes.  ueqx / ueqy point at buffer ucur.
+    double* ueq3[3][2] = {};
+    int ucur = 0;
     // reference per-site arrays (FP64, pitched)
     double *rho = nullptr, *ux = nullptr, *uy = nullptr, *ueqx = nullptr, *ueqy = nullptr;
     double *frx = nullptr, *fry = nullptr, *fsx = nullptr, *fsy = nullptr, *feq = nullptr;
@@ -168,6 +179,14 @@ void set_pgeom(scl_field* f, scl_geometry* g) {
     f->pg = g;
     g->refs++;
     if (old) geometry_release(old);
+}
+
+// The deviation forms' equilibrium-velocity buffers after `steps` steps.
+void advance_ueq(scl_field* f, int steps) {
+    if (f->precision == SCL_FP64) return;
+    f->ucur = (f->ucur + steps) % 3;
+    f->ueqx = f->ueq3[f->ucur][0];
+    f->ueqy = f->ueq3[f->ucur][1];
 }
 
 size_t n_sites(const scl_geometry* g) { return (size_t)g->pitch * g->ny; }
@@ -286,8 +305,18 @@ void fill_comp(CompArgs<T>& c, const scl_field* f, int src_buf, const double* fo
     c.rho0 = (T)f->rho0;
     c.has_force = force ? 1 : 0;
     for (int i = 0; i < Q; ++i) c.force[i] = force ? (T)force[i] : T(0);
-    c.ueqx = f->ueqx;
-    c.ueqy = f->ueqy;
+    if (f->precision == SCL_FP64) {
+        c.ueqx = f->ueqx;
+        c.ueqy = f->ueqy;
+        c.ueqx_in = f->ueqx;
+        c.ueqy_in = f->ueqy;
+    } else {
+        const int nxt = (f->ucur + 1) % 3;
+        c.ueqx_in = f->ueq3[f->ucur][0];
+        c.ueqy_in = f->ueq3[f->ucur][1];
+        c.ueqx = f->ueq3[nxt][0];
+        c.ueqy = f->ueq3[nxt][1];
+    }
     c.rho = f->rho;
     c.jx = f->ux;
     c.jy = f->uy;
@@ -345,6 +374,7 @@ int launch_fused(const scl_geometry* g, scl_field* const* fl, const double* cons
     if (e0) {
         HIPCHK(hipEventRecord(e1, g_lib.stream));
         g_lib.ev.emplace_back(e0, e1);
+        g_lib.ev_steps.push_back(1);
     }
     if (rec) {
         if constexpr (std::is_same<ST, StoreF64>::value) {
@@ -359,6 +389,66 @@ int launch_fused(const scl_geometry* g, scl_field* const* fl, const double* cons
             r->args = ra;
             *rec = r;
         }
+    }
+    return SCL_OK;
+}
+
+// Two fused steps (deviation forms, two components, gathering): P_{n+1}
+// from P_{n-1} in one launch (k_step2x).
+bool x2_supported(const scl_geometry* g, int precision) {
+    static const size_t esz[3] = {8, 4, 2};
+    return precision != SCL_FP64 && g->nx >= kX2Threads && !needs_wide(g, esz[precision]);
+}
+
+// whether two-component steps of `precision` on g go two per launch
+bool x2_wanted(const scl_geometry* g, int precision) {
+    if (g->fusion == 1 || !x2_supported(g, precision)) return false;
+    return g->fusion == 2 || g->x2_tuned[precision] >= 0;
+}
+
+int x2_rows_for(const scl_geometry* g, int precision) {
+    // short bands give many workgroups (two per CU fit in LDS for FP32
+    // storage, three for FP16); measured best at 4096^2: 16 (FP32), 32 (FP16)
+    const int r = g->x2_rows ? g->x2_rows : (g->x2_tuned[precision] > 0 ? g->x2_tuned[precision]
+                                                                       : (precision == SCL_FP32_DEV ? 16 : 32));
+    return std::min(r, g->ny);
+}
+
+bool x2_ok(const scl_geometry* g, scl_field* const* fl, int nc) {
+    return nc == 2 && fl[0]->phase == PH_P && fl[1]->phase == PH_P && x2_wanted(g, fl[0]->precision);
+}
+
+template <class ST>
+int launch_x2(const scl_geometry* g, scl_field* const* fl, const double* const* forces) {
+    const int rows = x2_rows_for(g, fl[0]->precision);
+    StepArgs<float> a1 = make_args<float>(g, rows, 4);
+    a1.gx = (g->nx + kX2Out - 1) / kX2Out;
+    for (int c = 0; c < 2; ++c) fill_comp(a1.c[c], fl[c], fl[c]->cur, forces[c]);
+    StepArgs<float> a2 = a1;
+    for (int c = 0; c < 2; ++c) {
+        const scl_field* f = fl[c];
+        const int k1 = (f->ucur + 1) % 3, k2 = (f->ucur + 2) % 3;
+        a2.c[c].ueqx_in = f->ueq3[k1][0];
+        a2.c[c].ueqy_in = f->ueq3[k1][1];
+        a2.c[c].ueqx = f->ueq3[k2][0];
+        a2.c[c].ueqy = f->ueq3[k2][1];
+    }
+    const int nblk = a1.gx * ((g->ny + rows - 1) / rows);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (g_lib.timing && g_lib.ev.size() < kMaxTimed) {
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, g_lib.stream));
+    }
+    if (g->open)
+        hipLaunchKernelGGL((k_step2x<ST, true>), dim3(nblk), dim3(kX2Threads), 0, g_lib.stream, a1, a2);
+    else
+        hipLaunchKernelGGL((k_step2x<ST, false>), dim3(nblk), dim3(kX2Threads), 0, g_lib.stream, a1, a2);
+    HIPCHK(hipGetLastError());
+    if (e0) {
+        HIPCHK(hipEventRecord(e1, g_lib.stream));
+        g_lib.ev.emplace_back(e0, e1);
+        g_lib.ev_steps.push_back(2);
     }
     return SCL_OK;
 }
@@ -492,7 +582,20 @@ int fused_steps(const scl_geometry* g, scl_field** fl, const double* const* forc
         CHK(ensure_s(fl[0]));
         CHK(ensure_s(fl[1]));
     }
-    for (int t = 0; t < nsteps; ++t) {
+    for (int t = 0; t < nsteps;) {
+        if constexpr (NC == 2) {
+            if (nsteps - t >= 2 && x2_ok(g, fl, NC)) {
+                const int rc = prec == SCL_FP32_DEV ? launch_x2<StoreF32>(g, fl, forces)
+                                                    : launch_x2<StoreF16>(g, fl, forces);
+                if (rc != SCL_OK) return rc;
+                for (int c = 0; c < NC; ++c) {
+                    fl[c]->cur ^= 1;
+                    advance_ueq(fl[c], 2);
+                }
+                t += 2;
+                continue;
+            }
+        }
         const bool gather = fl[0]->phase == PH_P;
         std::shared_ptr<Replay> rec;
         const bool last = t == nsteps - 1;
@@ -508,7 +611,9 @@ int fused_steps(const scl_geometry* g, scl_field** fl, const double* const* forc
             fl[c]->phase = PH_P;
             set_pgeom(fl[c], const_cast<scl_geometry*>(g));
             if (rec) fl[c]->replay = rec;
+            advance_ueq(fl[c], 1);
         }
+        t += 1;
     }
     return SCL_OK;
 }
@@ -611,6 +716,25 @@ int scl_set_lookahead(scl_geometry* g, int rows_ahead) {
     return SCL_OK;
 }
 
+int scl_set_step_fusion(scl_geometry* g, int steps_per_launch, int rows) {
+    if (!g) return fail(SCL_ERR_ARG, "null geometry");
+    if (steps_per_launch < 0 || steps_per_launch > 2)
+        return fail(SCL_ERR_ARG, "steps per launch must be 0 (automatic), 1 or 2");
+    if (rows < 0 || rows > 4096) return fail(SCL_ERR_ARG, "rows must be in 0..4096");
+    g->fusion = steps_per_launch;
+    g->x2_rows = rows;
+    return SCL_OK;
+}
+
+int scl_get_step_fusion(const scl_geometry* g, int precision, int* steps_per_launch, int* rows) {
+    if (!g || !steps_per_launch || !rows) return fail(SCL_ERR_ARG, "null argument");
+    if (precision < SCL_FP64 || precision > SCL_FP16_DEV) return fail(SCL_ERR_ARG, "bad precision");
+    const bool two = x2_wanted(g, precision);
+    *steps_per_launch = two ? 2 : 1;
+    *rows = two ? x2_rows_for(g, precision) : 0;
+    return SCL_OK;
+}
+
 int scl_get_tile(const scl_geometry* g, int ncomp, int precision, int* cols, int* rows, int* lookahead) {
     if (!g || !cols || !rows || !lookahead) return fail(SCL_ERR_ARG, "null argument");
     if (ncomp < 1 || ncomp > 2 || precision < SCL_FP64 || precision > SCL_FP16_DEV)
@@ -656,10 +780,20 @@ int scl_field_create(const scl_geometry* g, int precision, double tau, double rh
         HIPCHK(hipGetLastError());
     }
     const size_t n = n_sites(g);
-    CHK(dalloc(&f->ueqx, n));
-    CHK(dalloc(&f->ueqy, n));
-    HIPCHK(hipMemsetAsync(f->ueqx, 0, n * sizeof(double), g_lib.stream));
-    HIPCHK(hipMemsetAsync(f->ueqy, 0, n * sizeof(double), g_lib.stream));
+    if (precision == SCL_FP64) {
+        CHK(dalloc(&f->ueqx, n));
+        CHK(dalloc(&f->ueqy, n));
+        HIPCHK(hipMemsetAsync(f->ueqx, 0, n * sizeof(double), g_lib.stream));
+        HIPCHK(hipMemsetAsync(f->ueqy, 0, n * sizeof(double), g_lib.stream));
+    } else {
+        for (int k = 0; k < 3; ++k)
+            for (int d = 0; d < 2; ++d) {
+                CHK(dalloc(&f->ueq3[k][d], n));
+                HIPCHK(hipMemsetAsync(f->ueq3[k][d], 0, n * sizeof(double), g_lib.stream));
+            }
+        f->ueqx = f->ueq3[0][0];
+        f->ueqy = f->ueq3[0][1];
+    }
     f->g->refs++;
     *out = f.release();
     return SCL_OK;
@@ -679,8 +813,14 @@ int scl_field_destroy(scl_field* f) {
         (void)hipStreamSynchronize(g_lib.stream);
     }
     for (void* p : {f->f[0], f->f[1]}) dfree(p);
-    for (double* p : {f->rho, f->ux, f->uy, f->ueqx, f->ueqy, f->frx, f->fry, f->fsx, f->fsy, f->feq, f->stage})
-        dfree(p);
+    if (f->precision == SCL_FP64) {
+        dfree(f->ueqx);
+        dfree(f->ueqy);
+    } else {
+        for (auto& k : f->ueq3)
+            for (double* p : k) dfree(p);
+    }
+    for (double* p : {f->rho, f->ux, f->uy, f->frx, f->fry, f->fsx, f->fsy, f->feq, f->stage}) dfree(p);
     scl_geometry* g = f->g;
     scl_geometry* pg = f->pg;
     delete f;
@@ -1014,6 +1154,7 @@ int scl_timing(int reset, double* avg_ms, int* count) {
             (void)hipEventDestroy(p.second);
         }
         g_lib.ev.clear();
+        g_lib.ev_steps.clear();
         g_lib.timing = reset > 0;
         if (avg_ms) *avg_ms = 0.0;
         if (count) *count = 0;
@@ -1021,12 +1162,14 @@ int scl_timing(int reset, double* avg_ms, int* count) {
     }
     HIPCHK(hipStreamSynchronize(g_lib.stream));
     double tot = 0.0;
-    for (auto& p : g_lib.ev) {
+    long long steps = 0;
+    for (size_t k = 0; k < g_lib.ev.size(); ++k) {
         float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, p.first, p.second));
+        HIPCHK(hipEventElapsedTime(&ms, g_lib.ev[k].first, g_lib.ev[k].second));
         tot += ms;
+        steps += g_lib.ev_steps[k];
     }
-    if (avg_ms) *avg_ms = g_lib.ev.empty() ? 0.0 : tot / g_lib.ev.size();
+    if (avg_ms) *avg_ms = steps ? tot / steps : 0.0;
     if (count) *count = (int)g_lib.ev.size();
     return SCL_OK;
 }
@@ -1059,7 +1202,9 @@ int scl_autotune(scl_geometry* g, int ncomp, int precision, int* cols, int* rows
     if (rc == SCL_OK && ncomp == 2) rc = scl_solid_surface_force(g, fl[0], fl[1]);
     // candidates (the caller's own overrides are set aside while timing)
     const int saved[3] = {g->tile_cols, g->tile_rows, g->lookahead};
+    const int saved_fusion = g->fusion, saved_x2 = g->x2_rows;
     g->tile_cols = g->tile_rows = g->lookahead = 0;
+    g->fusion = 1;   // single-step shapes first
     int* slot = g->tuned[ncomp - 1][precision];
     const int saved_tuned[3] = {slot[0], slot[1], slot[2]};
     double best = 1e300;
@@ -1109,6 +1254,45 @@ int scl_autotune(scl_geometry* g, int ncomp, int precision, int* cols, int* rows
             }
         }
     }
+    // two steps per launch: rows per band, kept only if faster than single steps
+    if (rc == SCL_OK && bc && ncomp == 2 && x2_supported(g, precision)) {
+        slot[0] = bc;
+        slot[1] = br;
+        slot[2] = bl;
+        g->fusion = 2;
+        double best2 = 1e300;
+        int b2 = 0;
+        const int bandset[5] = {8, 16, 24, 32, 64};
+        for (int bi = 0; bi < 5 && rc == SCL_OK; ++bi) {
+            if (bandset[bi] > g->ny && bi > 0) continue;
+            g->x2_rows = bandset[bi];
+            const double* none[2] = {nullptr, nullptr};
+            rc = fused_steps<2>(g, fl, none, 2);
+            int reps = 4;
+            float ms = 0.f;
+            for (int pass = 0; pass < 2 && rc == SCL_OK; ++pass) {
+                if (hipEventRecord(e0, g_lib.stream) != hipSuccess) rc = fail(SCL_ERR_HIP, "event record failed");
+                if (rc == SCL_OK) rc = fused_steps<2>(g, fl, none, reps);
+                if (rc == SCL_OK && (hipEventRecord(e1, g_lib.stream) != hipSuccess ||
+                                     hipEventSynchronize(e1) != hipSuccess ||
+                                     hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                    rc = fail(SCL_ERR_HIP, "event timing failed");
+                if (pass == 0 && rc == SCL_OK) {
+                    const double per = ms / reps;
+                    int want = per > 0 ? (int)(3.0 / per) : 50;
+                    want = want < 4 ? 4 : (want > 50 ? 50 : want);
+                    reps = want & ~1;
+                }
+            }
+            if (rc == SCL_OK && ms / reps < best2) {
+                best2 = ms / reps;
+                b2 = bandset[bi];
+            }
+        }
+        if (rc == SCL_OK) g->x2_tuned[precision] = best2 < best ? b2 : -1;
+    }
+    g->fusion = saved_fusion;
+    g->x2_rows = saved_x2;
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     g->tile_cols = saved[0];

@@ -32,6 +32,10 @@ struct JobArgs {
     int cur;
     int steps;                  // steps of this job in the current round
     int item_begin, item_end;   // its tiles in the work-list
+    // deviation forms: rotating equilibrium-velocity buffers [k][component]
+    // [x/y]; step s reads buffer (ucur + s) % 3 and writes the next
+    double* ueq[3][2][2];
+    int ucur;
 };
 
 struct ServeHdr {
@@ -80,7 +84,17 @@ k_serve(const ServeHdr* __restrict__ hdr, int s, unsigned* __restrict__ ctr) {
         y1 = __builtin_amdgcn_readfirstlane(y1);
         const SCL_AS_CONST JobArgs<T>& J = jobs[j];
         if (s >= J.steps) continue;
-        const StepArgs<T> a = J.a;
+        StepArgs<T> a = J.a;
+        if constexpr (ST::kDev) {
+            const int ki = (J.ucur + s) % 3, ko = (J.ucur + s + 1) % 3;
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                a.c[c].ueqx_in = J.ueq[ki][c][0];
+                a.c[c].ueqy_in = J.ueq[ki][c][1];
+                a.c[c].ueqx = J.ueq[ko][c][0];
+                a.c[c].ueqy = J.ueq[ko][c][1];
+            }
+        }
         Bufs b;
         const int sw = (J.cur ^ s) & 1;
 #pragma unroll

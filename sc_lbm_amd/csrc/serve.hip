@@ -57,7 +57,8 @@ size_t align256(size_t n) { return (n + 255) / 256 * 256; }
 struct Layout {
     int pitch = 0, wpr = 0;
     long long plane = 0;
-    size_t mask = 0, f[2][2] = {}, ueq[2][2] = {}, rho[2] = {}, stage = 0, total = 0;
+    size_t mask = 0, f[2][2] = {}, ueq[3][2][2] = {}, rho[2] = {}, stage = 0, total = 0;
+    int nueq = 1;   // equilibrium-velocity buffer sets: 1 (FP64) or 3 (deviation forms, rotating)
     bool has_stage = false;
 };
 
@@ -75,8 +76,10 @@ Layout layout_of(const JobSpec& s, size_t es) {
     L.mask = take((size_t)s.ny * L.wpr * 8);
     for (int c = 0; c < 2; ++c)
         for (int b = 0; b < 2; ++b) L.f[c][b] = take((size_t)Q * L.plane * es);
-    for (int c = 0; c < 2; ++c)
-        for (int d = 0; d < 2; ++d) L.ueq[c][d] = take((size_t)L.plane * 8);
+    L.nueq = es == 8 ? 1 : 3;
+    for (int k = 0; k < L.nueq; ++k)
+        for (int c = 0; c < 2; ++c)
+            for (int d = 0; d < 2; ++d) L.ueq[k][c][d] = take((size_t)L.plane * 8);
     for (int c = 0; c < 2; ++c) L.rho[c] = take((size_t)L.plane * 8);
     L.has_stage = !s.f[0].empty() || (s.deliver & SCL_DELIVER_F);
     if (L.has_stage) L.stage = take((size_t)Q * L.plane * 8);
@@ -93,7 +96,8 @@ struct GJob {
         return Grid2{nx, ny, L.pitch, L.wpr, L.plane, (const uint64_t*)(base + L.mask)};
     }
     void* f(int c, int b) const { return base + L.f[c][b]; }
-    double* ueq(int c, int d) const { return (double*)(base + L.ueq[c][d]); }
+    double* ueq(int k, int c, int d) const { return (double*)(base + L.ueq[k][c][d]); }
+    int ucur = 0;   // deviation forms: buffer set holding the latest equilibrium velocities
     double* rho(int c) const { return (double*)(base + L.rho[c]); }
     double* stage() const { return (double*)(base + L.stage); }
     int nx = 0, ny = 0;
@@ -211,6 +215,7 @@ class GpuServe final : public serve::Backend {
             ja.a = args_of(jobs[k]->spec, *g);
             ja.cur = g->cur;
             ja.steps = steps[k];
+            fill_ueq(ja, *g);
             ja.item_begin = n;
             n = emit_tiles(k, g->nx, g->ny, nw_, rows_, items_h_, n);
             ja.item_end = n;
@@ -235,6 +240,7 @@ class GpuServe final : public serve::Backend {
             jobs[k]->mass[0] = mass_h_[2 * k];
             jobs[k]->mass[1] = mass_h_[2 * k + 1];
             g->cur ^= steps[k] & 1;
+            g->ucur = (g->ucur + steps[k]) % 3;
         }
         return SCL_OK;
     }
@@ -282,6 +288,13 @@ class GpuServe final : public serve::Backend {
     double device_ms() const override { return device_ms_.load(); }
 
   private:
+    static void fill_ueq(JobArgs<T>& ja, const GJob& g) {
+        for (int k = 0; k < 3; ++k)
+            for (int c = 0; c < 2; ++c)
+                for (int d = 0; d < 2; ++d) ja.ueq[k][c][d] = g.ueq(k < g.L.nueq ? k : 0, c, d);
+        ja.ucur = g.ucur;
+    }
+
     static size_t tiles_of(int nx, int ny, int nw, int rows) {
         return (size_t)((nx + 64 * nw - 1) / (64 * nw)) * (size_t)((ny + rows - 1) / rows);
     }
@@ -337,8 +350,10 @@ class GpuServe final : public serve::Backend {
             ca.rho0 = (T)(ST::kDev ? 1.0 : 0.0);
             ca.has_force = s.has_force[c] ? 1 : 0;
             for (int i = 0; i < Q; ++i) ca.force[i] = (T)s.force[c][i];
-            ca.ueqx = g.ueq(c, 0);
-            ca.ueqy = g.ueq(c, 1);
+            ca.ueqx = g.ueq(0, c, 0);
+            ca.ueqy = g.ueq(0, c, 1);
+            ca.ueqx_in = ca.ueqx;
+            ca.ueqy_in = ca.ueqy;
         }
         return a;
     }
@@ -509,6 +524,7 @@ class GpuServe final : public serve::Backend {
                         table[k].a = args_of(jobs[k]->spec, *g);
                         table[k].cur = g->cur;
                         table[k].steps = 10;
+                        fill_ueq(table[k], *g);
                         table[k].item_begin = n;
                         n = emit_tiles(k, g->nx, g->ny, nw, rows, items_h_, n);
                         table[k].item_end = n;

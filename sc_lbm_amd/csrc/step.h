@@ -45,6 +45,10 @@ struct CompArgs {
     int has_force;
     double* ueqx;      // equilibrium velocity store (dry-site fallback / replay output)
     double* ueqy;
+    // Deviation forms: the equilibrium velocities of the previous step, read
+    // at zero-density sites (rotating buffers; FP64 reads and writes ueqx/ueqy)
+    const double* ueqx_in;
+    const double* ueqy_in;
     // replay outputs (FP64, pitched), written only by the replay variant
     double* rho;
     double* jx;
@@ -155,33 +159,35 @@ __device__ inline void feq_ref(double rho, double ux, double uy, double (&e)[Q])
 }
 
 // BGK relaxation of one component in deviation form, FP32:
-//   d_i' = (1 - omega) d_i + omega (feq_i - w_i rho0) [+ F_i],
-//   feq_i - w_i rho0 = w_i drho + w_i rho (3 cu + 4.5 cu^2 - 1.5 u^2),
+//   d_i' = d_i + omega (deq_i - d_i) [+ F_i],
+//   deq_i = feq_i - w_i rho0 = w_i drho + w_i rho (3 cu + 4.5 cu^2 - 1.5 u^2),
 // with drho = rho - rho0 so small perturbations keep their digits.  Opposite
 // directions share cu^2 (their polynomials differ only in the sign of the
-// 3 cu term) and omega is folded into the weights, so each direction costs
-// two fused multiply-adds.
+// 3 cu term).
 //
-// The rest population closes the site's mass balance, d_0' = drho -
-// sum_{i>0} d_i', before the force is added.  Collision conserves mass; in
-// FP32 the rounded weights omega*w_i do not sum to omega, and evaluating d_0'
-// from its own formula added (rho - rho0) times that rounding to the mass of
-// every site at every step, always with the same sign: in a 1.0 / 0.06
-// two-phase lattice with tau = 0.8 that bias had grown to 1e-4 in density
-// after 1000 steps.  With the closure the error against FP64 stays at the
-// rounding level of the stored deviations.
+// Two choices keep the FP32 result close to FP64 over long runs:
+//  * the update acts on the non-equilibrium part deq_i - d_i, which is small,
+//    rather than adding (1 - omega) d_i and omega deq_i, which nearly cancel
+//    for omega near 2 (a porous two-phase lattice at tau = 0.707 drifted 4x
+//    further from FP64 with the latter form);
+//  * the rest population closes the site's mass balance, d_0' = drho -
+//    sum_{i>0} d_i', before the force is added.  Collision conserves mass;
+//    evaluating d_0' from its own formula added (rho - rho0) times the
+//    rounding of the weights to the mass of every site at every step, always
+//    with the same sign, which had grown to 1e-4 in density after 1000 steps
+//    of a 1.0 / 0.06 droplet at tau = 0.8.
 __device__ inline void relax_dev(const float (&p)[Q], float rho, float drho, float ux, float uy, float omega,
-                                 float omc, const float* force, bool has_force, float (&out)[Q]) {
+                                 const float* force, bool has_force, float (&out)[Q]) {
 #pragma clang fp contract(fast)
     const float usq15 = 1.5f * (ux * ux + uy * uy);
-    const float ow1 = omega * (1.0f / 9.0f), ow2 = omega * (1.0f / 36.0f);
-    const float wd1 = ow1 * drho, wr1 = ow1 * rho;
-    const float wd2 = ow2 * drho, wr2 = ow2 * rho;
+    const float wd1 = (1.0f / 9.0f) * drho, wr1 = (1.0f / 9.0f) * rho;
+    const float wd2 = (1.0f / 36.0f) * drho, wr2 = (1.0f / 36.0f) * rho;
     auto pair = [&](int i, int j, float cu, float wd, float wr) {
         const float t = (4.5f * cu) * cu - usq15;
         const float c3 = 3.0f * cu;
-        out[i] = omc * p[i] + (wd + wr * (t + c3));
-        out[j] = omc * p[j] + (wd + wr * (t - c3));
+        const float ei = wd + wr * (t + c3), ej = wd + wr * (t - c3);
+        out[i] = p[i] + omega * (ei - p[i]);
+        out[j] = p[j] + omega * (ej - p[j]);
     };
     pair(1, 3, ux, wd1, wr1);
     pair(2, 4, uy, wd1, wr1);
@@ -281,12 +287,95 @@ __device__ inline void finish1(const CompArgs<typename ST::T>& ca, void* dst, lo
             momentum(p, jx, jy);
             const float inv = rcp(rho);
             float out[Q];
-            relax_dev(p, rho, drho, jx * inv, jy * inv, ca.omega, ca.omc, ca.force, ca.has_force, out);
+            relax_dev(p, rho, drho, jx * inv, jy * inv, ca.omega, ca.force, ca.has_force, out);
 #pragma unroll
             for (int i = 0; i < Q; ++i) st_at<ST>(dst, (OffT)i * plane_b + home, out[i]);
             if constexpr (GW) ghost_stores<ST, OffT>(dst, plane_b, nbr, nb, out);
         }
     }
+}
+
+// Two-component collision of one fluid site in the deviation forms (FP32
+// arithmetic): fluid-fluid and fluid-solid forces from the neighbour
+// densities nr (valid where bit i of nb is set), the composite and
+// equilibrium velocities, and the relaxation.  Zero-density rule: a dry
+// component takes the equilibrium velocity of the previous step from
+// ueqx_in / ueqy_in and carries it forward into ueqx / ueqy; a component
+// below kUeqWatch records its new one there.
+template <class T>
+__device__ inline void collide2_dev(const StepArgs<T>& a, int s_idx, const float (&p)[2][Q], const float (&rc)[2],
+                                    const float (&nr)[2][Q], unsigned nb, float (&oa)[Q], float (&ob)[Q]) {
+#pragma clang fp contract(fast)
+    const auto& A = a.c[0];
+    const auto& B = a.c[1];
+    float sd[2][2] = {{0.f, 0.f}, {0.f, 0.f}};   // sums over fluid nbrs of w c drho_other
+    float sfx = 0.f, sfy = 0.f;                  // sums over fluid nbrs of w c
+#pragma unroll
+    for (int i = 1; i < Q; ++i) {
+        const bool f = (nb >> i) & 1u;
+        const float da = f ? nr[0][i] : 0.f, db = f ? nr[1][i] : 0.f;
+        const float wx = (float)(wint(i) * cx(i)), wy = (float)(wint(i) * cy(i));
+        if (cx(i) != 0) {
+            sd[0][0] += wx * db;
+            sd[1][0] += wx * da;
+            sfx += f ? wx : 0.f;
+        }
+        if (cy(i) != 0) {
+            sd[0][1] += wy * db;
+            sd[1][1] += wy * da;
+            sfy += f ? wy : 0.f;
+        }
+    }
+    const float da0 = rc[0], db0 = rc[1];
+    const float ra = A.rho0 + da0, rb = B.rho0 + db0;
+    // psi_other over fluid neighbours = sum w c drho + rho0_other * sum w c
+    const float ga = A.gneg * ra, gb = B.gneg * rb;
+    const float fa_x = ga * (sd[0][0] + B.rho0 * sfx);
+    const float fa_y = ga * (sd[0][1] + B.rho0 * sfy);
+    const float fb_x = gb * (sd[1][0] + A.rho0 * sfx);
+    const float fb_y = gb * (sd[1][1] + A.rho0 * sfy);
+    // adhesion: sum over solid nbrs of w c = -(sum over fluid nbrs)
+    const float fs_ax = A.gsneg * -sfx, fs_ay = A.gsneg * -sfy;
+    const float fs_bx = B.gsneg * -sfx, fs_by = B.gsneg * -sfy;
+    float jax, jay, jbx, jby;
+    momentum(p[0], jax, jay);
+    momentum(p[1], jbx, jby);
+    const float inv_den = rcp(ra * A.omega + rb * B.omega);
+    const float ucx = (jax * A.omega + jbx * B.omega) * inv_den;
+    const float ucy = (jay * A.omega + jby * B.omega) * inv_den;
+    float uea[2], ueb[2];
+    if (!is_dry(ra)) {
+        const float ir = rcp(ra);
+        uea[0] = ucx + A.inv_omega * (fa_x * ir + fs_ax);
+        uea[1] = ucy + A.inv_omega * (fa_y * ir + fs_ay);
+        if (fabsf(ra) < (float)kUeqWatch) {
+            G(A.ueqx)[s_idx] = uea[0];
+            G(A.ueqy)[s_idx] = uea[1];
+        }
+    } else {
+        const double ux0 = G(A.ueqx_in)[s_idx], uy0 = G(A.ueqy_in)[s_idx];
+        uea[0] = (float)ux0;
+        uea[1] = (float)uy0;
+        G(A.ueqx)[s_idx] = ux0;
+        G(A.ueqy)[s_idx] = uy0;
+    }
+    if (!is_dry(rb)) {
+        const float ir = rcp(rb);
+        ueb[0] = ucx + B.inv_omega * (fb_x * ir + fs_bx);
+        ueb[1] = ucy + B.inv_omega * (fb_y * ir + fs_by);
+        if (fabsf(rb) < (float)kUeqWatch) {
+            G(B.ueqx)[s_idx] = ueb[0];
+            G(B.ueqy)[s_idx] = ueb[1];
+        }
+    } else {
+        const double ux0 = G(B.ueqx_in)[s_idx], uy0 = G(B.ueqy_in)[s_idx];
+        ueb[0] = (float)ux0;
+        ueb[1] = (float)uy0;
+        G(B.ueqx)[s_idx] = ux0;
+        G(B.ueqy)[s_idx] = uy0;
+    }
+    relax_dev(p[0], ra, da0, uea[0], uea[1], A.omega, A.force, A.has_force, oa);
+    relax_dev(p[1], rb, db0, ueb[0], ueb[1], B.omega, B.force, B.has_force, ob);
 }
 
 template <class ST, bool REPLAY, class OffT, bool GW>
@@ -403,72 +492,8 @@ __device__ inline void finish2(const StepArgs<typename ST::T>& a, void* const (&
     } else {
         // deviation forms, FP32 arithmetic
         if (!REPLAY && fl) {
-#pragma clang fp contract(fast)
-            const auto& A = a.c[0];
-            const auto& B = a.c[1];
-            float sd[2][2] = {{0.f, 0.f}, {0.f, 0.f}};   // sums over fluid nbrs of w c drho_other
-            float sfx = 0.f, sfy = 0.f;                  // sums over fluid nbrs of w c
-#pragma unroll
-            for (int i = 1; i < Q; ++i) {
-                const bool f = (nb >> i) & 1u;
-                const float da = f ? nr[0][i] : 0.f, db = f ? nr[1][i] : 0.f;
-                const float wx = (float)(wint(i) * cx(i)), wy = (float)(wint(i) * cy(i));
-                if (cx(i) != 0) {
-                    sd[0][0] += wx * db;
-                    sd[1][0] += wx * da;
-                    sfx += f ? wx : 0.f;
-                }
-                if (cy(i) != 0) {
-                    sd[0][1] += wy * db;
-                    sd[1][1] += wy * da;
-                    sfy += f ? wy : 0.f;
-                }
-            }
-            const float da0 = rc[0], db0 = rc[1];
-            const float ra = A.rho0 + da0, rb = B.rho0 + db0;
-            // psi_other over fluid neighbours = sum w c drho + rho0_other * sum w c
-            const float ga = A.gneg * ra, gb = B.gneg * rb;
-            const float fa_x = ga * (sd[0][0] + B.rho0 * sfx);
-            const float fa_y = ga * (sd[0][1] + B.rho0 * sfy);
-            const float fb_x = gb * (sd[1][0] + A.rho0 * sfx);
-            const float fb_y = gb * (sd[1][1] + A.rho0 * sfy);
-            // adhesion: sum over solid nbrs of w c = -(sum over fluid nbrs)
-            const float fs_ax = A.gsneg * -sfx, fs_ay = A.gsneg * -sfy;
-            const float fs_bx = B.gsneg * -sfx, fs_by = B.gsneg * -sfy;
-            float jax, jay, jbx, jby;
-            momentum(p[0], jax, jay);
-            momentum(p[1], jbx, jby);
-            const float inv_den = rcp(ra * A.omega + rb * B.omega);
-            const float ucx = (jax * A.omega + jbx * B.omega) * inv_den;
-            const float ucy = (jay * A.omega + jby * B.omega) * inv_den;
-            float uea[2], ueb[2];
-            if (!is_dry(ra)) {
-                const float ir = rcp(ra);
-                uea[0] = ucx + A.inv_omega * (fa_x * ir + fs_ax);
-                uea[1] = ucy + A.inv_omega * (fa_y * ir + fs_ay);
-                if (fabsf(ra) < (float)kUeqWatch) {
-                    G(A.ueqx)[s_idx] = uea[0];
-                    G(A.ueqy)[s_idx] = uea[1];
-                }
-            } else {
-                uea[0] = (float)G(A.ueqx)[s_idx];
-                uea[1] = (float)G(A.ueqy)[s_idx];
-            }
-            if (!is_dry(rb)) {
-                const float ir = rcp(rb);
-                ueb[0] = ucx + B.inv_omega * (fb_x * ir + fs_bx);
-                ueb[1] = ucy + B.inv_omega * (fb_y * ir + fs_by);
-                if (fabsf(rb) < (float)kUeqWatch) {
-                    G(B.ueqx)[s_idx] = ueb[0];
-                    G(B.ueqy)[s_idx] = ueb[1];
-                }
-            } else {
-                ueb[0] = (float)G(B.ueqx)[s_idx];
-                ueb[1] = (float)G(B.ueqy)[s_idx];
-            }
             float oa[Q], ob[Q];
-            relax_dev(p[0], ra, da0, uea[0], uea[1], A.omega, A.omc, A.force, A.has_force, oa);
-            relax_dev(p[1], rb, db0, ueb[0], ueb[1], B.omega, B.omc, B.force, B.has_force, ob);
+            collide2_dev(a, s_idx, p, rc, nr, nb, oa, ob);
 #pragma unroll
             for (int i = 0; i < Q; ++i) {
                 st_at<ST>(dst[0], (OffT)i * plane_b + home, oa[i]);

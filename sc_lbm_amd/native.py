@@ -26,6 +26,8 @@ EXPORTS = [
     "scl_fluid_repulsion", "scl_equilibrium_velocities", "scl_collide1", "scl_collide2",
     "scl_volume_force", "scl_stream1", "scl_stream2", "scl_step1", "scl_step2", "scl_moments",
     "scl_set_tile", "scl_set_lookahead", "scl_get_tile", "scl_autotune", "scl_timing", "scl_probe_copy",
+    "scl_set_step_fusion",
+    "scl_get_step_fusion",
     # serving layer (bound in serve.py)
     "scl_server_default_config", "scl_server_create", "scl_server_submit", "scl_server_status",
     "scl_server_wait", "scl_server_cancel", "scl_server_result", "scl_server_release", "scl_server_drain",
@@ -84,6 +86,8 @@ def load():
         "scl_moments": [_vp, _vp, _dp],
         "scl_set_tile": [_vp, ctypes.c_int, ctypes.c_int],
         "scl_set_lookahead": [_vp, ctypes.c_int],
+        "scl_set_step_fusion": [_vp, ctypes.c_int, ctypes.c_int],
+        "scl_get_step_fusion": [_vp, ctypes.c_int, _ip, _ip],
         "scl_get_tile": [_vp, ctypes.c_int, ctypes.c_int, _ip, _ip, _ip],
         "scl_autotune": [_vp, ctypes.c_int, ctypes.c_int, _ip, _ip, _ip],
         "scl_timing": [ctypes.c_int, _dp, _ip],
@@ -141,6 +145,19 @@ class Geometry:
 
     def set_lookahead(self, rows_ahead):
         check(load().scl_set_lookahead(self.h, int(rows_ahead)))
+
+    def set_step_fusion(self, steps_per_launch, rows=0):
+        """Time steps per launch (0 automatic, 1 or 2) and rows per band of the
+        two-step kernel (0 automatic)."""
+        check(load().scl_set_step_fusion(self.h, int(steps_per_launch), int(rows)))
+
+    def step_fusion(self, precision=FP32_DEV):
+        """(steps per launch, band rows) two-component steps of `precision` use."""
+        if isinstance(precision, str):
+            precision = PRECISION[precision]
+        n, r = ctypes.c_int(), ctypes.c_int()
+        check(load().scl_get_step_fusion(self.h, precision, ctypes.byref(n), ctypes.byref(r)))
+        return n.value, r.value
 
     def tile(self, ncomp=2, precision=FP64):
         """(cols, rows, lookahead) the fused kernel uses on this geometry."""

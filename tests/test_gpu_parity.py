@@ -289,3 +289,47 @@ def test_geometry_change_between_fused_steps():
     for c in range(2):
         assert np.all(np.isfinite(got[c]))
         assert np.array_equal(got[c].reshape(-1), s2.get_f(c))
+
+
+def _dry_case(nx=320, ny=96):
+    """Component B absent (density exactly 0) outside a rectangle: exercises
+    the zero-density rule (stale equilibrium velocity) of the deviation forms."""
+    m = wl.add_circle(wl.walls_x(nx, ny), 21, nx // 3, ny // 2)
+    ra = wl.noisy(np.full((ny, nx), 1.0), 3, 0.01)
+    rb = np.zeros((ny, nx))
+    rb[20:60, 150:260] = 0.8
+    return dict(mask=m, tau=[1.0, 0.8], g_fluid=[0.9, 0.9], g_solid=[0.1, -0.1],
+                f=[wl.f_from_rho(ra, m), wl.f_from_rho(rb, m)], force=[wl.body_force(2e-5), None])
+
+
+@pytest.mark.parametrize("prec", [nat.FP32_DEV, nat.FP16_DEV])
+@pytest.mark.parametrize("name", ["open", "porous", "dry"])
+def test_two_steps_per_launch_equal_single_steps(prec, name):
+    """The two-step kernel (step n stays on chip) gives the populations of
+    two single-step launches bit for bit, on lattices whose width is not a
+    multiple of the strip, with solids, adhesion, a force and dry sites."""
+    if name == "open":
+        case = wl.spinodal(300, G=1.2, seed=4)
+        case["force"] = [None, None]
+    elif name == "porous":
+        case = wl.porous(nx=700, ny=230, porosity=0.75, seed=6, rmin=5, rmax=12)
+    else:
+        case = _dry_case()
+    out = []
+    for fusion in (1, 2):
+        g = nat.Geometry(case["mask"])
+        g.set_step_fusion(fusion, 24)
+        fl = [nat.Field(g, case["tau"][c], prec, 1.0, g_solid=case["g_solid"][c], g_fluid=case["g_fluid"][c])
+              for c in range(2)]
+        for c in range(2):
+            fl[c].set_f(case["f"][c])
+        nat.solid_surface_force(g, fl[0], fl[1])
+        for n in (1, 6, 9, 2):   # bootstrap step, then even and odd chunks
+            nat.step2(g, fl[0], fl[1], case["force"][0], case["force"][1], n)
+        out.append([fl[c].get_f() for c in range(2)])
+        for f in fl:
+            f.close()
+        g.close()
+    for c in range(2):
+        assert np.all(np.isfinite(out[0][c]))
+        assert np.array_equal(out[0][c], out[1][c])

@@ -49,12 +49,14 @@ def dens_ref(f):
 
 def footprint(nx, ny, es, stage):
     """Device bytes of one resident job (the backend's block layout: mask,
-    2 x 2 population buffers, 4 equilibrium-velocity and 2 density planes
-    in FP64, optional FP64 staging; each region 256-byte aligned)."""
+    2 x 2 population buffers, equilibrium-velocity planes (4 in FP64, 3 x 4
+    rotating in the deviation forms) and 2 density planes in FP64, optional
+    FP64 staging; each region 256-byte aligned)."""
     a = lambda n: (n + 255) // 256 * 256
     pitch = (nx + 63) // 64 * 64
     plane = pitch * ny
-    total = a(ny * pitch // 64 * 8) + 4 * a(9 * plane * es) + 6 * a(plane * 8)
+    nueq = 1 if es == 8 else 3
+    total = a(ny * pitch // 64 * 8) + 4 * a(9 * plane * es) + (4 * nueq + 2) * a(plane * 8)
     return total + (a(9 * plane * 8) if stage else 0)
 
 
