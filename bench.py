@@ -40,6 +40,7 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU baseline sample")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--tile", default="", help="cols,rows[,lookahead] override (default: autotuned)")
+    p.add_argument("--fusion", default="", help="steps per launch[,band rows] override (default: autotuned)")
     p.add_argument("--no-serve", action="store_true", help="skip the served job stream")
     p.add_argument("--serve-jobs", type=int, default=2000)
     p.add_argument("--serve-step-scale", type=float, default=0.025,
@@ -54,7 +55,7 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def time_form(nat, wl, case, form, steps, warmup, tile):
+def time_form(nat, wl, case, form, steps, warmup, tile, fusion=""):
     g = nat.Geometry(case["mask"])
     prec = nat.PRECISION[form]
     if tile:
@@ -64,6 +65,9 @@ def time_form(nat, wl, case, form, steps, warmup, tile):
             g.set_lookahead(v[2])
     else:
         g.autotune(2, prec)          # launch shape chosen by measurement, before the timed region
+    if fusion:
+        v = [int(t) for t in fusion.split(",")]
+        g.set_step_fusion(v[0], v[1] if len(v) > 1 else 0)
     fl = []
     for c in range(2):
         f = nat.Field(g, case["tau"][c], prec, 1.0, g_solid=0.0, g_fluid=case["g_fluid"][c])
@@ -225,7 +229,7 @@ def main():
     res = {}
     for form in a.forms.split(","):
         log("timing %s: %d warmup + %d steps" % (form, a.warmup, a.steps))
-        res[form] = time_form(nat, wl, case, form, a.steps, a.warmup, a.tile)
+        res[form] = time_form(nat, wl, case, form, a.steps, a.warmup, a.tile, a.fusion)
         log("  %s: %.1f MLUPS, %s %.4f ms per launch (%d steps), %.0f GB/s algorithmic"
             % (form, res[form]["mlups"], res[form]["kernel"], res[form]["kernel_avg_ms"],
                res[form]["steps_per_launch"], res[form]["achieved_gbs"] or 0.0))

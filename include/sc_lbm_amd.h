@@ -171,7 +171,7 @@ int scl_set_lookahead(scl_geometry* g, int rows_ahead);
  * bit for bit those of single steps).  0 (the default): two per launch unless
  * scl_autotune measured single steps faster; 2: two per launch wherever
  * supported; 1: one per launch.  rows: rows per band of the two-step kernel
- * (0 = autotuned, else 16 for FP32 and 32 for FP16 storage). */
+ * (0 = autotuned, else 32). */
 int scl_set_step_fusion(scl_geometry* g, int steps_per_launch, int rows);
 /* What two-component steps of `precision` on g use: steps per launch (1 or
  * 2) and the band rows of the two-step kernel (0 when one per launch). */

@@ -407,10 +407,8 @@ bool x2_wanted(const scl_geometry* g, int precision) {
 }
 
 int x2_rows_for(const scl_geometry* g, int precision) {
-    // short bands give many workgroups (two per CU fit in LDS for FP32
-    // storage, three for FP16); measured best at 4096^2: 16 (FP32), 32 (FP16)
-    const int r = g->x2_rows ? g->x2_rows : (g->x2_tuned[precision] > 0 ? g->x2_tuned[precision]
-                                                                       : (precision == SCL_FP32_DEV ? 16 : 32));
+    // measured best at 4096^2: 32 rows (FP32 and FP16 storage)
+    const int r = g->x2_rows ? g->x2_rows : (g->x2_tuned[precision] > 0 ? g->x2_tuned[precision] : 32);
     return std::min(r, g->ny);
 }
 
@@ -422,7 +420,7 @@ template <class ST>
 int launch_x2(const scl_geometry* g, scl_field* const* fl, const double* const* forces) {
     const int rows = x2_rows_for(g, fl[0]->precision);
     StepArgs<float> a1 = make_args<float>(g, rows, 4);
-    a1.gx = (g->nx + kX2Out - 1) / kX2Out;
+    a1.gx = (g->nx + x2_out<ST>() - 1) / x2_out<ST>();
     for (int c = 0; c < 2; ++c) fill_comp(a1.c[c], fl[c], fl[c]->cur, forces[c]);
     StepArgs<float> a2 = a1;
     for (int c = 0; c < 2; ++c) {
@@ -1262,7 +1260,7 @@ int scl_autotune(scl_geometry* g, int ncomp, int precision, int* cols, int* rows
         g->fusion = 2;
         double best2 = 1e300;
         int b2 = 0;
-        const int bandset[5] = {8, 16, 24, 32, 64};
+        const int bandset[5] = {8, 16, 32, 48, 64};
         for (int bi = 0; bi < 5 && rc == SCL_OK; ++bi) {
             if (bandset[bi] > g->ny && bi > 0) continue;
             g->x2_rows = bandset[bi];

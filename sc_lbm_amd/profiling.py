@@ -1,9 +1,10 @@
 """Lean drivers for profiler runs (rocprofv3 puts the program after `--`).
 
     python -m sc_lbm_amd.profiling step --form fp32dev --n 4096 --steps 20 [--porosity 0.8]
+            [--tile 512,128,1] [--fusion 2 --band 16]
         builds the 4096^2 two-component spinodal case (optionally with seeded
-        disk obstacles), runs `steps` fused steps of one storage form and
-        prints the mean kernel time;
+        disk obstacles), runs `steps` time steps of one storage form (one or
+        two per launch) and prints the mean kernel time per launch;
     python -m sc_lbm_amd.profiling copy --bytes 2147483648 --reps 10
         runs the plain copy kernel k_probe_copy (reads and writes `bytes`
         each launch) so the HBM byte counters can be calibrated.
@@ -29,6 +30,8 @@ def main(argv=None):
     s.add_argument("--steps", type=int, default=20)
     s.add_argument("--warmup", type=int, default=2)
     s.add_argument("--tile", default="")
+    s.add_argument("--fusion", type=int, default=0, help="time steps per launch: 0 automatic, 1 or 2")
+    s.add_argument("--band", type=int, default=0, help="rows per band of the two-step kernel (0 automatic)")
     s.add_argument("--porosity", type=float, default=1.0,
                    help="below 1: seeded disk obstacles (radii 8..20) to this fluid fraction")
     c = sub.add_parser("copy")
@@ -59,6 +62,8 @@ def main(argv=None):
             g.set_lookahead(v[2])
     else:
         g.autotune(2, nat.PRECISION[a.form])
+    if a.fusion or a.band:
+        g.set_step_fusion(a.fusion, a.band)
     fl = [nat.Field(g, case["tau"][k], nat.PRECISION[a.form], 1.0, g_fluid=case["g_fluid"][k])
           for k in range(2)]
     for k in range(2):
@@ -68,8 +73,11 @@ def main(argv=None):
     nat.synchronize()
     nat.timing(1)
     nat.step2(g, fl[0], fl[1], None, None, a.steps)
-    avg, cnt = nat.timing(0)
-    print(json.dumps({"form": a.form, "n": a.n, "porosity": a.porosity, "steps": a.steps, "kernel_avg_ms": avg, "launches": cnt,
+    avg, cnt = nat.timing(0)            # per time step
+    spl, band = g.step_fusion(nat.PRECISION[a.form])
+    print(json.dumps({"form": a.form, "n": a.n, "porosity": a.porosity, "steps": a.steps,
+                      "kernel": "k_step2x" if spl == 2 else "k_step", "steps_per_launch": spl, "band_rows": band,
+                      "kernel_avg_ms": avg * a.steps / cnt if cnt else None, "launches": cnt,
                       "tile": list(g.tile(2, nat.PRECISION[a.form])),
                       "mlups": a.n * a.n / (avg * 1e-3) / 1e6 if avg else None}))
 
