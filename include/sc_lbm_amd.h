@@ -171,7 +171,8 @@ int scl_set_lookahead(scl_geometry* g, int rows_ahead);
  * bit for bit those of single steps).  0 (the default): two per launch unless
  * scl_autotune measured single steps faster; 2: two per launch wherever
  * supported; 1: one per launch.  rows: rows per band of the two-step kernel
- * (0 = autotuned, else 32). */
+ * (0 = autotuned, else the height that makes the grid two whole waves of
+ * resident workgroups). */
 int scl_set_step_fusion(scl_geometry* g, int steps_per_launch, int rows);
 /* What two-component steps of `precision` on g use: steps per launch (1 or
  * 2) and the band rows of the two-step kernel (0 when one per launch). */
@@ -185,7 +186,8 @@ int scl_get_tile(const scl_geometry* g, int ncomp, int precision, int* cols, int
  * 64..512, rows 4..128, lookahead 1..2) on scratch fields at rest, keeps the
  * fastest for later steps on g and returns it (outputs may be NULL); for two
  * components in a deviation form it then times the two-step kernel over band
- * heights 8..64 and keeps two steps per launch only if faster.  Takes about
+ * heights (16, 32, and those filling 1..4 whole waves of workgroups) and
+ * keeps two steps per launch only if faster.  Takes about
  * 3 ms of GPU time per shape; needs memory for two scratch fields. */
 int scl_autotune(scl_geometry* g, int ncomp, int precision, int* cols, int* rows, int* lookahead);
 

@@ -70,6 +70,8 @@ struct Lib {
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
     std::vector<int> ev_steps;   // time steps covered by each recorded launch
+    int cus = 0;                 // compute units of the device
+    int x2_occ[3][2] = {};       // [precision][solid-free]: two-step workgroups per CU (0 = not queried)
 };
 Lib g_lib;
 const size_t kMaxTimed = 1 << 15;
@@ -84,6 +86,7 @@ int lib_ready() {
     HIPCHK(hipGetDeviceProperties(&prop, g_lib.device));
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
         return fail(SCL_ERR_HIP, "device %d is %s; this library is built for gfx950", g_lib.device, prop.gcnArchName);
+    g_lib.cus = prop.multiProcessorCount;
     // default priority (no priority argument): the library never changes it
     HIPCHK(hipStreamCreateWithFlags(&g_lib.stream, hipStreamNonBlocking));
     g_lib.ready = true;
@@ -406,9 +409,36 @@ bool x2_wanted(const scl_geometry* g, int precision) {
     return g->fusion == 2 || g->x2_tuned[precision] >= 0;
 }
 
+// Workgroups of the two-step kernel resident per CU (registers and LDS).
+int x2_occupancy(int precision, bool open) {
+    int& o = g_lib.x2_occ[precision][open ? 1 : 0];
+    if (!o) {
+        const void* k = precision == SCL_FP32_DEV
+                            ? (open ? reinterpret_cast<const void*>(&k_step2x<StoreF32, true>)
+                                    : reinterpret_cast<const void*>(&k_step2x<StoreF32, false>))
+                            : (open ? reinterpret_cast<const void*>(&k_step2x<StoreF16, true>)
+                                    : reinterpret_cast<const void*>(&k_step2x<StoreF16, false>));
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, kX2Threads, 0) != hipSuccess || n < 1) n = 2;
+        o = n;
+    }
+    return o;
+}
+
+// Band height that makes the grid `waves` whole waves of resident workgroups
+// (a partial last wave leaves CUs idle at the end of every launch).
+int x2_wave_rows(const scl_geometry* g, int precision, int waves) {
+    const int out = precision == SCL_FP32_DEV ? x2_out<StoreF32>() : x2_out<StoreF16>();
+    const int strips = (g->nx + out - 1) / out;
+    const int slots = x2_occupancy(precision, g->open) * std::max(g_lib.cus, 1);
+    const int bands = std::max(1, waves * slots / strips);
+    return std::max(8, std::min((g->ny + bands - 1) / bands, g->ny));
+}
+
 int x2_rows_for(const scl_geometry* g, int precision) {
-    // measured best at 4096^2: 32 rows (FP32 and FP16 storage)
-    const int r = g->x2_rows ? g->x2_rows : (g->x2_tuned[precision] > 0 ? g->x2_tuned[precision] : 32);
+    // default: two whole waves (52 rows at 4096^2 with FP32 storage, 46 with FP16)
+    const int r = g->x2_rows ? g->x2_rows
+                             : (g->x2_tuned[precision] > 0 ? g->x2_tuned[precision] : x2_wave_rows(g, precision, 2));
     return std::min(r, g->ny);
 }
 
@@ -1260,9 +1290,13 @@ int scl_autotune(scl_geometry* g, int ncomp, int precision, int* cols, int* rows
         g->fusion = 2;
         double best2 = 1e300;
         int b2 = 0;
-        const int bandset[5] = {8, 16, 32, 48, 64};
-        for (int bi = 0; bi < 5 && rc == SCL_OK; ++bi) {
-            if (bandset[bi] > g->ny && bi > 0) continue;
+        // whole waves of workgroups (1..4), and two fixed heights
+        int bandset[6] = {16, 32, x2_wave_rows(g, precision, 1), x2_wave_rows(g, precision, 2),
+                          x2_wave_rows(g, precision, 3), x2_wave_rows(g, precision, 4)};
+        for (int bi = 0; bi < 6 && rc == SCL_OK; ++bi) {
+            bool seen = bandset[bi] > g->ny && bi > 0;
+            for (int bj = 0; bj < bi; ++bj) seen = seen || bandset[bj] == bandset[bi];
+            if (seen) continue;
             g->x2_rows = bandset[bi];
             const double* none[2] = {nullptr, nullptr};
             rc = fused_steps<2>(g, fl, none, 2);
