@@ -170,34 +170,50 @@ __device__ inline void feq_ref(double rho, double ux, double uy, double (&e)[Q])
 //    rather than adding (1 - omega) d_i and omega deq_i, which nearly cancel
 //    for omega near 2 (a porous two-phase lattice at tau = 0.707 drifted 4x
 //    further from FP64 with the latter form);
-//  * the rest population closes the site's mass balance, d_0' = drho -
-//    sum_{i>0} d_i', before the force is added.  Collision conserves mass;
-//    evaluating d_0' from its own formula added (rho - rho0) times the
-//    rounding of the weights to the mass of every site at every step, always
-//    with the same sign, which had grown to 1e-4 in density after 1000 steps
-//    of a 1.0 / 0.06 droplet at tau = 0.8.
+//  * the rest population closes the site's mass balance:
+//    d_0' = d_0 + sum_{i>0} (d_i - d_i') + sum_i F_i, summing the changes,
+//    which are small, rather than the populations.  Collision conserves
+//    mass; evaluating d_0' from its own formula added (rho - rho0) times the
+//    rounding of the weights to every site's mass at every step, always with
+//    the same sign (1e-4 in density after 1000 steps of a 1.0 / 0.06 droplet
+//    at tau = 0.8), and d_0' = drho - sum_{i>0} d_i' still drifted: in a
+//    steady state the rounding of those large partial sums repeats every
+//    step (a porous two-phase run lost 2e-5 of a component's mass in 1e4
+//    steps; summing the changes cut an FP32 emulation's drift 1000x).
 __device__ inline void relax_dev(const float (&p)[Q], float rho, float drho, float ux, float uy, float omega,
                                  const float* force, bool has_force, float (&out)[Q]) {
-#pragma clang fp contract(fast)
-    const float usq15 = 1.5f * (ux * ux + uy * uy);
+    // Every fused multiply-add is written out (the file is built without
+    // contraction): the compiler's own choice of which products to fuse
+    // could differ between the kernels this is inlined into, and the
+    // single-step and two-step kernels must give the same bits.
+    const float usq15 = 1.5f * fmaf(ux, ux, uy * uy);
     const float wd1 = (1.0f / 9.0f) * drho, wr1 = (1.0f / 9.0f) * rho;
     const float wd2 = (1.0f / 36.0f) * drho, wr2 = (1.0f / 36.0f) * rho;
+    float r = 0.f;   // sum over i > 0 of (d_i - d_i')
     auto pair = [&](int i, int j, float cu, float wd, float wr) {
-        const float t = (4.5f * cu) * cu - usq15;
+        const float t = fmaf(4.5f * cu, cu, -usq15);
         const float c3 = 3.0f * cu;
-        const float ei = wd + wr * (t + c3), ej = wd + wr * (t - c3);
-        out[i] = p[i] + omega * (ei - p[i]);
-        out[j] = p[j] + omega * (ej - p[j]);
+        const float ei = fmaf(wr, t + c3, wd), ej = fmaf(wr, t - c3, wd);
+        float oi = fmaf(omega, ei - p[i], p[i]);
+        float oj = fmaf(omega, ej - p[j], p[j]);
+        if (has_force) {
+            oi += force[i];
+            oj += force[j];
+        }
+        out[i] = oi;
+        out[j] = oj;
+        r += (p[i] - oi) + (p[j] - oj);
     };
     pair(1, 3, ux, wd1, wr1);
     pair(2, 4, uy, wd1, wr1);
     pair(5, 7, ux + uy, wd2, wr2);
     pair(6, 8, uy - ux, wd2, wr2);
-    out[0] = drho - (((out[1] + out[3]) + (out[2] + out[4])) + ((out[5] + out[7]) + (out[6] + out[8])));
+    float fs = 0.f;   // mass the force adds
     if (has_force) {
 #pragma unroll
-        for (int i = 0; i < Q; ++i) out[i] += force[i];
+        for (int i = 0; i < Q; ++i) fs += force[i];
     }
+    out[0] = (p[0] + r) + fs;
 }
 
 // Reciprocal for the deviation forms (hardware approximation, 1 ulp).
@@ -305,49 +321,58 @@ __device__ inline void finish1(const CompArgs<typename ST::T>& ca, void* dst, lo
 template <class T>
 __device__ inline void collide2_dev(const StepArgs<T>& a, int s_idx, const float (&p)[2][Q], const float (&rc)[2],
                                     const float (&nr)[2][Q], unsigned nb, float (&oa)[Q], float (&ob)[Q]) {
-#pragma clang fp contract(fast)
+    // fused multiply-adds written out, as in relax_dev
     const auto& A = a.c[0];
     const auto& B = a.c[1];
-    float sd[2][2] = {{0.f, 0.f}, {0.f, 0.f}};   // sums over fluid nbrs of w c drho_other
-    float sfx = 0.f, sfy = 0.f;                  // sums over fluid nbrs of w c
+    // Neighbour sums sum_i w_i c_i v(x + c_i) as differences of opposite
+    // neighbours first:
+    //   x: (v1 - v3)/9 + ((v5 - v7) + (v8 - v6))/36,
+    //   y: (v2 - v4)/9 + ((v5 - v7) - (v8 - v6))/36.
+    // Summed term by term, the rounding is relative to terms much larger than
+    // the result, and in a steady state it repeats every step and adds up to
+    // a net force (a static droplet drifted 1.4e-2 lattice units in 2e4
+    // steps); the differences of neighbouring values are small and nearly
+    // exact.  Solid neighbours contribute zero.
+    float v[2][Q], fl[Q];
 #pragma unroll
     for (int i = 1; i < Q; ++i) {
         const bool f = (nb >> i) & 1u;
-        const float da = f ? nr[0][i] : 0.f, db = f ? nr[1][i] : 0.f;
-        const float wx = (float)(wint(i) * cx(i)), wy = (float)(wint(i) * cy(i));
-        if (cx(i) != 0) {
-            sd[0][0] += wx * db;
-            sd[1][0] += wx * da;
-            sfx += f ? wx : 0.f;
-        }
-        if (cy(i) != 0) {
-            sd[0][1] += wy * db;
-            sd[1][1] += wy * da;
-            sfy += f ? wy : 0.f;
-        }
+        v[0][i] = f ? nr[0][i] : 0.f;
+        v[1][i] = f ? nr[1][i] : 0.f;
+        fl[i] = f ? 1.f : 0.f;
     }
+    auto nsum = [](const float (&a)[Q], float& sx, float& sy) {
+        const float d57 = a[5] - a[7], d86 = a[8] - a[6];
+        sx = fmaf(1.0f / 9.0f, a[1] - a[3], (1.0f / 36.0f) * (d57 + d86));
+        sy = fmaf(1.0f / 9.0f, a[2] - a[4], (1.0f / 36.0f) * (d57 - d86));
+    };
+    float sd[2][2];   // [c]: sums over fluid nbrs of w c drho of the other component
+    float sfx, sfy;   // sums over fluid nbrs of w c
+    nsum(v[1], sd[0][0], sd[0][1]);
+    nsum(v[0], sd[1][0], sd[1][1]);
+    nsum(fl, sfx, sfy);
     const float da0 = rc[0], db0 = rc[1];
     const float ra = A.rho0 + da0, rb = B.rho0 + db0;
     // psi_other over fluid neighbours = sum w c drho + rho0_other * sum w c
     const float ga = A.gneg * ra, gb = B.gneg * rb;
-    const float fa_x = ga * (sd[0][0] + B.rho0 * sfx);
-    const float fa_y = ga * (sd[0][1] + B.rho0 * sfy);
-    const float fb_x = gb * (sd[1][0] + A.rho0 * sfx);
-    const float fb_y = gb * (sd[1][1] + A.rho0 * sfy);
+    const float fa_x = ga * fmaf(B.rho0, sfx, sd[0][0]);
+    const float fa_y = ga * fmaf(B.rho0, sfy, sd[0][1]);
+    const float fb_x = gb * fmaf(A.rho0, sfx, sd[1][0]);
+    const float fb_y = gb * fmaf(A.rho0, sfy, sd[1][1]);
     // adhesion: sum over solid nbrs of w c = -(sum over fluid nbrs)
     const float fs_ax = A.gsneg * -sfx, fs_ay = A.gsneg * -sfy;
     const float fs_bx = B.gsneg * -sfx, fs_by = B.gsneg * -sfy;
     float jax, jay, jbx, jby;
     momentum(p[0], jax, jay);
     momentum(p[1], jbx, jby);
-    const float inv_den = rcp(ra * A.omega + rb * B.omega);
-    const float ucx = (jax * A.omega + jbx * B.omega) * inv_den;
-    const float ucy = (jay * A.omega + jby * B.omega) * inv_den;
+    const float inv_den = rcp(fmaf(ra, A.omega, rb * B.omega));
+    const float ucx = fmaf(jax, A.omega, jbx * B.omega) * inv_den;
+    const float ucy = fmaf(jay, A.omega, jby * B.omega) * inv_den;
     float uea[2], ueb[2];
     if (!is_dry(ra)) {
         const float ir = rcp(ra);
-        uea[0] = ucx + A.inv_omega * (fa_x * ir + fs_ax);
-        uea[1] = ucy + A.inv_omega * (fa_y * ir + fs_ay);
+        uea[0] = fmaf(A.inv_omega, fmaf(fa_x, ir, fs_ax), ucx);
+        uea[1] = fmaf(A.inv_omega, fmaf(fa_y, ir, fs_ay), ucy);
         if (fabsf(ra) < (float)kUeqWatch) {
             G(A.ueqx)[s_idx] = uea[0];
             G(A.ueqy)[s_idx] = uea[1];
@@ -361,8 +386,8 @@ __device__ inline void collide2_dev(const StepArgs<T>& a, int s_idx, const float
     }
     if (!is_dry(rb)) {
         const float ir = rcp(rb);
-        ueb[0] = ucx + B.inv_omega * (fb_x * ir + fs_bx);
-        ueb[1] = ucy + B.inv_omega * (fb_y * ir + fs_by);
+        ueb[0] = fmaf(B.inv_omega, fmaf(fb_x, ir, fs_bx), ucx);
+        ueb[1] = fmaf(B.inv_omega, fmaf(fb_y, ir, fs_by), ucy);
         if (fabsf(rb) < (float)kUeqWatch) {
             G(B.ueqx)[s_idx] = ueb[0];
             G(B.ueqy)[s_idx] = ueb[1];
@@ -517,14 +542,15 @@ __device__ inline void finish2(const StepArgs<typename ST::T>& a, void* const (&
 // no solid site (the mask is not read and every neighbourhood is fluid, so
 // the bounce-back selects and the adhesion sums fold away at compile time).
 // ---------------------------------------------------------------------------
-// Occupancy request: the solid-free FP32-arithmetic kernels fit in 128
-// registers, i.e. four waves per SIMD.
+// Occupancy request: the solid-free FP32-arithmetic kernels fit in 168
+// registers, i.e. three waves per SIMD (four measured no faster, and with the
+// FP64 force sums they would spill).
 template <class ST, bool OPEN, int LA>
 #ifndef SCL_GATHER_WAVES
 #define SCL_GATHER_WAVES 1
 #endif
 constexpr int step_waves_per_eu() {
-    return (OPEN && LA == 1 && sizeof(typename ST::T) == 4) ? 4
+    return (OPEN && LA == 1 && sizeof(typename ST::T) == 4) ? 3
                                                             : ((!OPEN && sizeof(typename ST::T) == 4) ? SCL_GATHER_WAVES : 1);
 }
 
