@@ -278,6 +278,9 @@ typedef struct scl_server_config {
     double mass_tol;            /* watchdog: |mass - mass0| > mass_tol*|mass0| fails a job;
                                    0 = by precision (1e-6 FP64, 1e-3 FP32, 0.1 FP16) */
     int max_bypass;             /* later jobs admitted past a waiting one before it blocks admission */
+    int steps_per_launch;       /* 0 = automatic: with FP32 storage, jobs at least 256 columns wide
+                                   advance two steps per launch (same bits as one); 1 = one */
+    int band_rows;              /* rows per band of the two-step tiles; 0 = 32            */
 } scl_server_config;
 
 typedef struct scl_server_stats {
@@ -293,6 +296,7 @@ typedef struct scl_server_stats {
     int resident, queued, max_resident;
     size_t arena_bytes, arena_used, arena_peak;
     int tile_cols, tile_rows, lookahead, round_steps, workgroups;
+    int steps_per_launch;       /* 2 when wide jobs take two steps per launch          */
 } scl_server_stats;
 
 /* Defaults: FP32 deviations, 8 GiB arena, 64 slots, 4096 pending, rounds of

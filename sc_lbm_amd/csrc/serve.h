@@ -15,10 +15,19 @@
 // increments a second counter, and the workgroup that completes it zeroes
 // both, so consecutive launches (and replays of a captured graph) start from
 // zero without a memset.
+//
+// Two steps per launch (FP32 deviations, jobs at least 256 columns wide with
+// two or more steps in the round): k_serve2x takes tiles of a second list
+// (strips of x2_out columns) and computes each with step2x_tile, the body of
+// the single-lattice two-step kernel; launch q of a round advances steps 2q
+// and 2q+1 of such a job.  An odd last step goes through k_serve.  A pair
+// reads one buffer and writes the other, so after q pairs the job's
+// populations are in buffer cur ^ (q & 1).
 #pragma once
 #include "../host/serve_ic.h"
 #include "convert.h"
 #include "step.h"
+#include "step2x.h"
 
 namespace scl {
 
@@ -31,6 +40,8 @@ struct JobArgs {
     StepArgs<T> a;
     int cur;
     int steps;                  // steps of this job in the current round
+    int x2;                     // 1: steps 2q, 2q+1 go through k_serve2x (q < steps / 2)
+    int fin;                    // buffer holding the populations at the end of the round
     int item_begin, item_end;   // its tiles in the work-list
     // deviation forms: rotating equilibrium-velocity buffers [k][component]
     // [x/y]; step s reads buffer (ucur + s) % 3 and writes the next
@@ -40,11 +51,14 @@ struct JobArgs {
 
 struct ServeHdr {
     const void* jobs;           // JobArgs<T>[n_jobs]
-    const uint64_t* items;      // work-list
+    const uint64_t* items;      // work-list: every job's tiles (the mass reduction's)
     double* partial;            // 2 per tile (mass per component)
     double* mass;               // 2 per job
     int n_items, n_jobs;
     int bw;                     // tile width in columns
+    int n_step;                 // k_serve takes items [0, n_step): the jobs it has steps of
+    const uint64_t* items2;     // two-step tiles (strips of x2_out columns)
+    int n_items2;
 };
 
 // Tile encoding, 16 bits each from the low end: job, strip, y0, y1.
@@ -66,7 +80,7 @@ k_serve(const ServeHdr* __restrict__ hdr, int s, unsigned* __restrict__ ctr) {
     using T = typename ST::T;
     __shared__ DensRing<2, T, NW> lds;
     __shared__ int ticket;
-    const int n = hdr->n_items;
+    const int n = hdr->n_step;
     const SCL_AS_CONST JobArgs<T>* jobs = K((const JobArgs<T>*)hdr->jobs);
     const SCL_AS_CONST uint64_t* items = K(hdr->items);
     for (;;) {
@@ -84,6 +98,7 @@ k_serve(const ServeHdr* __restrict__ hdr, int s, unsigned* __restrict__ ctr) {
         y1 = __builtin_amdgcn_readfirstlane(y1);
         const SCL_AS_CONST JobArgs<T>& J = jobs[j];
         if (s >= J.steps) continue;
+        if (J.x2 && s != J.steps - 1) continue;   // the pairs are k_serve2x's
         StepArgs<T> a = J.a;
         if constexpr (ST::kDev) {
             const int ki = (J.ucur + s) % 3, ko = (J.ucur + s + 1) % 3;
@@ -96,7 +111,9 @@ k_serve(const ServeHdr* __restrict__ hdr, int s, unsigned* __restrict__ ctr) {
             }
         }
         Bufs b;
-        const int sw = (J.cur ^ s) & 1;
+        // buffer read at step s: one swap per launch (after s / 2 pairs for a
+        // two-step job)
+        const int sw = (J.cur ^ (J.x2 ? (s >> 1) : s)) & 1;
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
             b.src[c] = sw ? a.c[c].dst : a.c[c].src;
@@ -106,6 +123,60 @@ k_serve(const ServeHdr* __restrict__ hdr, int s, unsigned* __restrict__ ctr) {
     }
     if (threadIdx.x == 0) {
         // every fetch of this workgroup precedes this release
+        const unsigned d = __hip_atomic_fetch_add(ctr + 1, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (d == gridDim.x - 1) {
+            __hip_atomic_store(ctr, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(ctr + 1, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+// Launch q of a round: steps 2q and 2q+1 of every two-step job that has them,
+// one tile of the second list per ticket (FP32 storage; the general variant,
+// since the jobs carry masks).
+template <class ST>
+__global__ __launch_bounds__(kX2Threads) void k_serve2x(const ServeHdr* __restrict__ hdr, int q,
+                                                        unsigned* __restrict__ ctr) {
+    __shared__ X2Lds<ST, false> lds;
+    __shared__ int ticket;
+    const int n = hdr->n_items2;
+    const SCL_AS_CONST JobArgs<float>* jobs = K((const JobArgs<float>*)hdr->jobs);
+    const SCL_AS_CONST uint64_t* items = K(hdr->items2);
+    for (;;) {
+        __syncthreads();   // the previous tile is done with `ticket` and the LDS
+        if (threadIdx.x == 0)
+            ticket = (int)__hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        const int t = __builtin_amdgcn_readfirstlane(ticket);
+        if (t >= n) break;
+        int j, bx, y0, y1;
+        unpack_item(items[t], j, bx, y0, y1);
+        j = __builtin_amdgcn_readfirstlane(j);
+        bx = __builtin_amdgcn_readfirstlane(bx);
+        y0 = __builtin_amdgcn_readfirstlane(y0);
+        y1 = __builtin_amdgcn_readfirstlane(y1);
+        const SCL_AS_CONST JobArgs<float>& J = jobs[j];
+        if (2 * q + 1 >= J.steps) continue;
+        StepArgs<float> a1 = J.a, a2 = J.a;
+        const int s = 2 * q, rb = (J.cur ^ q) & 1;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const void* src = rb ? J.a.c[c].dst : J.a.c[c].src;
+            void* dst = rb ? const_cast<void*>(J.a.c[c].src) : J.a.c[c].dst;
+            a1.c[c].src = src;
+            a2.c[c].dst = dst;
+            a1.c[c].ueqx_in = J.ueq[(J.ucur + s) % 3][c][0];
+            a1.c[c].ueqy_in = J.ueq[(J.ucur + s) % 3][c][1];
+            a1.c[c].ueqx = J.ueq[(J.ucur + s + 1) % 3][c][0];
+            a1.c[c].ueqy = J.ueq[(J.ucur + s + 1) % 3][c][1];
+            a2.c[c].ueqx_in = a1.c[c].ueqx;
+            a2.c[c].ueqy_in = a1.c[c].ueqy;
+            a2.c[c].ueqx = J.ueq[(J.ucur + s + 2) % 3][c][0];
+            a2.c[c].ueqy = J.ueq[(J.ucur + s + 2) % 3][c][1];
+        }
+        step2x_tile<ST, false>(a1, a2, bx, y0, y1, lds);
+    }
+    if (threadIdx.x == 0) {
         const unsigned d = __hip_atomic_fetch_add(ctr + 1, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
         if (d == gridDim.x - 1) {
             __hip_atomic_store(ctr, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -145,7 +216,7 @@ __global__ __launch_bounds__(256) void k_serve_mass(const ServeHdr* __restrict__
         int j, bx, y0, y1;
         unpack_item(hdr->items[t], j, bx, y0, y1);
         const JobArgs<T>& J = jobs[j];
-        const int fin = J.cur ^ (J.steps & 1);
+        const int fin = J.fin;
         const typename ST::S* P[2];
 #pragma unroll
         for (int c = 0; c < 2; ++c)

@@ -26,6 +26,7 @@
 #include <map>
 #include <memory>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../host/serve_core.h"
@@ -117,9 +118,9 @@ class GpuServe final : public serve::Backend {
         if (stream_) (void)hipStreamSynchronize(stream_);
         for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
         for (void* p : {(void*)arena_, (void*)hdr_d_, (void*)ctr_d_, table_d_, (void*)items_d_, (void*)partial_d_,
-                        (void*)mass_d_})
+                        (void*)mass_d_, (void*)items2_d_})
             if (p) (void)hipFree(p);
-        for (void* p : {(void*)hdr_h_, table_h_, (void*)items_h_, (void*)mass_h_})
+        for (void* p : {(void*)hdr_h_, table_h_, (void*)items_h_, (void*)mass_h_, (void*)items2_h_})
             if (p) (void)hipHostFree(p);
         for (hipEvent_t e : ev_)
             if (e) (void)hipEventDestroy(e);
@@ -176,6 +177,20 @@ class GpuServe final : public serve::Backend {
         }
         rc = occupancy(nw_, la_, &grid_, err);
         if (rc != SCL_OK) return rc;
+        if (cfg_.steps_per_launch < 0 || cfg_.steps_per_launch > 2 || cfg_.band_rows < 0 || cfg_.band_rows > 4096) {
+            *err = "steps_per_launch is 0, 1 or 2 and band_rows 0..4096";
+            return SCL_ERR_ARG;
+        }
+        if constexpr (std::is_same<ST, StoreF32>::value) {
+            // two steps per launch: FP32 storage only (with FP16 storage the
+            // general two-step variant is slower than single steps)
+            x2_ = cfg_.steps_per_launch != 1;
+            x2rows_ = cfg_.band_rows ? cfg_.band_rows : 32;
+            int per_cu = 0;
+            SHIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per_cu, reinterpret_cast<const void*>(&k_serve2x<ST>), kX2Threads, 0));
+            grid2_ = std::max(1, per_cu) * cus_;
+        }
         SHIP(hipStreamSynchronize(stream_));
         return SCL_OK;
     }
@@ -199,33 +214,54 @@ class GpuServe final : public serve::Backend {
     int run_round(const std::vector<Job*>& jobs, const std::vector<int>& steps, int rsteps,
                   std::string* err) override {
         SHIP(hipSetDevice(dev_));
+        // two-step jobs: at least 256 columns and two steps in the round
+        std::vector<char> two(jobs.size(), 0);
+        for (size_t k = 0; k < jobs.size(); ++k)
+            two[k] = x2_ && rsteps >= 2 && steps[k] >= 2 && jobs[k]->spec.nx >= kX2Threads;
+        // work-list order: first the jobs k_serve has steps of (largest first),
+        // then the two-step jobs with an even step count (their tiles only
+        // serve the mass reduction)
         std::vector<int> order(jobs.size());
         for (size_t k = 0; k < order.size(); ++k) order[k] = (int)k;
-        std::stable_sort(order.begin(), order.end(),
-                         [&](int a, int b) { return jobs[a]->sites() > jobs[b]->sites(); });
-        size_t need = 0;
-        for (Job* j : jobs) need += tiles_of(j->spec.nx, j->spec.ny, nw_, rows_);
+        auto single = [&](int k) { return !two[k] || (steps[k] & 1); };
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+            if (single(a) != single(b)) return single(a);
+            return jobs[a]->sites() > jobs[b]->sites();
+        });
+        size_t need = 0, need2 = 0;
+        for (size_t k = 0; k < jobs.size(); ++k) {
+            need += tiles_of(jobs[k]->spec.nx, jobs[k]->spec.ny, nw_, rows_);
+            if (two[k]) need2 += tiles2_of(jobs[k]->spec.nx, jobs[k]->spec.ny, x2rows_);
+        }
         int rc = grow_items(need, err);
+        if (rc == SCL_OK) rc = grow_items2(need2, err);
         if (rc != SCL_OK) return rc;
         JobArgs<T>* table = (JobArgs<T>*)table_h_;
-        int n = 0;
+        int n = 0, n_step = 0, n2 = 0;
         for (int k : order) {
             const GJob* g = (const GJob*)jobs[k]->dev;
             JobArgs<T>& ja = table[k];
             ja.a = args_of(jobs[k]->spec, *g);
             ja.cur = g->cur;
             ja.steps = steps[k];
+            ja.x2 = two[k];
+            ja.fin = g->cur ^ (launches(steps[k], two[k]) & 1);
             fill_ueq(ja, *g);
             ja.item_begin = n;
             n = emit_tiles(k, g->nx, g->ny, nw_, rows_, items_h_, n);
             ja.item_end = n;
+            if (single(k)) n_step = n;
+            if (two[k]) n2 = emit_tiles2(k, g->nx, g->ny, x2rows_, items2_h_, n2);
         }
-        *hdr_h_ = ServeHdr{table_d_, items_d_, partial_d_, mass_d_, n, (int)jobs.size(), 64 * nw_};
+        *hdr_h_ = ServeHdr{table_d_, items_d_, partial_d_, mass_d_, n, (int)jobs.size(), 64 * nw_, n_step,
+                           items2_d_, n2};
         SHIP(hipMemcpyAsync(table_d_, table_h_, jobs.size() * sizeof(JobArgs<T>), hipMemcpyHostToDevice, stream_));
         SHIP(hipMemcpyAsync(items_d_, items_h_, (size_t)n * sizeof(uint64_t), hipMemcpyHostToDevice, stream_));
+        if (n2) SHIP(hipMemcpyAsync(items2_d_, items2_h_, (size_t)n2 * sizeof(uint64_t), hipMemcpyHostToDevice,
+                                    stream_));
         SHIP(hipMemcpyAsync(hdr_d_, hdr_h_, sizeof(ServeHdr), hipMemcpyHostToDevice, stream_));
         hipGraphExec_t ge = nullptr;
-        rc = graph(rsteps, &ge, err);
+        rc = graph(rsteps, n2 > 0, &ge, err);
         if (rc != SCL_OK) return rc;
         SHIP(hipEventRecord(ev_[0], stream_));
         SHIP(hipGraphLaunch(ge, stream_));
@@ -239,7 +275,7 @@ class GpuServe final : public serve::Backend {
             GJob* g = (GJob*)jobs[k]->dev;
             jobs[k]->mass[0] = mass_h_[2 * k];
             jobs[k]->mass[1] = mass_h_[2 * k + 1];
-            g->cur ^= steps[k] & 1;
+            g->cur ^= launches(steps[k], two[k]) & 1;
             g->ucur = (g->ucur + steps[k]) % 3;
         }
         return SCL_OK;
@@ -285,6 +321,7 @@ class GpuServe final : public serve::Backend {
         *la = la_;
     }
     int workgroups() const override { return grid_; }
+    int steps_per_launch() const override { return x2_ ? 2 : 1; }
     double device_ms() const override { return device_ms_.load(); }
 
   private:
@@ -297,6 +334,35 @@ class GpuServe final : public serve::Backend {
 
     static size_t tiles_of(int nx, int ny, int nw, int rows) {
         return (size_t)((nx + 64 * nw - 1) / (64 * nw)) * (size_t)((ny + rows - 1) / rows);
+    }
+
+    // Launches that swap a job's buffers in a round of `steps` steps.
+    static int launches(int steps, bool two) { return two ? steps / 2 + (steps & 1) : steps; }
+
+    static constexpr int kOut2 = x2_out<StoreF32>();
+    static size_t tiles2_of(int nx, int ny, int rows) {
+        return (size_t)((nx + kOut2 - 1) / kOut2) * (size_t)((ny + rows - 1) / rows);
+    }
+
+    // Two-step tiles of job k, band by band.
+    static int emit_tiles2(int k, int nx, int ny, int rows, uint64_t* out, int n) {
+        const int gx = (nx + kOut2 - 1) / kOut2;
+        for (int y0 = 0; y0 < ny; y0 += rows)
+            for (int bx = 0; bx < gx; ++bx) out[n++] = pack_item(k, bx, y0, std::min(y0 + rows, ny));
+        return n;
+    }
+
+    int grow_items2(size_t need, std::string* err) {
+        if (need <= items2_cap_) return SCL_OK;
+        const size_t cap = std::max<size_t>(items2_cap_ * 2, need);
+        if (items2_d_) SHIP(hipFree(items2_d_));
+        if (items2_h_) SHIP(hipHostFree(items2_h_));
+        items2_d_ = nullptr;
+        items2_h_ = nullptr;
+        SHIP(hipMalloc(&items2_d_, cap * sizeof(uint64_t)));
+        SHIP(hipHostMalloc(&items2_h_, cap * sizeof(uint64_t), hipHostMallocDefault));
+        items2_cap_ = cap;
+        return SCL_OK;
     }
 
     // Tiles of job k, band by band, strips left to right within a band.
@@ -434,14 +500,20 @@ class GpuServe final : public serve::Backend {
         return SCL_OK;
     }
 
-    int graph(int rsteps, hipGraphExec_t* out, std::string* err) {
-        auto it = graphs_.find(rsteps);
+    // The round as a graph: rsteps single-step launches, or (two) per pair q
+    // a two-step launch then the single-step launches of steps 2q and 2q+1.
+    int graph(int rsteps, bool two, hipGraphExec_t* out, std::string* err) {
+        const int key = 2 * rsteps + (two ? 1 : 0);
+        auto it = graphs_.find(key);
         if (it != graphs_.end()) {
             *out = it->second;
             return SCL_OK;
         }
         SHIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-        for (int s = 0; s < rsteps; ++s) launch_step(nw_, la_, grid_, s);
+        for (int s = 0; s < rsteps; ++s) {
+            if (two && !(s & 1) && s + 1 < rsteps) launch_pair(s / 2);
+            launch_step(nw_, la_, grid_, s);
+        }
         hipLaunchKernelGGL(k_serve_mass<ST>, dim3(mass_grid_), dim3(256), 0, stream_, hdr_d_);
         hipLaunchKernelGGL(k_serve_mass_final<T>, dim3(cfg_.max_jobs), dim3(256), 0, stream_, hdr_d_);
         hipGraph_t gr = nullptr;
@@ -457,9 +529,14 @@ class GpuServe final : public serve::Backend {
             *err = std::string("graph instantiation failed: ") + hip_msg(e);
             return SCL_ERR_HIP;
         }
-        graphs_[rsteps] = ge;
+        graphs_[key] = ge;
         *out = ge;
         return SCL_OK;
+    }
+
+    void launch_pair(int q) {
+        if constexpr (std::is_same<ST, StoreF32>::value)
+            hipLaunchKernelGGL(k_serve2x<ST>, dim3(grid2_), dim3(kX2Threads), 0, stream_, hdr_d_, q, ctr_d_);
     }
 
     // Tile shape by measurement: two resident 2048 x 2048 porous jobs
@@ -524,12 +601,14 @@ class GpuServe final : public serve::Backend {
                         table[k].a = args_of(jobs[k]->spec, *g);
                         table[k].cur = g->cur;
                         table[k].steps = 10;
+                        table[k].x2 = 0;
+                        table[k].fin = g->cur;
                         fill_ueq(table[k], *g);
                         table[k].item_begin = n;
                         n = emit_tiles(k, g->nx, g->ny, nw, rows, items_h_, n);
                         table[k].item_end = n;
                     }
-                    *hdr_h_ = ServeHdr{table_d_, items_d_, partial_d_, mass_d_, n, 2, 64 * nw};
+                    *hdr_h_ = ServeHdr{table_d_, items_d_, partial_d_, mass_d_, n, 2, 64 * nw, n, items2_d_, 0};
                     if (hipMemcpyAsync(table_d_, table_h_, 2 * sizeof(JobArgs<T>), hipMemcpyHostToDevice,
                                        stream_) != hipSuccess ||
                         hipMemcpyAsync(items_d_, items_h_, (size_t)n * sizeof(uint64_t), hipMemcpyHostToDevice,
@@ -588,6 +667,11 @@ class GpuServe final : public serve::Backend {
     double* mass_d_ = nullptr;
     double* mass_h_ = nullptr;
     int nw_ = 4, rows_ = 32, la_ = 1, grid_ = 0, mass_grid_ = 1024;
+    bool x2_ = false;                  // two steps per launch for wide jobs (FP32 storage)
+    int x2rows_ = 32, grid2_ = 0;      // band rows of the two-step tiles; its persistent grid
+    uint64_t* items2_d_ = nullptr;
+    uint64_t* items2_h_ = nullptr;
+    size_t items2_cap_ = 0;
     std::map<int, hipGraphExec_t> graphs_;
     hipEvent_t ev_[2] = {nullptr, nullptr};
     std::atomic<double> device_ms_{0.0};   // written by the request loop, read by stats

@@ -64,15 +64,17 @@ struct X2Lds {
     typename ST::S pd[OPEN ? 1 : 2][2][3][kX2Threads];   // c_y = -1 (4, 7, 8)
 };
 
-// a1: step n (src = P_{n-1}, u_eq buffers of step n); a2: step n+1 (dst =
-// P_{n+1}, u_eq buffers of step n+1).  Geometry, rows per band and the
-// number of strips (gx) are taken from a1.  Requires nx >= kX2Threads.
+// One tile: strip bx (columns bx*OUT-3 .. bx*OUT+252), band of rows
+// [y0, y1).  a1: step n (src = P_{n-1}, u_eq buffers of step n); a2: step
+// n+1 (dst = P_{n+1}, u_eq buffers of step n+1); geometry from a1.
+// Requires nx >= kX2Threads; the caller separates consecutive tiles of a
+// workgroup with a barrier (L is reused).
 template <class ST, bool OPEN>
-__global__ __launch_bounds__(kX2Threads) void k_step2x(const StepArgs<float> a1, const StepArgs<float> a2) {
+__device__ inline void step2x_tile(const StepArgs<float>& a1, const StepArgs<float>& a2, const int bx, const int y0,
+                                   const int y1, X2Lds<ST, OPEN>& L) {
     using S = typename ST::S;
     using OffT = uint32_t;
     constexpr OffT ES = sizeof(S);
-    __shared__ X2Lds<ST, OPEN> L;
 
     const int nx = a1.nx, ny = a1.ny, wpr = a1.wpr;
     const OffT plane_b = (OffT)a1.plane * ES, pitch_b = (OffT)a1.pitch * ES;
@@ -86,19 +88,6 @@ __global__ __launch_bounds__(kX2Threads) void k_step2x(const StepArgs<float> a1,
         else return neigh_bits(s, c, n);
     };
 
-    // Workgroup -> (strip, band), XCD-aware as in k_step.
-    const int nbx = a1.gx;
-    const int nby = (ny + a1.rows - 1) / a1.rows;
-    int bid = blockIdx.x;
-    {
-        const int nblk = nbx * nby;
-        const int xcd = bid & 7, k = bid >> 3;
-        const int per = (nblk + 7) >> 3;
-        const int full = nblk - 8 * (per - 1);
-        bid = xcd < full ? xcd * per + k : full * per + (xcd - full) * (per - 1) + k;
-    }
-    const int bx = bid % nbx, by = bid / nbx;
-
     const int t = threadIdx.x;
     constexpr int OUT = x2_out<ST>();
     const int x0 = bx * OUT;
@@ -109,7 +98,6 @@ __global__ __launch_bounds__(kX2Threads) void k_step2x(const StepArgs<float> a1,
     const bool out_lane = t >= kX2Halo && t < kX2Halo + nout;
     const bool s1_lane = t >= 1 && t < kX2Threads - 1;   // step n valid
     const bool s2_lane = t >= 2 && t < kX2Threads - 2;   // step n+1 gather valid
-    const int y0 = by * a1.rows, y1 = min(y0 + a1.rows, ny);
     auto wr = [&](int r) {
         r %= ny;
         return r < 0 ? r + ny : r;
@@ -295,6 +283,27 @@ __global__ __launch_bounds__(kX2Threads) void k_step2x(const StepArgs<float> a1,
         row(r + 2, pZ, pX, pY);
         if (r + 3 > y1 + 1) break;
     }
+}
+
+// The lattice in one launch: strips of x2_out<ST>() columns (a1.gx of them)
+// times bands of a1.rows rows; workgroup ids remapped so that each XCD owns
+// a contiguous range of tiles, as in k_step.
+template <class ST, bool OPEN>
+__global__ __launch_bounds__(kX2Threads) void k_step2x(const StepArgs<float> a1, const StepArgs<float> a2) {
+    __shared__ X2Lds<ST, OPEN> L;
+    const int nbx = a1.gx;
+    const int nby = (a1.ny + a1.rows - 1) / a1.rows;
+    int bid = blockIdx.x;
+    {
+        const int nblk = nbx * nby;
+        const int xcd = bid & 7, k = bid >> 3;
+        const int per = (nblk + 7) >> 3;
+        const int full = nblk - 8 * (per - 1);
+        bid = xcd < full ? xcd * per + k : full * per + (xcd - full) * (per - 1) + k;
+    }
+    const int bx = bid % nbx, by = bid / nbx;
+    const int y0 = by * a1.rows;
+    step2x_tile<ST, OPEN>(a1, a2, bx, y0, min(y0 + a1.rows, a1.ny), L);
 }
 
 }  // namespace scl

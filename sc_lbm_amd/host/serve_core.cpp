@@ -535,6 +535,7 @@ int Scheduler::stats(scl_server_stats* st) {
     be_->shape(&st->tile_cols, &st->tile_rows, &st->lookahead);
     st->round_steps = cfg_.round_steps;
     st->workgroups = be_->workgroups();
+    st->steps_per_launch = be_->steps_per_launch();
     return SCL_OK;
 }
 

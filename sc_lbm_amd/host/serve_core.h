@@ -147,6 +147,8 @@ class Backend {
     // Launch shape in use (cols, rows, lookahead) and workgroups per step.
     virtual void shape(int* cols, int* rows, int* la) const = 0;
     virtual int workgroups() const { return 0; }
+    // Time steps a wide job advances per launch (1 or 2).
+    virtual int steps_per_launch() const { return 1; }
     // Device time of the rounds so far, in milliseconds (0 if not measured).
     virtual double device_ms() const { return 0.0; }
 };

@@ -59,7 +59,7 @@ class Config(ctypes.Structure):
         ("precision", ctypes.c_int), ("budget_bytes", ctypes.c_size_t), ("max_jobs", ctypes.c_int),
         ("max_pending", ctypes.c_int), ("round_steps", ctypes.c_int), ("tile_cols", ctypes.c_int),
         ("tile_rows", ctypes.c_int), ("lookahead", ctypes.c_int), ("mass_tol", ctypes.c_double),
-        ("max_bypass", ctypes.c_int),
+        ("max_bypass", ctypes.c_int), ("steps_per_launch", ctypes.c_int), ("band_rows", ctypes.c_int),
     ]
 
 
@@ -74,6 +74,7 @@ class Stats(ctypes.Structure):
         ("max_resident", ctypes.c_int), ("arena_bytes", ctypes.c_size_t), ("arena_used", ctypes.c_size_t),
         ("arena_peak", ctypes.c_size_t), ("tile_cols", ctypes.c_int), ("tile_rows", ctypes.c_int),
         ("lookahead", ctypes.c_int), ("round_steps", ctypes.c_int), ("workgroups", ctypes.c_int),
+        ("steps_per_launch", ctypes.c_int),
     ]
 
     def as_dict(self):

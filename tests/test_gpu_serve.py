@@ -204,3 +204,42 @@ def test_autotuned_server_runs_and_reports_its_shape():
     fixed = run_alone(dict(nx=200, ny=120, steps=33, ic_seed=9, radius=30, deliver=serve.DELIVER_F, **DROP))
     for c in range(2):
         assert np.array_equal(tuned[c], fixed[c])
+
+
+def test_two_steps_per_launch_in_the_server_keep_results():
+    """FP32 storage: jobs at least 256 columns wide advance two steps per
+    launch of the persistent kernel; an odd step count, a round shorter than
+    two steps and narrow jobs take single steps.  The result is the same bits
+    with one step per launch, in a mix, and on the single-lattice path."""
+    wide = dict(POROUS, nx=300, ny=120, steps=151, geom_seed=4, ic_seed=6, deliver=serve.DELIVER_F)
+    two = run_alone(wide, "fp32dev", round_steps=16)
+    one = run_alone(wide, "fp32dev", round_steps=16, steps_per_launch=1)
+    odd = run_alone(wide, "fp32dev", round_steps=5, band_rows=17)
+    for c in range(2):
+        assert np.array_equal(two[c], one[c])
+        assert np.array_equal(two[c], odd[c])
+    with gpu_server("fp32dev") as srv:
+        assert srv.stats()["steps_per_launch"] == 2
+        t0 = srv.stats()["clock_ms"]
+        others = [dict(nx=257, ny=64, steps=90, ic_seed=3, radius=12, **DROP),
+                  dict(nx=128, ny=200, steps=61, ic_seed=4, radius=20, **DROP),
+                  dict(nx=512, ny=40, steps=33, ic_seed=5, radius=10, release_ms=t0 + 3.0, **DROP)]
+        ids = [srv.submit(**o) for o in others[:2]]
+        jid = srv.submit(**dict(wide, release_ms=t0 + 1.0))
+        ids.append(srv.submit(**others[2]))
+        assert srv.wait(jid, 60000)["state"] == "done"
+        mixed = [srv.result(jid, serve.F, c) for c in range(2)]
+        for i in ids:
+            assert srv.wait(i, 60000)["state"] == "done"
+    for c in range(2):
+        assert np.array_equal(two[c], mixed[c])
+    case = wl.served_case(**{k: v for k, v in wide.items() if k not in ("steps", "deliver")})
+    g = nat.Geometry(case["mask"])
+    fl = [nat.Field(g, case["tau"][c], nat.FP32_DEV, 1.0, g_solid=case["g_solid"][c], g_fluid=case["g_fluid"][c])
+          for c in range(2)]
+    for c in range(2):
+        fl[c].set_f(case["f"][c])
+    nat.solid_surface_force(g, fl[0], fl[1])
+    nat.step2(g, fl[0], fl[1], case["force"][0], case["force"][1], wide["steps"])
+    for c in range(2):
+        assert np.array_equal(fl[c].get_f().reshape(-1), two[c].reshape(-1))
