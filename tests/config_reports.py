@@ -223,15 +223,18 @@ def c3(rep):
     same = all(np.array_equal(f200[c], ref[c]) for c in range(2))
     rep.append("- FP64 populations after 200 steps equal to the oracle's bit for bit: **%s**" % same)
     out = {}
-    runs = [("fp64", case, "fp64"), ("fp32dev", case, "fp32dev")] + \
-           [("fp64 perturbed %.0e" % rel, perturbed(case, rel), "fp64") for rel in PERTS]
-    for name, cs, form in runs:
-        r = Run(cs, form, rho0=1.0)
+    # FP32 deviations about rho0 = 1 and about the midpoint of the two phases'
+    # densities (the storage spacing scales with |f - w rho0|)
+    runs = [("fp64", case, "fp64", 1.0), ("fp32dev (rho0 1.0)", case, "fp32dev", 1.0),
+            ("fp32dev (rho0 0.53)", case, "fp32dev", 0.53)] + \
+           [("fp64 perturbed %.0e" % rel, perturbed(case, rel), "fp64", 1.0) for rel in PERTS]
+    for name, cs, form, rho0 in runs:
+        r = Run(cs, form, rho0=rho0)
         r.step(10000)
         out[name] = [r.f(c) for c in range(2)]
         r.close()
     m = case["mask"]
-    for name, _, _ in runs[1:]:
+    for name, _, _, _ in runs[1:]:
         drho = max(float(np.max(np.abs(out["fp64"][c].reshape(9, -1).sum(0) - out[name][c].reshape(9, -1).sum(0))))
                    for c in range(2))
         du = 0.0
