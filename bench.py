@@ -168,7 +168,12 @@ def served_stream(nat, a):
                 offered_load=a.serve_load, capacity_mlups=cap, arrival_rate_per_s=rate,
                 p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
                 mean_ms=float(lat.mean()), max_ms=float(lat.max()),
-                aggregate_mlups=lu_done / (last - first) / 1e3, busy_mlups=st2["lattice_updates"] / st2["busy_ms"] / 1e3,
+                aggregate_mlups=lu_done / (last - first) / 1e3,
+                # fluid sites only (solids count in the dense figure above)
+                aggregate_fluid_mlups=(st2["fluid_updates"] - st1["fluid_updates"]) / (last - first) / 1e3,
+                fluid_fraction=(st2["fluid_updates"] - st1["fluid_updates"]) /
+                max(1, st2["lattice_updates"] - st1["lattice_updates"]),
+                busy_mlups=st2["lattice_updates"] / st2["busy_ms"] / 1e3,
                 device_mlups=st2["lattice_updates"] / st2["device_ms"] / 1e3 if st2["device_ms"] else None,
                 busy_ms=st2["busy_ms"], device_ms=st2["device_ms"], setup_ms=st2["setup_ms"],
                 finish_ms=st2["finish_ms"], rounds=st2["rounds"],

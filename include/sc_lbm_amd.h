@@ -297,6 +297,7 @@ typedef struct scl_server_stats {
     size_t arena_bytes, arena_used, arena_peak;
     int tile_cols, tile_rows, lookahead, round_steps, workgroups;
     int steps_per_launch;       /* 2 when wide jobs take two steps per launch          */
+    long long fluid_updates;    /* lattice updates of fluid sites only                 */
 } scl_server_stats;
 
 /* Defaults: FP32 deviations, 8 GiB arena, 64 slots, 4096 pending, rounds of

@@ -352,10 +352,13 @@ void Scheduler::prep_loop() {
         job_mask(g, &m);
         std::vector<uint64_t> bits;
         pack_mask(m, g.nx, g.ny, &bits);
+        long long nf = 0;
+        for (uint64_t w : bits) nf += __builtin_popcountll(w);
         lk.lock();
         it = jobs_.find(id);
         if (it != jobs_.end() && !finished(it->second->state)) {
             it->second->bits.swap(bits);
+            it->second->fluid_sites = nf;
             it->second->prepared = true;
             cv_work_.notify_one();
         }
@@ -521,6 +524,7 @@ int Scheduler::stats(scl_server_stats* st) {
     st->rounds = rounds_;
     st->job_steps = job_steps_;
     st->lattice_updates = lups_;
+    st->fluid_updates = fluid_lups_;
     st->busy_ms = busy_ms_;
     st->device_ms = be_->device_ms();
     st->setup_ms = setup_ms_;
@@ -730,6 +734,7 @@ void Scheduler::loop() {
             for (size_t k = 0; k < jobs.size(); ++k) {
                 job_steps_ += steps[k];
                 lups_ += jobs[k]->sites() * steps[k];
+                fluid_lups_ += jobs[k]->fluid_sites * steps[k];
             }
         }
         for (auto& e : failed) {

@@ -114,6 +114,7 @@ struct Job {
     std::vector<double> rho[2], f[2]; // deliverables (reference layout)
     std::vector<uint64_t> bits;       // packed mask (pack_mask), built before admission
     bool prepared = false;
+    long long fluid_sites = 0;        // fluid sites of the mask (set with bits)
     void* dev = nullptr;              // backend state
     long long sites() const { return (long long)spec.nx * spec.ny; }
 };
@@ -198,7 +199,8 @@ class Scheduler {
     std::chrono::steady_clock::time_point t0_;
     // statistics
     long long n_submitted_ = 0, n_admitted_ = 0, n_done_ = 0, n_failed_ = 0, n_cancelled_ = 0,
-              n_timed_out_ = 0, n_busy_ = 0, n_rejected_ = 0, rounds_ = 0, job_steps_ = 0, lups_ = 0;
+              n_timed_out_ = 0, n_busy_ = 0, n_rejected_ = 0, rounds_ = 0, job_steps_ = 0, lups_ = 0,
+              fluid_lups_ = 0;
     double busy_ms_ = 0, setup_ms_ = 0, finish_ms_ = 0;
     int max_resident_ = 0;
 };

@@ -74,7 +74,7 @@ class Stats(ctypes.Structure):
         ("max_resident", ctypes.c_int), ("arena_bytes", ctypes.c_size_t), ("arena_used", ctypes.c_size_t),
         ("arena_peak", ctypes.c_size_t), ("tile_cols", ctypes.c_int), ("tile_rows", ctypes.c_int),
         ("lookahead", ctypes.c_int), ("round_steps", ctypes.c_int), ("workgroups", ctypes.c_int),
-        ("steps_per_launch", ctypes.c_int),
+        ("steps_per_launch", ctypes.c_int), ("fluid_updates", ctypes.c_longlong),
     ]
 
     def as_dict(self):

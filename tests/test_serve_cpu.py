@@ -307,3 +307,22 @@ def test_wait_timeout_and_unknown_ids(cpulib):
                 srv.status(a)
     finally:
         cpulib.sct_set_round_delay(0)
+
+
+def test_fluid_updates_count_fluid_sites_only(cpulib):
+    """The statistics count lattice updates of fluid sites beside all sites:
+    a generated porous job and a job with a caller mask."""
+    gen = dict(nx=48, ny=40, steps=20, porosity=0.8, rmin=3, rmax=6, walls=True, geom_seed=5, ic_seed=1, radius=8,
+               **DROP)
+    case = wl.porous(nx=40, ny=24, porosity=0.8, seed=3, rmin=3, rmax=5)
+    caller = dict(nx=40, ny=24, steps=12, mask=case["mask"], f=case["f"], tau=case["tau"], g_fluid=case["g_fluid"])
+    with cpu_server(cpulib) as srv:
+        ids = [srv.submit(**gen), srv.submit(**caller)]
+        for i in ids:
+            assert srv.wait(i, 30000)["state"] == "done"
+        st = srv.stats()
+    fluid_gen = int(serve.job_mask(**gen).sum())
+    fluid_caller = int(case["mask"].sum())
+    assert st["lattice_updates"] == 48 * 40 * 20 + 40 * 24 * 12
+    assert st["fluid_updates"] == fluid_gen * 20 + fluid_caller * 12
+    assert 0 < st["fluid_updates"] < st["lattice_updates"]
