@@ -22,6 +22,8 @@ the quantities the configuration is judged on:
 The oracle is the checker here, never the thing measured.
 """
 import argparse
+import hashlib
+import json
 import os
 import sys
 import time
@@ -36,6 +38,20 @@ from sc_lbm_amd import native as nat  # noqa: E402
 from sc_lbm_amd import workloads as wl  # noqa: E402
 
 FORMS = {"fp64": nat.FP64, "fp32dev": nat.FP32_DEV, "fp16dev": nat.FP16_DEV}
+# SHA-256 of the compiled reference's final populations for the long cases
+# (tests/golden/make_config_digests.py)
+DIGESTS = os.path.join(ROOT, "tests", "golden", "config_digests.json")
+
+
+def ref_digest_line(name, f, steps):
+    """Whether the FP64 populations equal the compiled reference's after the
+    case's full step count (by SHA-256 of their float64 bytes)."""
+    if not os.path.exists(DIGESTS) or name not in json.load(open(DIGESTS)):
+        return "- (no reference digest for %s)" % name
+    want = json.load(open(DIGESTS))[name]
+    got = [hashlib.sha256(np.ascontiguousarray(x, dtype="<f8").tobytes()).hexdigest() for x in f]
+    return ("- FP64 populations after %d steps equal to the compiled reference's bit for bit (SHA-256 of both "
+            "components): **%s**" % (steps, got == want["f_sha256"] and want["steps"] == steps))
 
 
 def log(msg):
@@ -164,6 +180,7 @@ def c2(rep):
     same = all(np.array_equal(f2000[c], ref[c]) for c in range(2))
     rep.append("- FP64 populations after 2000 steps equal to the oracle's bit for bit: **%s**" % same)
     r.step(18000)
+    rep.append(ref_digest_line("c2", [r.f(c) for c in range(2)], 20000))
     rho64 = [r.rho(c) for c in range(2)]
     r.close()
     R64, dp64 = droplet_stats(rho64[0], rho64[1], 0.9)
@@ -186,6 +203,7 @@ def c2(rep):
     same = all(np.array_equal(f2000[c], ref[c]) for c in range(2))
     rep.append("- FP64 populations after 2000 steps equal to the oracle's bit for bit: **%s**" % same)
     r.step(18000)
+    rep.append(ref_digest_line("c2_hot", [r.f(c) for c in range(2)], 20000))
     rho64 = [r.rho(c) for c in range(2)]
     r.close()
     R64, dp64 = droplet_stats(rho64[0], rho64[1], 2.2)
@@ -228,11 +246,21 @@ def c3(rep):
     runs = [("fp64", case, "fp64", 1.0), ("fp32dev (rho0 1.0)", case, "fp32dev", 1.0),
             ("fp32dev (rho0 0.53)", case, "fp32dev", 0.53)] + \
            [("fp64 perturbed %.0e" % rel, perturbed(case, rel), "fp64", 1.0) for rel in PERTS]
+    n_fluid = int(case["mask"].sum())
     for name, cs, form, rho0 in runs:
         r = Run(cs, form, rho0=rho0)
-        r.step(10000)
+        r.step(2)   # bootstrap and shape selection outside the timing
+        nat.synchronize()
+        t = time.perf_counter()
+        r.step(9998)
+        nat.synchronize()
+        dt = time.perf_counter() - t
         out[name] = [r.f(c) for c in range(2)]
         r.close()
+        if "perturbed" not in name:
+            rep.append("- %s throughput: %.0f MLUPS (all sites), %.0f MLUPS on fluid sites"
+                       % (name, case["mask"].size * 9998 / dt / 1e6, n_fluid * 9998 / dt / 1e6))
+    rep.append(ref_digest_line("c3", out["fp64"], 10000))
     m = case["mask"]
     for name, _, _, _ in runs[1:]:
         drho = max(float(np.max(np.abs(out["fp64"][c].reshape(9, -1).sum(0) - out[name][c].reshape(9, -1).sum(0))))
