@@ -318,9 +318,39 @@ __device__ inline void finish1(const CompArgs<typename ST::T>& ca, void* dst, lo
 // component takes the equilibrium velocity of the previous step from
 // ueqx_in / ueqy_in and carries it forward into ueqx / ueqy; a component
 // below kUeqWatch records its new one there.
+// Equilibrium-velocity buffers of one step: [component] read (previous
+// step's) and written.
+struct UeqPtrs {
+    const double* in_x[2];
+    const double* in_y[2];
+    double* out_x[2];
+    double* out_y[2];
+};
+
+template <class T>
+__device__ inline UeqPtrs ueq_ptrs(const StepArgs<T>& a) {
+    return UeqPtrs{{a.c[0].ueqx_in, a.c[1].ueqx_in}, {a.c[0].ueqy_in, a.c[1].ueqy_in},
+                   {a.c[0].ueqx, a.c[1].ueqx}, {a.c[0].ueqy, a.c[1].ueqy}};
+}
+
+// The collision with the physics parameters of `a` and the equilibrium-
+// velocity buffers of `u` (the two-step kernel uses one parameter set for
+// both of its steps).
+template <class T>
+__device__ inline void collide2_core(const StepArgs<T>& a, const UeqPtrs& u, int s_idx, const float (&p)[2][Q],
+                                     const float (&rc)[2], const float (&nr)[2][Q], unsigned nb, float (&oa)[Q],
+                                     float (&ob)[Q]);
+
 template <class T>
 __device__ inline void collide2_dev(const StepArgs<T>& a, int s_idx, const float (&p)[2][Q], const float (&rc)[2],
                                     const float (&nr)[2][Q], unsigned nb, float (&oa)[Q], float (&ob)[Q]) {
+    collide2_core(a, ueq_ptrs(a), s_idx, p, rc, nr, nb, oa, ob);
+}
+
+template <class T>
+__device__ inline void collide2_core(const StepArgs<T>& a, const UeqPtrs& u, int s_idx, const float (&p)[2][Q],
+                                     const float (&rc)[2], const float (&nr)[2][Q], unsigned nb, float (&oa)[Q],
+                                     float (&ob)[Q]) {
     // fused multiply-adds written out, as in relax_dev
     const auto& A = a.c[0];
     const auto& B = a.c[1];
@@ -374,30 +404,30 @@ __device__ inline void collide2_dev(const StepArgs<T>& a, int s_idx, const float
         uea[0] = fmaf(A.inv_omega, fmaf(fa_x, ir, fs_ax), ucx);
         uea[1] = fmaf(A.inv_omega, fmaf(fa_y, ir, fs_ay), ucy);
         if (fabsf(ra) < (float)kUeqWatch) {
-            G(A.ueqx)[s_idx] = uea[0];
-            G(A.ueqy)[s_idx] = uea[1];
+            G(u.out_x[0])[s_idx] = uea[0];
+            G(u.out_y[0])[s_idx] = uea[1];
         }
     } else {
-        const double ux0 = G(A.ueqx_in)[s_idx], uy0 = G(A.ueqy_in)[s_idx];
+        const double ux0 = G(u.in_x[0])[s_idx], uy0 = G(u.in_y[0])[s_idx];
         uea[0] = (float)ux0;
         uea[1] = (float)uy0;
-        G(A.ueqx)[s_idx] = ux0;
-        G(A.ueqy)[s_idx] = uy0;
+        G(u.out_x[0])[s_idx] = ux0;
+        G(u.out_y[0])[s_idx] = uy0;
     }
     if (!is_dry(rb)) {
         const float ir = rcp(rb);
         ueb[0] = fmaf(B.inv_omega, fmaf(fb_x, ir, fs_bx), ucx);
         ueb[1] = fmaf(B.inv_omega, fmaf(fb_y, ir, fs_by), ucy);
         if (fabsf(rb) < (float)kUeqWatch) {
-            G(B.ueqx)[s_idx] = ueb[0];
-            G(B.ueqy)[s_idx] = ueb[1];
+            G(u.out_x[1])[s_idx] = ueb[0];
+            G(u.out_y[1])[s_idx] = ueb[1];
         }
     } else {
-        const double ux0 = G(B.ueqx_in)[s_idx], uy0 = G(B.ueqy_in)[s_idx];
+        const double ux0 = G(u.in_x[1])[s_idx], uy0 = G(u.in_y[1])[s_idx];
         ueb[0] = (float)ux0;
         ueb[1] = (float)uy0;
-        G(B.ueqx)[s_idx] = ux0;
-        G(B.ueqy)[s_idx] = uy0;
+        G(u.out_x[1])[s_idx] = ux0;
+        G(u.out_y[1])[s_idx] = uy0;
     }
     relax_dev(p[0], ra, da0, uea[0], uea[1], A.omega, A.force, A.has_force, oa);
     relax_dev(p[1], rb, db0, ueb[0], ueb[1], B.omega, B.force, B.has_force, ob);

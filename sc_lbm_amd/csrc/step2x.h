@@ -40,7 +40,10 @@
 
 namespace scl {
 
-constexpr int kX2Threads = 256;
+#ifndef SCL_X2_THREADS
+#define SCL_X2_THREADS 256
+#endif
+constexpr int kX2Threads = SCL_X2_THREADS;
 constexpr int kX2Halo = 3;
 // Output columns per strip: 224 with FP32 storage (seven whole 128-byte
 // lines per row and plane, so no line is written partly by two strips);
@@ -48,7 +51,9 @@ constexpr int kX2Halo = 3;
 // arithmetic than the split lines cost in traffic).  Stores are cached
 // rather than streamed: the two halves of a split line then merge in L2.
 template <class ST>
-__host__ __device__ constexpr int x2_out() { return sizeof(typename ST::S) == 4 ? 224 : kX2Threads - 2 * kX2Halo; }
+__host__ __device__ constexpr int x2_out() {
+    return sizeof(typename ST::S) == 4 ? kX2Threads - 32 : kX2Threads - 2 * kX2Halo;
+}
 
 // direction class (0: c_y = +1, 1: c_y = 0, 2: c_y = -1) and index within it
 __host__ __device__ constexpr int x2_class(int i) { return cy(i) == 1 ? 0 : (cy(i) == 0 ? 1 : 2); }
@@ -88,6 +93,7 @@ __device__ inline void step2x_tile(const StepArgs<float>& a1, const StepArgs<flo
         else return neigh_bits(s, c, n);
     };
 
+    const UeqPtrs u1 = ueq_ptrs(a1), u2 = ueq_ptrs(a2);
     const int t = threadIdx.x;
     constexpr int OUT = x2_out<ST>();
     const int x0 = bx * OUT;
@@ -213,7 +219,7 @@ __device__ inline void step2x_tile(const StepArgs<float>& a1, const StepArgs<flo
         if (s1_lane && (OPEN || (nb_0 & 1u))) {
             float nr[2][Q], rc[2], oa[Q], ob[Q];
             nbr_dens(L.d1, r, nr, rc);
-            collide2_dev(a1, wr(r) * a1.pitch + X, pa, rc, nr, OPEN ? 0x1FFu : nb_0, oa, ob);
+            collide2_core(a1, u1, wr(r) * a1.pitch + X, pa, rc, nr, OPEN ? 0x1FFu : nb_0, oa, ob);
 #pragma unroll
             for (int i = 1; i < Q; ++i) {
                 pr(r, 0, i, t) = ST::enc(oa[i]);
@@ -253,7 +259,8 @@ __device__ inline void step2x_tile(const StepArgs<float>& a1, const StepArgs<flo
         if (h >= y0 && h < y1 && out_lane && (OPEN || (nb_m2 & 1u))) {
             float nr[2][Q], rc[2], oa[Q], ob[Q];
             nbr_dens(L.d2, h, nr, rc);
-            collide2_dev(a2, h * a2.pitch + X, s2p, rc, nr, OPEN ? 0x1FFu : nb_m2, oa, ob);
+            // step n+1: the parameters of a1 (the same), the buffers of a2
+            collide2_core(a1, u2, h * a1.pitch + X, s2p, rc, nr, OPEN ? 0x1FFu : nb_m2, oa, ob);
             const OffT home = (OffT)h * pitch_b + (OffT)X * ES;
 #pragma unroll
             for (int i = 0; i < Q; ++i) {
