@@ -4,9 +4,11 @@
 // A launch reads P_{n-1} and writes P_{n+1}; the intermediate post-collision
 // populations P_n never leave the workgroup, so a population crosses HBM
 // once per two steps.  The per-site arithmetic is that of the single-step
-// kernel (collide2_dev, the same gather and bounce-back, P_n rounded to the
+// kernel (collide2_core, the same gather and bounce-back, P_n rounded to the
 // storage form before it is gathered again), so the result is bit for bit
-// that of two single steps.
+// that of two single steps.  Both collisions take their physics parameters
+// from a1 (a2 holds the same ones; reading one set keeps the scalar
+// registers from spilling) and their equilibrium-velocity buffers apart.
 //
 // Work decomposition.  A workgroup of 256 threads owns one column per thread
 // over a strip of 256 columns and walks a band of rows from south to north.
@@ -40,16 +42,20 @@
 
 namespace scl {
 
+// Threads (columns) per strip.  Wider strips re-read fewer halo columns but
+// synchronise more waves per barrier: at 4096^2 FP32, 384 / 448 / 512
+// threads ran at 43.7k / 49.4k / 49.6k MLUPS against 57.6k with 256.
 #ifndef SCL_X2_THREADS
 #define SCL_X2_THREADS 256
 #endif
 constexpr int kX2Threads = SCL_X2_THREADS;
 constexpr int kX2Halo = 3;
-// Output columns per strip: 224 with FP32 storage (seven whole 128-byte
-// lines per row and plane, so no line is written partly by two strips);
-// with FP16 storage every valid column, 250 (the aligned 192 would cost more
-// arithmetic than the split lines cost in traffic).  Stores are cached
-// rather than streamed: the two halves of a split line then merge in L2.
+// Output columns per strip: with FP32 storage the threads minus 32, i.e.
+// 224 (seven whole 128-byte lines per row and plane, so no line is written
+// partly by two strips); with FP16 storage every valid column, 250 (the
+// aligned 192 would cost more arithmetic than the split lines cost in
+// traffic).  Stores are cached rather than streamed: the two halves of a
+// split line then merge in L2.
 template <class ST>
 __host__ __device__ constexpr int x2_out() {
     return sizeof(typename ST::S) == 4 ? kX2Threads - 32 : kX2Threads - 2 * kX2Halo;
