@@ -136,7 +136,7 @@ def served_stream(nat, a):
         sts = []
         for k, i in enumerate(ids):
             sts.append(srv.wait(i))
-            if (k + 1) % 500 == 0:
+            if (k + 1) % 100 == 0:
                 log("  %d jobs finished" % (k + 1))
         rho32 = {k: [srv.result(ids[k], serve.RHO, c) for c in range(2)]
                  for k in sample if sts[k]["state"] == "done"}
@@ -162,7 +162,12 @@ def served_stream(nat, a):
                               ic="droplet" if j["ic"] == serve.IC_DROPLET else "stratified",
                               rho_min_fp64=min(float(s64.result(i, serve.RHO, c).min()) for c in range(2))))
     worst.sort(key=lambda w: -w["err"])
-    return dict(jobs=len(jobs), done=len(done), failed=sum(s["state"] == "failed" for s in sts),
+    failed_jobs = [dict(job=k, nx=jobs[k]["nx"], ny=jobs[k]["ny"], steps=jobs[k]["steps"],
+                        tau=round(jobs[k]["tau"][0], 4), G=round(jobs[k]["g_fluid"][0], 4),
+                        porosity=round(jobs[k]["porosity"], 4), error=sts[k].get("error", ""),
+                        steps_done=sts[k].get("steps_done"))
+                   for k, st in enumerate(sts) if st["state"] == "failed"]
+    return dict(jobs=len(jobs), done=len(done), failed=sum(s["state"] == "failed" for s in sts), failed_jobs=failed_jobs,
                 step_scale=a.serve_step_scale, steps_range=[int(round(2000 * a.serve_step_scale)),
                                                             int(round(50000 * a.serve_step_scale))],
                 offered_load=a.serve_load, capacity_mlups=cap, arrival_rate_per_s=rate,
