@@ -256,6 +256,11 @@ typedef struct scl_job {
     double release_ms;          /* earliest admission, on the server clock            */
     double timeout_ms;          /* > 0: dropped (TIMED_OUT) if not done this long after release */
     int deliver;                /* SCL_DELIVER_* bits                                  */
+    /* deviation forms: reference density rho0 per component (populations are
+     * stored as f_i - w_i rho0); 0 = automatic: the midpoint of rho_hi and
+     * rho_lo for a generated state, the mean density of the sites holding
+     * populations for caller arrays; rounded to a multiple of 2^-10 */
+    double rho0[2];
 } scl_job;
 
 typedef struct scl_job_status {

@@ -788,7 +788,9 @@ int scl_field_create(const scl_geometry* g, int precision, double tau, double rh
     f->omega = 1 / tau;              // include/fluid.h:62
     f->omc = 1.0 - f->omega;         // src/lbm.cpp:284, 308
     f->inv_omega = 1.0 / f->omega;   // src/lbm.cpp:213, 226
-    f->rho0 = precision == SCL_FP64 ? 0.0 : rho0;
+    // the kernels hold rho0 in FP32 and the conversions in FP64: one value
+    // representable in both, so the encoding and the kernels agree
+    f->rho0 = precision == SCL_FP64 ? 0.0 : (double)(float)rho0;
     const size_t bytes = n_pop(g) * f->esize();
     for (int b = 0; b < 2; ++b) {
         HIPCHK(hipMalloc(&f->f[b], bytes));

@@ -285,9 +285,9 @@ class GpuServe final : public serve::Backend {
         SHIP(hipSetDevice(dev_));
         const GJob* g = (const GJob*)j.dev;
         const Grid2 gr = g->grid();
-        const double rho0 = ST::kDev ? 1.0 : 0.0;
         const size_t n = (size_t)g->nx * g->ny;
         for (int c = 0; c < 2; ++c) {
+            const double rho0 = ST::kDev ? j.spec.rho0[c] : 0.0;
             const S* P = (const S*)g->f(c, g->cur);
             if (j.spec.deliver & SCL_DELIVER_RHO) {
                 hipLaunchKernelGGL(k_job_density<ST>, site_grid(g->nx, g->ny), dim3(256), 0, stream_, gr, P, rho0,
@@ -413,7 +413,7 @@ class GpuServe final : public serve::Backend {
             ca.inv_omega = (T)(1.0 / omega);
             ca.gneg = (T)(-1.0 * s.g_fluid[c]);
             ca.gsneg = (T)(-1.0 * s.g_solid[c]);
-            ca.rho0 = (T)(ST::kDev ? 1.0 : 0.0);
+            ca.rho0 = (T)(ST::kDev ? s.rho0[c] : 0.0);
             ca.has_force = s.has_force[c] ? 1 : 0;
             for (int i = 0; i < Q; ++i) ca.force[i] = (T)s.force[c][i];
             ca.ueqx = g.ueq(0, c, 0);
@@ -448,8 +448,8 @@ class GpuServe final : public serve::Backend {
         SHIP(hipMemcpyAsync(base + L.mask, j.bits.data(), j.bits.size() * 8, hipMemcpyHostToDevice, stream_));
         const Grid2 gr = g->grid();
         const dim3 sg = site_grid(s.nx, s.ny);
-        const double rho0 = ST::kDev ? 1.0 : 0.0;
         for (int c = 0; c < 2; ++c) {
+            const double rho0 = ST::kDev ? s.rho0[c] : 0.0;
             S* s0 = (S*)g->f(c, 0);
             if (!s.f[0].empty()) {
                 SHIP(hipMemcpy2DAsync(g->stage(), (size_t)L.pitch * 8, s.f[c].data(), (size_t)s.nx * 8,

@@ -164,6 +164,34 @@ int spec_from_job(const scl_job* j, JobSpec* s, std::string* err) {
     s->release_ms = j->release_ms;
     s->timeout_ms = j->timeout_ms;
     s->deliver = j->deliver;
+    // Deviation reference density: storage spacing scales with |f - w rho0|,
+    // so the middle of the densities present keeps the worst phase finest.
+    for (int c = 0; c < 2; ++c) {
+        double r = j->rho0[c];
+        if (!(r >= 0.0) || !std::isfinite(r)) {
+            *err = "rho0 must be finite and >= 0";
+            return SCL_ERR_ARG;
+        }
+        if (r == 0.0) {
+            if (j->f[0]) {
+                double sum = 0.0;
+                long long cnt = 0;
+                for (size_t k = 0; k < n; ++k) {
+                    double d = 0.0;
+                    for (int i = 0; i < 9; ++i) d += s->f[c][(size_t)i * n + k];
+                    if (d != 0.0) {
+                        sum += d;
+                        ++cnt;
+                    }
+                }
+                r = cnt ? sum / cnt : 1.0;
+            } else {
+                r = 0.5 * (s->rho_hi + s->rho_lo);
+            }
+        }
+        r = std::round(r * 1024.0) / 1024.0;
+        s->rho0[c] = r > 0.0 && r < 1e6 ? r : 1.0;
+    }
     return SCL_OK;
 }
 

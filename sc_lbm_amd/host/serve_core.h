@@ -72,6 +72,7 @@ struct JobSpec {
     long long steps = 0;
     double release_ms = 0.0, timeout_ms = 0.0;
     int deliver = 0;
+    double rho0[2] = {1.0, 1.0};      // deviation reference density (resolved, 2^-10 multiples)
 };
 
 // Validates and copies a caller's scl_job.  Returns SCL_OK or an error with

@@ -36,7 +36,7 @@ class Job(ctypes.Structure):
         ("radius", ctypes.c_double), ("noise", ctypes.c_double), ("ic_seed", ctypes.c_uint64),
         ("tau", ctypes.c_double * 2), ("g_fluid", ctypes.c_double * 2), ("g_solid", ctypes.c_double * 2),
         ("force", _dp * 2), ("steps", ctypes.c_longlong), ("release_ms", ctypes.c_double),
-        ("timeout_ms", ctypes.c_double), ("deliver", ctypes.c_int),
+        ("timeout_ms", ctypes.c_double), ("deliver", ctypes.c_int), ("rho0", ctypes.c_double * 2),
     ]
 
 
@@ -136,7 +136,7 @@ def _check(lib, rc):
 def make_job(nx, ny, steps, mask=None, porosity=1.0, rmin=8, rmax=20, walls=False, geom_seed=0, f=None,
              ic=IC_UNIFORM, rho_hi=1.0, rho_lo=1.0, radius=0.0, noise=0.0, ic_seed=0, tau=(1.0, 1.0),
              g_fluid=(0.0, 0.0), g_solid=(0.0, 0.0), force=(None, None), release_ms=0.0, timeout_ms=0.0,
-             deliver=0):
+             deliver=0, rho0=(0.0, 0.0)):
     """An scl_job plus the arrays it points to (kept alive in job._keep)."""
     j = Job()
     keep = []
@@ -164,6 +164,7 @@ def make_job(nx, ny, steps, mask=None, porosity=1.0, rmin=8, rmax=20, walls=Fals
             keep.append(a)
             j.force[c] = a.ctypes.data_as(_dp)
     j.steps, j.release_ms, j.timeout_ms, j.deliver = int(steps), float(release_ms), float(timeout_ms), int(deliver)
+    j.rho0[0], j.rho0[1] = float(rho0[0]), float(rho0[1])
     j._keep = keep
     return j
 

@@ -4,6 +4,8 @@ alone, in a mix of jobs of other sizes admitted while it runs, under
 memory back-pressure, with other tile shapes, and on the single-lattice
 path; the watchdog isolates a failing job; FP32 deviations stay within
 1e-5 of FP64 on a served job."""
+import math
+
 import numpy as np
 import pytest
 
@@ -58,6 +60,13 @@ def footprint(nx, ny, es, stage):
     nueq = 1 if es == 8 else 3
     total = a(ny * pitch // 64 * 8) + 4 * a(9 * plane * es) + (4 * nueq + 2) * a(plane * 8)
     return total + (a(9 * plane * 8) if stage else 0)
+
+
+def served_rho0(job):
+    """The server's automatic deviation reference density of a generated
+    initial state: the midpoint of rho_hi and rho_lo, rounded (half away from
+    zero) to a multiple of 2^-10."""
+    return math.floor(0.5 * (job["rho_hi"] + job["rho_lo"]) * 1024.0 + 0.5) / 1024.0
 
 
 def run_alone(job, precision="fp32dev", **cfg):
@@ -133,7 +142,7 @@ def test_alone_equals_mix_and_single_lattice_path(precision):
     # the single-lattice path from the same initial state
     case = wl.served_case(**POROUS)
     g = nat.Geometry(case["mask"])
-    fl = [nat.Field(g, case["tau"][c], nat.PRECISION[precision], 1.0, g_solid=case["g_solid"][c],
+    fl = [nat.Field(g, case["tau"][c], nat.PRECISION[precision], served_rho0(POROUS), g_solid=case["g_solid"][c],
                     g_fluid=case["g_fluid"][c]) for c in range(2)]
     for c in range(2):
         fl[c].set_f(case["f"][c])
@@ -235,11 +244,37 @@ def test_two_steps_per_launch_in_the_server_keep_results():
         assert np.array_equal(two[c], mixed[c])
     case = wl.served_case(**{k: v for k, v in wide.items() if k not in ("steps", "deliver")})
     g = nat.Geometry(case["mask"])
-    fl = [nat.Field(g, case["tau"][c], nat.FP32_DEV, 1.0, g_solid=case["g_solid"][c], g_fluid=case["g_fluid"][c])
-          for c in range(2)]
+    fl = [nat.Field(g, case["tau"][c], nat.FP32_DEV, served_rho0(wide), g_solid=case["g_solid"][c],
+                    g_fluid=case["g_fluid"][c]) for c in range(2)]
     for c in range(2):
         fl[c].set_f(case["f"][c])
     nat.solid_surface_force(g, fl[0], fl[1])
     nat.step2(g, fl[0], fl[1], case["force"][0], case["force"][1], wide["steps"])
     for c in range(2):
         assert np.array_equal(fl[c].get_f().reshape(-1), two[c].reshape(-1))
+
+
+def test_reference_density_of_the_deviation_forms():
+    """A served job stores f_i - w_i rho0 with rho0 the job's (automatic: the
+    midpoint of its phases' densities): the same bits as the single-lattice
+    path with that rho0; an explicit rho0 = 1 gives other bits, the same
+    physics within the FP32 tolerance, and its own single-lattice twin."""
+    job = dict(POROUS, steps=120, deliver=serve.DELIVER_F)
+    auto = run_alone(job, "fp32dev")
+    one = run_alone(dict(job, rho0=(1.0, 1.0)), "fp32dev")
+    assert served_rho0(job) == 543.0 / 1024.0
+    assert not np.array_equal(auto[0], one[0])
+    for c in range(2):
+        ra, r1 = auto[c].reshape(9, -1).sum(0), one[c].reshape(9, -1).sum(0)
+        assert np.max(np.abs(ra - r1)) < 1e-5
+    case = wl.served_case(**job)
+    for rho0, want in ((served_rho0(job), auto), (1.0, one)):
+        g = nat.Geometry(case["mask"])
+        fl = [nat.Field(g, case["tau"][c], nat.FP32_DEV, rho0, g_solid=case["g_solid"][c],
+                        g_fluid=case["g_fluid"][c]) for c in range(2)]
+        for c in range(2):
+            fl[c].set_f(case["f"][c])
+        nat.solid_surface_force(g, fl[0], fl[1])
+        nat.step2(g, fl[0], fl[1], case["force"][0], case["force"][1], job["steps"])
+        for c in range(2):
+            assert np.array_equal(fl[c].get_f().reshape(-1), want[c].reshape(-1))
