@@ -203,7 +203,9 @@ def c2(rep):
     same = all(np.array_equal(f2000[c], ref[c]) for c in range(2))
     rep.append("- FP64 populations after 2000 steps equal to the oracle's bit for bit: **%s**" % same)
     r.step(18000)
-    rep.append(ref_digest_line("c2_hot", [r.f(c) for c in range(2)], 20000))
+    f64 = [r.f(c) for c in range(2)]
+    rep.append(ref_digest_line("c2_hot", f64, 20000))
+    u64 = [velocity(f64[c], hot["mask"]) for c in range(2)]
     rho64 = [r.rho(c) for c in range(2)]
     r.close()
     R64, dp64 = droplet_stats(rho64[0], rho64[1], 2.2)
@@ -220,11 +222,14 @@ def c2(rep):
     for form, rho0 in (("fp32dev", 0.5), ("fp16dev", 0.5)):
         r = Run(hot, form, rho0=rho0)
         r.step(20000)
+        fx = [r.f(c) for c in range(2)]
         rho = [r.rho(c) for c in range(2)]
         r.close()
         R, dp = droplet_stats(rho[0], rho[1], 2.2)
         err = max(float(np.max(np.abs(rho[c] - rho64[c]))) for c in range(2))
-        rep.append("- %s (rho0 %.1f): radius %.4f, dp %.6g, max |rho - rho_FP64| %.3g" % (form, rho0, R, dp, err))
+        du = max(float(np.max(np.abs(velocity(fx[c], hot["mask"])[k] - u64[c][k]))) for c in range(2) for k in range(2))
+        rep.append("- %s (rho0 %.1f): radius %.4f, dp %.6g, max |rho - rho_FP64| %.3g, max |u - u_FP64| %.3g"
+                   % (form, rho0, R, dp, err, du))
     rep.append("")
 
 
@@ -311,8 +316,13 @@ def c5(rep, n=4096, steps=10000, every=500):
         log("  C5 step %d: L %s; max |d rho| vs FP64 %s (%.0f s)"
             % ((k + 1) * every, " ".join("%.4f" % curves[x][-1] for x in names),
                " ".join("%.2e" % errs[x][-1] for x in names[1:]), time.time() - t))
+    ufin = {name: [velocity(runs[name].f(c), case["mask"]) for c in range(2)] for name in names}
+    du = {name: max(float(np.max(np.abs(ufin[name][c][k] - ufin["fp64"][c][k]))) for c in range(2) for k in range(2))
+          for name in names[1:]}
     for r in runs.values():
         r.close()
+    rep.append("Max |u - u_FP64| over both components after %d steps: " % steps +
+               "; ".join("%s %.3g" % (name, du[name]) for name in names[1:]) + ".\n")
     rep.append("Max |rho - rho_FP64| over both components after %d steps: FP64 from populations perturbed by "
                "1e-12: %.3g; by 6e-8 (the unit roundoff of FP32): %.3g; FP32 deviations: %.3g; FP16 deviations: "
                "%.3g.  Spinodal decomposition amplifies differences in the initial state by about 1e8 in the "
