@@ -33,3 +33,15 @@ def test_errors_are_codes_not_exceptions():
     assert lib.scl_geometry_create(0, 5, None, ctypes.byref(h)) == nat.SCL_ERR_ARG
     assert b"bad geometry size" in lib.scl_last_error()
     assert lib.scl_set_tile(None, 64, 4) == nat.SCL_ERR_ARG
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus N with N > 1 exits with a message
+    (checked before any device is touched)."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    p = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], stdout=subprocess.PIPE,
+                       stderr=subprocess.PIPE, text=True, timeout=120)
+    assert p.returncode == 2
+    assert "one GPU" in p.stdout
