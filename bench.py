@@ -6,12 +6,13 @@ initial condition, rho_A = rho_B = 1 +- 1% seeded noise, tau = 1, G = 1.2),
 synthetic and resident in HBM before the timed region.  The headline value
 is the FP32-deviation form; the FP64 form (bit-exact with the reference) and
 the FP16-deviation storage form are timed beside it.  The deviation forms
-advance two steps per launch (k_step2x) when the autotuner measures that
-faster, FP64 one (k_step).  The roofline object is per launch of the
-dominant kernel: algorithmic bytes (every population of both components
-read once and written once, 144 B per site in FP32) over the HIP-event
-average launch duration; `effective_pct_hbm` is MLUPS x those bytes per
-site over the peak, the share a one-step-per-launch kernel would need.
+advance three (k_stepk) or two (k_step2x) steps per launch, as the
+autotuner measures fastest, FP64 one (k_step).  The roofline object is per
+launch of the dominant kernel: algorithmic bytes (every population of both
+components read once and written once, 144 B per site in FP32) over the
+HIP-event average launch duration; `effective_pct_hbm` is MLUPS x those
+bytes per site over the peak, the share a one-step-per-launch kernel would
+need.
 
 Prints one JSON line (rank 0, N = 1).
 """
@@ -93,9 +94,9 @@ def time_form(nat, wl, case, form, steps, warmup, tile, fusion=""):
     mlups = n_sites * steps / dt / 1e6
     launch_ms = avg_ms * steps / cnt if cnt else 0.0
     # one launch reads and writes every population once, whether it advances
-    # one step (k_step) or two (k_step2x)
+    # one step (k_step), two (k_step2x) or three (k_stepk)
     return dict(form=form, seconds=dt, ms_per_step=dt * 1e3 / steps, mlups=mlups,
-                kernel="k_step2x" if spl == 2 else "k_step", steps_per_launch=spl, band_rows=band,
+                kernel={1: "k_step", 2: "k_step2x", 3: "k_stepk"}[spl], steps_per_launch=spl, band_rows=band,
                 kernel_avg_ms=launch_ms, kernel_launches=cnt, tile=list(tile_used),
                 achieved_gbs=n_sites * BYTES_PER_LU[form] / (launch_ms * 1e-3) / 1e9 if launch_ms > 0 else None,
                 effective_pct_hbm=100.0 * mlups * 1e6 * BYTES_PER_LU[form] / (HBM_PEAK_GBS * 1e9), mass=mass)

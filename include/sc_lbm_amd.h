@@ -167,15 +167,17 @@ int scl_set_tile(scl_geometry* g, int cols, int rows);
 int scl_set_lookahead(scl_geometry* g, int rows_ahead);
 /* Time steps per launch on g (no reference counterpart).  Two-component
  * deviation-form fields on lattices at least 256 columns wide can advance two
- * steps per launch (the intermediate populations stay on chip; results are
- * bit for bit those of single steps).  0 (the default): two per launch unless
- * scl_autotune measured single steps faster; 2: two per launch wherever
- * supported; 1: one per launch.  rows: rows per band of the two-step kernel
- * (0 = autotuned, else the height that makes the grid two whole waves of
- * resident workgroups). */
+ * or three steps per launch (the intermediate populations stay on chip;
+ * results are bit for bit those of single steps).  0 (the default): what
+ * scl_autotune measured fastest, else three per launch with FP32 storage and
+ * two with FP16 storage; 1, 2 or 3: that many per launch wherever supported
+ * (a call's remaining steps go two or one per launch).  rows: rows per band
+ * of the multi-step kernels (0 = autotuned, else the height that makes the
+ * grid two whole waves of resident workgroups). */
 int scl_set_step_fusion(scl_geometry* g, int steps_per_launch, int rows);
-/* What two-component steps of `precision` on g use: steps per launch (1 or
- * 2) and the band rows of the two-step kernel (0 when one per launch). */
+/* What two-component steps of `precision` on g use: steps per launch (1, 2
+ * or 3) and the band rows of the multi-step kernel (0 when one per
+ * launch). */
 int scl_get_step_fusion(const scl_geometry* g, int precision, int* steps_per_launch, int* rows);
 /* The shape the fused kernel uses for `ncomp` components of `precision` on
  * g: the caller's overrides, else the autotuned one, else the default. */
@@ -187,8 +189,10 @@ int scl_get_tile(const scl_geometry* g, int ncomp, int precision, int* cols, int
  * fastest for later steps on g and returns it (outputs may be NULL); for two
  * components in a deviation form it then times the two-step kernel over band
  * heights (16, 32, and those filling 1..4 whole waves of workgroups) and
- * keeps two steps per launch only if faster.  Takes about
- * 3 ms of GPU time per shape; needs memory for two scratch fields. */
+ * keeps two steps per launch only if faster, then the three-step kernel
+ * (bands of 32, 64 and 1..4 whole waves), kept only if faster than both.
+ * Takes about 3 ms of GPU time per shape; needs memory for two scratch
+ * fields. */
 int scl_autotune(scl_geometry* g, int ncomp, int precision, int* cols, int* rows, int* lookahead);
 
 /* Average duration in milliseconds of the fused kernel launches, measured

@@ -27,6 +27,7 @@
 #include "runtime.h"
 #include "step.h"
 #include "step2x.h"
+#include "stepk.h"
 
 using namespace scl;
 
@@ -120,9 +121,10 @@ struct scl_geometry {
     int lookahead = 0;
     int tuned[2][3][3] = {};            // [ncomp-1][precision] -> {cols, rows, lookahead}
     bool open = false;                  // no solid site: the fused kernel skips the mask
-    int fusion = 0;                     // time steps per launch: 0 automatic, 1, or 2 where supported
-    int x2_rows = 0;                    // rows per band of the two-step kernel (0 = automatic)
+    int fusion = 0;                     // time steps per launch: 0 automatic, or 1..3 where supported
+    int x2_rows = 0;                    // rows per band of the multi-step kernels (0 = automatic)
     int x2_tuned[3] = {};               // [precision]: autotuned band rows; -1: single steps were faster
+    int x3_tuned[3] = {};               // [precision]: the same for three steps per launch; -1: slower
     double* part = nullptr;             // moment partials (3 per row + 3)
     // Fields keep their geometry alive: destroying a geometry that fields
     // still use only marks it, and the last field destroyed frees it.
@@ -152,10 +154,12 @@ struct scl_field {
     // P).  Held by reference.
     scl_geometry* pg = nullptr;
     // Deviation forms: equilibrium velocities of the zero-density rule in
-    // three rotating buffers (x, y); a step reads buffer ucur and writes the
-    // next, so steps fused into one launch never read what the same launch
-    // writes.  ueqx / ueqy point at buffer ucur.
-    double* ueq3[3][2] = {};
+    // kUeqRing rotating buffer sets (x, y) of one block (UeqRing's layout);
+    // a step reads set ucur and writes the next, so steps fused into one
+    // launch (up to kUeqRing - 1 of them) never read what the same launch
+    // writes.  ueqx / ueqy point at set ucur.
+    double* ueq_block = nullptr;
+    double* ueqs[kUeqRing][2] = {};
     int ucur = 0;
     // reference per-site arrays (FP64, pitched)
     double *rho = nullptr, *ux = nullptr, *uy = nullptr, *ueqx = nullptr, *ueqy = nullptr;
@@ -188,9 +192,9 @@ void set_pgeom(scl_field* f, scl_geometry* g) {
 // The deviation forms' equilibrium-velocity buffers after `steps` steps.
 void advance_ueq(scl_field* f, int steps) {
     if (f->precision == SCL_FP64) return;
-    f->ucur = (f->ucur + steps) % 3;
-    f->ueqx = f->ueq3[f->ucur][0];
-    f->ueqy = f->ueq3[f->ucur][1];
+    f->ucur = (f->ucur + steps) % kUeqRing;
+    f->ueqx = f->ueqs[f->ucur][0];
+    f->ueqy = f->ueqs[f->ucur][1];
 }
 
 size_t n_sites(const scl_geometry* g) { return (size_t)g->pitch * g->ny; }
@@ -315,11 +319,11 @@ void fill_comp(CompArgs<T>& c, const scl_field* f, int src_buf, const double* fo
         c.ueqx_in = f->ueqx;
         c.ueqy_in = f->ueqy;
     } else {
-        const int nxt = (f->ucur + 1) % 3;
-        c.ueqx_in = f->ueq3[f->ucur][0];
-        c.ueqy_in = f->ueq3[f->ucur][1];
-        c.ueqx = f->ueq3[nxt][0];
-        c.ueqy = f->ueq3[nxt][1];
+        const int nxt = (f->ucur + 1) % kUeqRing;
+        c.ueqx_in = f->ueqs[f->ucur][0];
+        c.ueqy_in = f->ueqs[f->ucur][1];
+        c.ueqx = f->ueqs[nxt][0];
+        c.ueqy = f->ueqs[nxt][1];
     }
     c.rho = f->rho;
     c.jx = f->ux;
@@ -407,7 +411,7 @@ bool x2_supported(const scl_geometry* g, int precision) {
 // whether two-component steps of `precision` on g go two per launch
 bool x2_wanted(const scl_geometry* g, int precision) {
     if (g->fusion == 1 || !x2_supported(g, precision)) return false;
-    return g->fusion == 2 || g->x2_tuned[precision] >= 0;
+    return g->fusion >= 2 || g->x2_tuned[precision] >= 0;
 }
 
 // Workgroups of the two-step kernel resident per CU (registers and LDS).
@@ -415,10 +419,10 @@ int x2_occupancy(int precision, bool open) {
     int& o = g_lib.x2_occ[precision][open ? 1 : 0];
     if (!o) {
         const void* k = precision == SCL_FP32_DEV
-                            ? (open ? reinterpret_cast<const void*>(&k_step2x<StoreF32, true>)
-                                    : reinterpret_cast<const void*>(&k_step2x<StoreF32, false>))
-                            : (open ? reinterpret_cast<const void*>(&k_step2x<StoreF16, true>)
-                                    : reinterpret_cast<const void*>(&k_step2x<StoreF16, false>));
+                            ? (open ? reinterpret_cast<const void*>(&k_step2x<StoreF32, true, true>)
+                                    : reinterpret_cast<const void*>(&k_step2x<StoreF32, false, true>))
+                            : (open ? reinterpret_cast<const void*>(&k_step2x<StoreF16, true, true>)
+                                    : reinterpret_cast<const void*>(&k_step2x<StoreF16, false, true>));
         int n = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, kX2Threads, 0) != hipSuccess || n < 1) n = 2;
         o = n;
@@ -456,11 +460,11 @@ int launch_x2(const scl_geometry* g, scl_field* const* fl, const double* const* 
     StepArgs<float> a2 = a1;
     for (int c = 0; c < 2; ++c) {
         const scl_field* f = fl[c];
-        const int k1 = (f->ucur + 1) % 3, k2 = (f->ucur + 2) % 3;
-        a2.c[c].ueqx_in = f->ueq3[k1][0];
-        a2.c[c].ueqy_in = f->ueq3[k1][1];
-        a2.c[c].ueqx = f->ueq3[k2][0];
-        a2.c[c].ueqy = f->ueq3[k2][1];
+        const int k1 = (f->ucur + 1) % kUeqRing, k2 = (f->ucur + 2) % kUeqRing;
+        a2.c[c].ueqx_in = f->ueqs[k1][0];
+        a2.c[c].ueqy_in = f->ueqs[k1][1];
+        a2.c[c].ueqx = f->ueqs[k2][0];
+        a2.c[c].ueqy = f->ueqs[k2][1];
     }
     const int nblk = a1.gx * ((g->ny + rows - 1) / rows);
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -469,10 +473,16 @@ int launch_x2(const scl_geometry* g, scl_field* const* fl, const double* const* 
         HIPCHK(hipEventCreate(&e1));
         HIPCHK(hipEventRecord(e0, g_lib.stream));
     }
-    if (g->open)
-        hipLaunchKernelGGL((k_step2x<ST, true>), dim3(nblk), dim3(kX2Threads), 0, g_lib.stream, a1, a2);
-    else
-        hipLaunchKernelGGL((k_step2x<ST, false>), dim3(nblk), dim3(kX2Threads), 0, g_lib.stream, a1, a2);
+    // kernels without the force terms where neither component has a force
+    const bool force = forces[0] || forces[1];
+    const dim3 grid(nblk), block(kX2Threads);
+    if (g->open) {
+        if (force) hipLaunchKernelGGL((k_step2x<ST, true, true>), grid, block, 0, g_lib.stream, a1, a2);
+        else hipLaunchKernelGGL((k_step2x<ST, true, false>), grid, block, 0, g_lib.stream, a1, a2);
+    } else {
+        if (force) hipLaunchKernelGGL((k_step2x<ST, false, true>), grid, block, 0, g_lib.stream, a1, a2);
+        else hipLaunchKernelGGL((k_step2x<ST, false, false>), grid, block, 0, g_lib.stream, a1, a2);
+    }
     HIPCHK(hipGetLastError());
     if (e0) {
         HIPCHK(hipEventRecord(e1, g_lib.stream));
@@ -480,6 +490,96 @@ int launch_x2(const scl_geometry* g, scl_field* const* fl, const double* const* 
         g_lib.ev_steps.push_back(2);
     }
     return SCL_OK;
+}
+
+// Three fused steps (k_stepk with K = 3): P_{n+2} from P_{n-1} in one
+// launch.  Automatic choice (fusion 0): what scl_autotune measured fastest,
+// else three per launch with FP32 storage (72k against 61k MLUPS for two on
+// a 4096^2 lattice) and two with FP16 storage (74k against 82k).
+bool x3_wanted(const scl_geometry* g, int precision) {
+    if (!x2_supported(g, precision) || g->fusion == 1 || g->fusion == 2) return false;
+    if (g->fusion == 3) return true;
+    if (g->x3_tuned[precision]) return g->x3_tuned[precision] > 0;
+    return precision == SCL_FP32_DEV;
+}
+
+template <class ST, bool OPEN>
+int x3_occupancy() {
+    static int o = 0;
+    if (!o) {
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &n, reinterpret_cast<const void*>(&k_stepk<ST, OPEN, 3, x3_threads<ST, OPEN>(), true>),
+                x3_threads<ST, OPEN>(), 0) != hipSuccess ||
+            n < 1)
+            n = 1;
+        o = n;
+    }
+    return o;
+}
+
+// Band height that makes the three-step grid `waves` whole waves of
+// resident workgroups (78 rows at 4096^2 with FP32 storage for two waves).
+template <class ST, bool OPEN>
+int x3_wave_rows_t(const scl_geometry* g, int waves) {
+    constexpr int out = xk_out<ST, 3, x3_threads<ST, OPEN>()>();
+    const int strips = (g->nx + out - 1) / out;
+    const int bands = std::max(1, waves * x3_occupancy<ST, OPEN>() * std::max(g_lib.cus, 1) / strips);
+    return std::max(8, std::min((g->ny + bands - 1) / bands, g->ny));
+}
+
+int x3_wave_rows(const scl_geometry* g, int precision, int waves) {
+    if (precision == SCL_FP32_DEV)
+        return g->open ? x3_wave_rows_t<StoreF32, true>(g, waves) : x3_wave_rows_t<StoreF32, false>(g, waves);
+    return g->open ? x3_wave_rows_t<StoreF16, true>(g, waves) : x3_wave_rows_t<StoreF16, false>(g, waves);
+}
+
+// rows per band: the caller's, else the autotuned, else two whole waves
+int x3_rows_for(const scl_geometry* g, int precision) {
+    const int r = g->x2_rows ? g->x2_rows
+                             : (g->x3_tuned[precision] > 0 ? g->x3_tuned[precision] : x3_wave_rows(g, precision, 2));
+    return std::min(r, g->ny);
+}
+
+template <class ST, bool OPEN>
+int launch_x3_t(const scl_geometry* g, scl_field* const* fl, const double* const* forces) {
+    constexpr int T = x3_threads<ST, OPEN>();
+    constexpr int OUT = xk_out<ST, 3, T>();
+    const int rows = x3_rows_for(g, fl[0]->precision);
+    XkArgs<3> A{};
+    A.a = make_args<float>(g, rows, 4);
+    A.a.gx = (g->nx + OUT - 1) / OUT;
+    for (int c = 0; c < 2; ++c) {
+        const scl_field* f = fl[c];
+        fill_comp(A.a.c[c], f, f->cur, forces[c]);
+        A.dst[c] = f->f[f->cur ^ 1];
+        A.u.base[c] = f->ueq_block;
+        A.u.cur[c] = f->ucur;
+    }
+    A.u.n = (long long)n_sites(g);
+    const int nblk = A.a.gx * ((g->ny + rows - 1) / rows);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (g_lib.timing && g_lib.ev.size() < kMaxTimed) {
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, g_lib.stream));
+    }
+    if (forces[0] || forces[1])
+        hipLaunchKernelGGL((k_stepk<ST, OPEN, 3, T, true>), dim3(nblk), dim3(T), 0, g_lib.stream, A);
+    else
+        hipLaunchKernelGGL((k_stepk<ST, OPEN, 3, T, false>), dim3(nblk), dim3(T), 0, g_lib.stream, A);
+    HIPCHK(hipGetLastError());
+    if (e0) {
+        HIPCHK(hipEventRecord(e1, g_lib.stream));
+        g_lib.ev.emplace_back(e0, e1);
+        g_lib.ev_steps.push_back(3);
+    }
+    return SCL_OK;
+}
+
+template <class ST>
+int launch_x3(const scl_geometry* g, scl_field* const* fl, const double* const* forces) {
+    return g->open ? launch_x3_t<ST, true>(g, fl, forces) : launch_x3_t<ST, false>(g, fl, forces);
 }
 
 template <int NC, bool GATHER>
@@ -613,6 +713,17 @@ int fused_steps(const scl_geometry* g, scl_field** fl, const double* const* forc
     }
     for (int t = 0; t < nsteps;) {
         if constexpr (NC == 2) {
+            if (nsteps - t >= 3 && x2_ok(g, fl, NC) && x3_wanted(g, prec)) {
+                const int rc = prec == SCL_FP32_DEV ? launch_x3<StoreF32>(g, fl, forces)
+                                                    : launch_x3<StoreF16>(g, fl, forces);
+                if (rc != SCL_OK) return rc;
+                for (int c = 0; c < NC; ++c) {
+                    fl[c]->cur ^= 1;
+                    advance_ueq(fl[c], 3);
+                }
+                t += 3;
+                continue;
+            }
             if (nsteps - t >= 2 && x2_ok(g, fl, NC)) {
                 const int rc = prec == SCL_FP32_DEV ? launch_x2<StoreF32>(g, fl, forces)
                                                     : launch_x2<StoreF16>(g, fl, forces);
@@ -747,8 +858,8 @@ int scl_set_lookahead(scl_geometry* g, int rows_ahead) {
 
 int scl_set_step_fusion(scl_geometry* g, int steps_per_launch, int rows) {
     if (!g) return fail(SCL_ERR_ARG, "null geometry");
-    if (steps_per_launch < 0 || steps_per_launch > 2)
-        return fail(SCL_ERR_ARG, "steps per launch must be 0 (automatic), 1 or 2");
+    if (steps_per_launch < 0 || steps_per_launch > 3)
+        return fail(SCL_ERR_ARG, "steps per launch must be 0 (automatic), 1, 2 or 3");
     if (rows < 0 || rows > 4096) return fail(SCL_ERR_ARG, "rows must be in 0..4096");
     g->fusion = steps_per_launch;
     g->x2_rows = rows;
@@ -758,6 +869,11 @@ int scl_set_step_fusion(scl_geometry* g, int steps_per_launch, int rows) {
 int scl_get_step_fusion(const scl_geometry* g, int precision, int* steps_per_launch, int* rows) {
     if (!g || !steps_per_launch || !rows) return fail(SCL_ERR_ARG, "null argument");
     if (precision < SCL_FP64 || precision > SCL_FP16_DEV) return fail(SCL_ERR_ARG, "bad precision");
+    if (x3_wanted(g, precision)) {
+        *steps_per_launch = 3;
+        *rows = x3_rows_for(g, precision);
+        return SCL_OK;
+    }
     const bool two = x2_wanted(g, precision);
     *steps_per_launch = two ? 2 : 1;
     *rows = two ? x2_rows_for(g, precision) : 0;
@@ -817,13 +933,12 @@ int scl_field_create(const scl_geometry* g, int precision, double tau, double rh
         HIPCHK(hipMemsetAsync(f->ueqx, 0, n * sizeof(double), g_lib.stream));
         HIPCHK(hipMemsetAsync(f->ueqy, 0, n * sizeof(double), g_lib.stream));
     } else {
-        for (int k = 0; k < 3; ++k)
-            for (int d = 0; d < 2; ++d) {
-                CHK(dalloc(&f->ueq3[k][d], n));
-                HIPCHK(hipMemsetAsync(f->ueq3[k][d], 0, n * sizeof(double), g_lib.stream));
-            }
-        f->ueqx = f->ueq3[0][0];
-        f->ueqy = f->ueq3[0][1];
+        CHK(dalloc(&f->ueq_block, 2 * kUeqRing * n));
+        HIPCHK(hipMemsetAsync(f->ueq_block, 0, 2 * kUeqRing * n * sizeof(double), g_lib.stream));
+        for (int k = 0; k < kUeqRing; ++k)
+            for (int d = 0; d < 2; ++d) f->ueqs[k][d] = f->ueq_block + (size_t)(2 * k + d) * n;
+        f->ueqx = f->ueqs[0][0];
+        f->ueqy = f->ueqs[0][1];
     }
     f->g->refs++;
     *out = f.release();
@@ -848,8 +963,7 @@ int scl_field_destroy(scl_field* f) {
         dfree(f->ueqx);
         dfree(f->ueqy);
     } else {
-        for (auto& k : f->ueq3)
-            for (double* p : k) dfree(p);
+        dfree(f->ueq_block);
     }
     for (double* p : {f->rho, f->ux, f->uy, f->frx, f->fry, f->fsx, f->fsy, f->feq, f->stage}) dfree(p);
     scl_geometry* g = f->g;
@@ -1325,6 +1439,41 @@ int scl_autotune(scl_geometry* g, int ncomp, int precision, int* cols, int* rows
             }
         }
         if (rc == SCL_OK) g->x2_tuned[precision] = best2 < best ? b2 : -1;
+        // three steps per launch, kept only if faster than both
+        g->fusion = 3;
+        double best3 = 1e300;
+        int b3 = 0;
+        int bandset3[6] = {32, 64, x3_wave_rows(g, precision, 1), x3_wave_rows(g, precision, 2),
+                           x3_wave_rows(g, precision, 3), x3_wave_rows(g, precision, 4)};
+        for (int bi = 0; bi < 6 && rc == SCL_OK; ++bi) {
+            bool seen = bandset3[bi] > g->ny && bi > 0;
+            for (int bj = 0; bj < bi; ++bj) seen = seen || bandset3[bj] == bandset3[bi];
+            if (seen) continue;
+            g->x2_rows = bandset3[bi];
+            const double* none[2] = {nullptr, nullptr};
+            rc = fused_steps<2>(g, fl, none, 3);
+            int reps = 6;
+            float ms = 0.f;
+            for (int pass = 0; pass < 2 && rc == SCL_OK; ++pass) {
+                if (hipEventRecord(e0, g_lib.stream) != hipSuccess) rc = fail(SCL_ERR_HIP, "event record failed");
+                if (rc == SCL_OK) rc = fused_steps<2>(g, fl, none, reps);
+                if (rc == SCL_OK && (hipEventRecord(e1, g_lib.stream) != hipSuccess ||
+                                     hipEventSynchronize(e1) != hipSuccess ||
+                                     hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                    rc = fail(SCL_ERR_HIP, "event timing failed");
+                if (pass == 0 && rc == SCL_OK) {
+                    const double per = ms / reps;
+                    int want = per > 0 ? (int)(3.0 / per) : 48;
+                    want = want < 6 ? 6 : (want > 48 ? 48 : want);
+                    reps = want / 3 * 3;
+                }
+            }
+            if (rc == SCL_OK && ms / reps < best3) {
+                best3 = ms / reps;
+                b3 = bandset3[bi];
+            }
+        }
+        if (rc == SCL_OK) g->x3_tuned[precision] = best3 < std::min(best, best2) ? b3 : -1;
     }
     g->fusion = saved_fusion;
     g->x2_rows = saved_x2;

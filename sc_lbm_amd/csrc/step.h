@@ -132,6 +132,14 @@ __device__ inline T dens(const T (&p)[Q]) {
     for (int i = 0; i < Q; ++i) r += p[i];
     return r;
 }
+// FP32 (deviation forms): the same sum without the leading 0 + (which only
+// turns a -0 sum into +0 and costs an instruction per site).
+__device__ inline float dens(const float (&p)[Q]) {
+    float r = p[0];
+#pragma unroll
+    for (int i = 1; i < Q; ++i) r += p[i];
+    return r;
+}
 
 // Momentum with the reference's Cx / Cy sums; the zero terms (f*0) are
 // dropped, which leaves every nonzero result unchanged.
@@ -139,6 +147,10 @@ template <class T>
 __device__ inline void momentum(const T (&p)[Q], T& jx, T& jy) {
     jx = ((((((T(0) + p[1]) - p[3]) + p[5]) - p[6]) - p[7]) + p[8]);
     jy = ((((((T(0) + p[2]) - p[4]) + p[5]) + p[6]) - p[7]) - p[8]);
+}
+__device__ inline void momentum(const float (&p)[Q], float& jx, float& jy) {
+    jx = ((((p[1] - p[3]) + p[5]) - p[6]) - p[7]) + p[8];
+    jy = ((((p[2] - p[4]) + p[5]) + p[6]) - p[7]) - p[8];
 }
 
 // Reference feq (src/fluid.cpp:254-276), evaluated as written.
@@ -220,6 +232,16 @@ __device__ inline void relax_dev(const float (&p)[Q], float rho, float drho, flo
 __device__ inline float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
 __device__ inline bool is_dry(double r) { return fabs(r) <= kZeroTol * fmax(1.0, fabs(r)); }
+// The same test for an FP32 density, in FP32: for finite r it holds exactly
+// when |r| <= kZeroTol (the factor max(1, |r|) only matters above 1, where
+// the test fails), and for infinite r.  kZeroTolF is the largest float not
+// above kZeroTol.
+constexpr float kZeroTolF = 0x1.cd2b28p-54f;
+static_assert((double)kZeroTolF <= kZeroTol && (double)0x1.cd2b2ap-54f > kZeroTol, "kZeroTolF");
+__device__ inline bool is_dry(float r) {
+    const float m = fabsf(r);
+    return m <= kZeroTolF || m == __builtin_inff();
+}
 
 // Bounce-back mode of the gathering kernels: 0 selects the pull source per
 // direction from the mask bits; 1 / 2 read every pull from its source slot
@@ -325,6 +347,10 @@ struct UeqPtrs {
     const double* in_y[2];
     double* out_x[2];
     double* out_y[2];
+    __device__ const double* ix(int c) const { return in_x[c]; }
+    __device__ const double* iy(int c) const { return in_y[c]; }
+    __device__ double* ox(int c) const { return out_x[c]; }
+    __device__ double* oy(int c) const { return out_y[c]; }
 };
 
 template <class T>
@@ -334,10 +360,14 @@ __device__ inline UeqPtrs ueq_ptrs(const StepArgs<T>& a) {
 }
 
 // The collision with the physics parameters of `a` and the equilibrium-
-// velocity buffers of `u` (the two-step kernel uses one parameter set for
-// both of its steps).
-template <class T>
-__device__ inline void collide2_core(const StepArgs<T>& a, const UeqPtrs& u, int s_idx, const float (&p)[2][Q],
+// velocity buffers of `u` (the multi-step kernels use one parameter set for
+// all of their steps).  U: UeqPtrs, or any type with the same accessors
+// ix / iy (read, component c) and ox / oy (written).
+// FORCE = false: no body force on either component (has_force is then not
+// read, so kernels built for force-free lattices carry neither the force
+// values nor the additions).
+template <class T, class U, bool FORCE = true>
+__device__ inline void collide2_core(const StepArgs<T>& a, const U& u, int s_idx, const float (&p)[2][Q],
                                      const float (&rc)[2], const float (&nr)[2][Q], unsigned nb, float (&oa)[Q],
                                      float (&ob)[Q]);
 
@@ -347,8 +377,8 @@ __device__ inline void collide2_dev(const StepArgs<T>& a, int s_idx, const float
     collide2_core(a, ueq_ptrs(a), s_idx, p, rc, nr, nb, oa, ob);
 }
 
-template <class T>
-__device__ inline void collide2_core(const StepArgs<T>& a, const UeqPtrs& u, int s_idx, const float (&p)[2][Q],
+template <class T, class U, bool FORCE>
+__device__ inline void collide2_core(const StepArgs<T>& a, const U& u, int s_idx, const float (&p)[2][Q],
                                      const float (&rc)[2], const float (&nr)[2][Q], unsigned nb, float (&oa)[Q],
                                      float (&ob)[Q]) {
     // fused multiply-adds written out, as in relax_dev
@@ -404,33 +434,33 @@ __device__ inline void collide2_core(const StepArgs<T>& a, const UeqPtrs& u, int
         uea[0] = fmaf(A.inv_omega, fmaf(fa_x, ir, fs_ax), ucx);
         uea[1] = fmaf(A.inv_omega, fmaf(fa_y, ir, fs_ay), ucy);
         if (fabsf(ra) < (float)kUeqWatch) {
-            G(u.out_x[0])[s_idx] = uea[0];
-            G(u.out_y[0])[s_idx] = uea[1];
+            G(u.ox(0))[s_idx] = uea[0];
+            G(u.oy(0))[s_idx] = uea[1];
         }
     } else {
-        const double ux0 = G(u.in_x[0])[s_idx], uy0 = G(u.in_y[0])[s_idx];
+        const double ux0 = G(u.ix(0))[s_idx], uy0 = G(u.iy(0))[s_idx];
         uea[0] = (float)ux0;
         uea[1] = (float)uy0;
-        G(u.out_x[0])[s_idx] = ux0;
-        G(u.out_y[0])[s_idx] = uy0;
+        G(u.ox(0))[s_idx] = ux0;
+        G(u.oy(0))[s_idx] = uy0;
     }
     if (!is_dry(rb)) {
         const float ir = rcp(rb);
         ueb[0] = fmaf(B.inv_omega, fmaf(fb_x, ir, fs_bx), ucx);
         ueb[1] = fmaf(B.inv_omega, fmaf(fb_y, ir, fs_by), ucy);
         if (fabsf(rb) < (float)kUeqWatch) {
-            G(u.out_x[1])[s_idx] = ueb[0];
-            G(u.out_y[1])[s_idx] = ueb[1];
+            G(u.ox(1))[s_idx] = ueb[0];
+            G(u.oy(1))[s_idx] = ueb[1];
         }
     } else {
-        const double ux0 = G(u.in_x[1])[s_idx], uy0 = G(u.in_y[1])[s_idx];
+        const double ux0 = G(u.ix(1))[s_idx], uy0 = G(u.iy(1))[s_idx];
         ueb[0] = (float)ux0;
         ueb[1] = (float)uy0;
-        G(u.out_x[1])[s_idx] = ux0;
-        G(u.out_y[1])[s_idx] = uy0;
+        G(u.ox(1))[s_idx] = ux0;
+        G(u.oy(1))[s_idx] = uy0;
     }
-    relax_dev(p[0], ra, da0, uea[0], uea[1], A.omega, A.force, A.has_force, oa);
-    relax_dev(p[1], rb, db0, ueb[0], ueb[1], B.omega, B.force, B.has_force, ob);
+    relax_dev(p[0], ra, da0, uea[0], uea[1], A.omega, A.force, FORCE && A.has_force, oa);
+    relax_dev(p[1], rb, db0, ueb[0], ueb[1], B.omega, B.force, FORCE && B.has_force, ob);
 }
 
 template <class ST, bool REPLAY, class OffT, bool GW>

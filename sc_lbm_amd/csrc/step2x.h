@@ -80,7 +80,7 @@ struct X2Lds {
 // n+1 (dst = P_{n+1}, u_eq buffers of step n+1); geometry from a1.
 // Requires nx >= kX2Threads; the caller separates consecutive tiles of a
 // workgroup with a barrier (L is reused).
-template <class ST, bool OPEN>
+template <class ST, bool OPEN, bool FORCE = true>
 __device__ inline void step2x_tile(const StepArgs<float>& a1, const StepArgs<float>& a2, const int bx, const int y0,
                                    const int y1, X2Lds<ST, OPEN>& L) {
     using S = typename ST::S;
@@ -225,7 +225,7 @@ __device__ inline void step2x_tile(const StepArgs<float>& a1, const StepArgs<flo
         if (s1_lane && (OPEN || (nb_0 & 1u))) {
             float nr[2][Q], rc[2], oa[Q], ob[Q];
             nbr_dens(L.d1, r, nr, rc);
-            collide2_core(a1, u1, wr(r) * a1.pitch + X, pa, rc, nr, OPEN ? 0x1FFu : nb_0, oa, ob);
+            collide2_core<float, UeqPtrs, FORCE>(a1, u1, wr(r) * a1.pitch + X, pa, rc, nr, OPEN ? 0x1FFu : nb_0, oa, ob);
 #pragma unroll
             for (int i = 1; i < Q; ++i) {
                 pr(r, 0, i, t) = ST::enc(oa[i]);
@@ -266,12 +266,12 @@ __device__ inline void step2x_tile(const StepArgs<float>& a1, const StepArgs<flo
             float nr[2][Q], rc[2], oa[Q], ob[Q];
             nbr_dens(L.d2, h, nr, rc);
             // step n+1: the parameters of a1 (the same), the buffers of a2
-            collide2_core(a1, u2, h * a1.pitch + X, s2p, rc, nr, OPEN ? 0x1FFu : nb_m2, oa, ob);
+            collide2_core<float, UeqPtrs, FORCE>(a1, u2, h * a1.pitch + X, s2p, rc, nr, OPEN ? 0x1FFu : nb_m2, oa, ob);
             const OffT home = (OffT)h * pitch_b + (OffT)X * ES;
 #pragma unroll
             for (int i = 0; i < Q; ++i) {
-                ST::stc((S*)((char*)a2.c[0].dst + (OffT)i * plane_b + home), oa[i]);
-                ST::stc((S*)((char*)a2.c[1].dst + (OffT)i * plane_b + home), ob[i]);
+                ST::stc((S*)((char*)a2.c[0].dst + (size_t)i * plane_b + home), oa[i]);
+                ST::stc((S*)((char*)a2.c[1].dst + (size_t)i * plane_b + home), ob[i]);
             }
         }
         // rotate (the S_n populations rotate by name)
@@ -301,7 +301,7 @@ __device__ inline void step2x_tile(const StepArgs<float>& a1, const StepArgs<flo
 // The lattice in one launch: strips of x2_out<ST>() columns (a1.gx of them)
 // times bands of a1.rows rows; workgroup ids remapped so that each XCD owns
 // a contiguous range of tiles, as in k_step.
-template <class ST, bool OPEN>
+template <class ST, bool OPEN, bool FORCE>
 __global__ __launch_bounds__(kX2Threads) void k_step2x(const StepArgs<float> a1, const StepArgs<float> a2) {
     __shared__ X2Lds<ST, OPEN> L;
     const int nbx = a1.gx;
@@ -316,7 +316,7 @@ __global__ __launch_bounds__(kX2Threads) void k_step2x(const StepArgs<float> a1,
     }
     const int bx = bid % nbx, by = bid / nbx;
     const int y0 = by * a1.rows;
-    step2x_tile<ST, OPEN>(a1, a2, bx, y0, min(y0 + a1.rows, a1.ny), L);
+    step2x_tile<ST, OPEN, FORCE>(a1, a2, bx, y0, min(y0 + a1.rows, a1.ny), L);
 }
 
 }  // namespace scl

@@ -1,0 +1,352 @@
+// stepk.h -- K fused time steps per launch for the two-component deviation
+// forms (FP32 arithmetic; FP32 or FP16 storage), the generalisation of
+// step2x.h to K stages.
+//
+// A launch reads P_{n-1} and writes P_{n+K-1}; the post-collision
+// populations of the K-1 intermediate steps stay in LDS, so a population
+// crosses HBM once per K steps.  Every stage computes a site exactly as a
+// single step does (collide2_core, the same gather and bounce-back, each
+// intermediate rounded to the storage form before it is gathered again), so
+// the result is bit for bit that of K single steps.
+//
+// Work decomposition.  A workgroup of T threads owns one column per thread
+// and walks a band of rows from south to north.  Stage s (1..K) collides row
+// r - 2(s-1) while stage 1 collides row r; stage s >= 2 first gathers row
+// r - 2s + 3 from the post-collision ring of stage s-1.  Stage s is valid on
+// columns 2s-1 .. T-2s of the strip, so a strip writes OUT columns from
+// column H = 2K-1 on (OUT: a multiple of 32 with FP32 storage, so no
+// 128-byte line is written by two strips) and bands recompute 2(K-1) rows
+// of the first stage on each side.  Per row: (a) file the densities of
+// S_n(r + 1), loaded one row earlier, and issue the loads of row r + 2;
+// then for each stage its gather and its collision, 2K - 1 barriers in all.
+// The barriers wait only for LDS, so the loads stay in flight.
+//
+// LDS per stage: densities in rings of 3 rows; post-collision populations
+// by direction class (the next stage's gather at row g reads direction i
+// from row g - c_iy): c_iy = +1 in 3 rows, c_iy = 0 in 2, c_iy = -1 in 1 (2
+// with solids, whose bounce-back reads the own row in every direction); the
+// rest population stays in a register.
+//
+// Zero-density rule: stage s reads the equilibrium velocities of the step
+// before from one buffer set and writes the next, which stage s + 1 reads;
+// the K + 1 sets of a launch are distinct (K < kUeqRing), so no buffer is
+// read by one workgroup while another writes it (columns computed by two
+// strips write identical values).
+#pragma once
+#include "step2x.h"
+
+namespace scl {
+
+template <class ST, int K, int T>
+__host__ __device__ constexpr int xk_out() {
+    return sizeof(typename ST::S) == 4 ? (T - 2 * (2 * K - 1)) / 32 * 32 : T - 2 * (2 * K - 1);
+}
+
+// Threads per strip of the three-step kernel: the widest strip (at most
+// 256) whose LDS lets two workgroups share a CU (160 KiB).  LDS per column:
+// FP32 storage 328 B solid-free, 376 B with solids; FP16 storage 200 / 224 B.
+template <class ST, bool OPEN>
+__host__ __device__ constexpr int x3_threads() {
+    return sizeof(typename ST::S) == 2 ? 256 : (OPEN ? 240 : 208);
+}
+
+template <class ST, bool OPEN, int K, int T>
+struct XkLds {
+    float d[K][3][2][T];                                 // densities of stage s
+    typename ST::S pu[K - 1][3][2][3][T];                // post-collision, c_y = +1 (2, 5, 6)
+    typename ST::S pz[K - 1][2][2][2][T];                // c_y = 0 (1, 3)
+    typename ST::S pd[K - 1][OPEN ? 1 : 2][2][3][T];     // c_y = -1 (4, 7, 8)
+};
+
+// Equilibrium-velocity buffers of a deviation-form field: one block of
+// kUeqRing sets of (x, y) buffers of n doubles each; set k, direction d at
+// base + (2k + d) n.  A step reads set cur and writes set cur + 1 (mod
+// kUeqRing).  Eight scalar registers describe every set of both components,
+// where separate pointers for K stages would take 16 K and spill.
+constexpr int kUeqRing = 4;
+struct UeqRing {
+    double* base[2];
+    long long n;
+    int cur[2];
+};
+// The buffers stage s (0-based) of a launch reads and writes.
+struct UeqStage {
+    const UeqRing& r;
+    int s;
+    __device__ double* buf(int c, int next, int d) const {
+        const int k = (r.cur[c] + s + next) & (kUeqRing - 1);
+        return r.base[c] + (long long)(2 * k + d) * r.n;
+    }
+    __device__ const double* ix(int c) const { return buf(c, 0, 0); }
+    __device__ const double* iy(int c) const { return buf(c, 0, 1); }
+    __device__ double* ox(int c) const { return buf(c, 1, 0); }
+    __device__ double* oy(int c) const { return buf(c, 1, 1); }
+};
+
+// Arguments of one K-step launch: physics, geometry and P_{n-1} (a.c[c].src)
+// from `a` (its equilibrium-velocity pointers are unused); P_{n+K-1} into
+// dst; the equilibrium-velocity sets in u.
+template <int K>
+struct XkArgs {
+    StepArgs<float> a;
+    void* dst[2];
+    UeqRing u;
+};
+
+template <class ST, bool OPEN, int K, int T, bool FORCE>
+__device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0, const int y1,
+                                  XkLds<ST, OPEN, K, T>& L) {
+    using S = typename ST::S;
+    using OffT = uint32_t;
+    constexpr OffT ES = sizeof(S);
+    constexpr int H = 2 * K - 1;            // halo columns on each side
+    constexpr int OUT = xk_out<ST, K, T>();
+    const StepArgs<float>& a = A.a;
+    const int nx = a.nx, ny = a.ny, wpr = a.wpr;
+    const OffT plane_b = (OffT)a.plane * ES, pitch_b = (OffT)a.pitch * ES;
+    const uint64_t* __restrict__ mask = a.mask;
+    auto tri = [&](int row, int cm, int c, int cp) -> unsigned {
+        if constexpr (OPEN) return 7u;
+        else return triple(mask, wpr, row, cm, c, cp);
+    };
+    auto neigh = [&](unsigned s, unsigned c, unsigned n) -> unsigned {
+        if constexpr (OPEN) return 0x1FFu;
+        else return neigh_bits(s, c, n);
+    };
+
+    const int t = threadIdx.x;
+    const int x0 = bx * OUT;
+    int X = x0 - H + t;   // one wrap at most (nx >= T)
+    X = X < 0 ? X + nx : (X >= nx ? X - nx : X);
+    const int Xm = wrap1(X - 1, nx), Xp = wrap1(X + 1, nx);
+    const int nout = min(OUT, nx - x0);
+    const bool out_lane = t >= H && t < H + nout;
+    auto wr = [&](int r) {
+        r %= ny;
+        return r < 0 ? r + ny : r;
+    };
+    auto r3 = [&](int r) { return (r - y0 + 3 * (K + 2)) % 3; };   // r >= y0 - 3(K+2)
+    auto r2 = [&](int r) { return (r - y0 + 2 * (K + 2)) & 1; };
+    // LDS slot of direction i (1..8) of row `row`, column `col`, ring of stage q (0-based)
+    auto pr = [&](int q, int row, int c, int i, int col) -> S& {
+        if (x2_class(i) == 0) return L.pu[q][r3(row)][c][x2_index(i)][col];
+        if (x2_class(i) == 1) return L.pz[q][r2(row)][c][x2_index(i)][col];
+        return L.pd[q][OPEN ? 0 : r2(row)][c][x2_index(i)][col];
+    };
+    auto nbits = [&](int r) -> unsigned {
+        return neigh(tri(wr(r - 1), Xm, X, Xp), tri(wr(r), Xm, X, Xp), tri(wr(r + 1), Xm, X, Xp));
+    };
+    // S_n(X, r) = Gather(P_{n-1}), zero at solid sites (a solid site loads its
+    // own cell and keeps zeros: no branch round the loads)
+    auto gather1 = [&](int r, unsigned nb, float (&p)[2][Q]) {
+        OffT off[Q];
+        pull_offsets<ST, true, OffT>(plane_b, pitch_b, X, Xm, Xp, wr(r), wr(r - 1), wr(r + 1), nb, off);
+        const bool fluid = OPEN || (nb & 1u);
+        const OffT home = (OffT)wr(r) * pitch_b + (OffT)X * ES;
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int i = 0; i < Q; ++i) {
+                const float v = ld_at<ST, OffT>(a.c[c].src, fluid ? off[i] : home);
+                p[c][i] = fluid ? v : 0.f;
+            }
+    };
+    uint64_t mw[3] = {0, 0, 0};   // mask words of row r + 4, loading during row r
+    auto mask_words = [&](int row) {
+        if constexpr (!OPEN) {
+            const size_t base = (size_t)wr(row) * wpr;
+            mw[0] = G(mask)[base + (Xm >> 6)];
+            mw[1] = G(mask)[base + (X >> 6)];
+            mw[2] = G(mask)[base + (Xp >> 6)];
+        }
+    };
+    auto tri_of_words = [&]() -> unsigned {
+        if constexpr (OPEN) return 7u;
+        else
+            return (unsigned)((mw[0] >> (Xm & 63)) & 1ull) | ((unsigned)((mw[1] >> (X & 63)) & 1ull) << 1) |
+                   ((unsigned)((mw[2] >> (Xp & 63)) & 1ull) << 2);
+    };
+    auto nbr_dens = [&](int q, int r, float (&nr)[2][Q], float (&rc)[2]) {
+        const int sS = r3(r - 1), sC = r3(r), sN = r3(r + 1);
+#pragma unroll
+        for (int i = 1; i < Q; ++i) {
+            const int sl = cy(i) == 1 ? sN : (cy(i) == -1 ? sS : sC);
+#pragma unroll
+            for (int c = 0; c < 2; ++c) nr[c][i] = L.d[q][sl][c][t + cx(i)];
+        }
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            rc[c] = L.d[q][sC][c][t];
+            nr[c][0] = rc[c];
+        }
+    };
+
+    const int r_first = y0 - 2 * K + 2, r_last = y1 + 2 * K - 3;   // rows stage 1 collides
+    const int load_last = y1 + 2 * K - 2;                          // last row loaded
+    // S_n populations rotate by name through three arrays (loop unrolled by 3)
+    float pX[2][Q], pY[2][Q], pZ[2][Q];
+    float sc[K - 1][2][Q], sp[K - 1][2][Q];   // stage s >= 2: rows gathered now / collided now
+    float rest_now[K - 1][2], rest_prev[K - 1][2];
+    // neighbourhood window: nbw[j] = bits of row r + 2 - j
+    unsigned nbw[2 * K + 1];
+#pragma unroll
+    for (int j = 0; j < 2 * K + 1; ++j) nbw[j] = 0u;
+#pragma unroll
+    for (int q = 0; q < K - 1; ++q)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            rest_now[q][c] = rest_prev[q][c] = 0.f;
+#pragma unroll
+            for (int i = 0; i < Q; ++i) sc[q][c][i] = sp[q][c][i] = 0.f;
+        }
+    // prologue: densities of rows r_first - 1, r_first; populations of row
+    // r_first (pX); loads of row r_first + 1 (pY) in flight
+    {
+        const unsigned nbq = nbits(r_first - 1);
+        gather1(r_first - 1, nbq, pZ);
+#pragma unroll
+        for (int c = 0; c < 2; ++c) L.d[0][r3(r_first - 1)][c][t] = dens(pZ[c]);
+        nbw[2] = nbits(r_first);
+        gather1(r_first, nbw[2], pX);
+#pragma unroll
+        for (int c = 0; c < 2; ++c) L.d[0][r3(r_first)][c][t] = dens(pX[c]);
+        nbw[1] = nbits(r_first + 1);
+        gather1(r_first + 1, nbw[1], pY);
+        mask_words(r_first + 3);
+    }
+    unsigned t_r1 = tri(wr(r_first + 1), Xm, X, Xp), t_r2 = tri(wr(r_first + 2), Xm, X, Xp);
+
+    auto row = [&](const int r, float (&pa)[2][Q], float (&pb)[2][Q], float (&pn)[2][Q]) {
+        // (a) stage 1: loads of row r + 2 (the last row reloads load_last,
+        // cached, rather than branching round the loads); densities of r + 1
+        const unsigned t_r3 = tri_of_words();
+        mask_words(r + 4);
+        nbw[0] = neigh(t_r1, t_r2, t_r3);
+        t_r1 = t_r2;
+        t_r2 = t_r3;
+        gather1(min(r + 2, load_last), nbw[0], pn);
+#pragma unroll
+        for (int c = 0; c < 2; ++c) L.d[0][r3(r + 1)][c][t] = dens(pb[c]);
+        __syncthreads();
+        // stage 1: collide row r into ring 0
+        if (t >= 1 && t < T - 1 && (OPEN || (nbw[2] & 1u))) {
+            float nr[2][Q], rc[2], oa[Q], ob[Q];
+            nbr_dens(0, r, nr, rc);
+            collide2_core<float, UeqStage, FORCE>(a, UeqStage{A.u, 0}, wr(r) * a.pitch + X, pa, rc, nr, OPEN ? 0x1FFu : nbw[2], oa, ob);
+            if constexpr (K > 1) {
+#pragma unroll
+                for (int i = 1; i < Q; ++i) {
+                    pr(0, r, 0, i, t) = ST::enc(oa[i]);
+                    pr(0, r, 1, i, t) = ST::enc(ob[i]);
+                }
+                rest_now[0][0] = ST::dec(ST::enc(oa[0]));
+                rest_now[0][1] = ST::dec(ST::enc(ob[0]));
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int s = 2; s <= K; ++s) {
+            const int q = s - 2;   // register / ring index of this stage's inputs
+            // gather of stage s at row g from ring q (stage s-1)
+            const int g = r - 2 * s + 3;
+            if (g >= y0 - 2 * (K - s) - 1 && t >= 2 * s - 2 && t < T - 2 * s + 2) {
+                const unsigned nbg = nbw[2 * s - 1];
+                if (OPEN || (nbg & 1u)) {
+#pragma unroll
+                    for (int c = 0; c < 2; ++c) {
+                        sc[q][c][0] = rest_prev[q][c];
+#pragma unroll
+                        for (int i = 1; i < Q; ++i) {
+                            const bool src_fluid = OPEN || ((nbg >> opp(i)) & 1u);
+                            const S v = src_fluid ? pr(q, g - cy(i), c, i, t - cx(i)) : pr(q, g, c, opp(i), t);
+                            sc[q][c][i] = ST::dec(v);
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int c = 0; c < 2; ++c)
+#pragma unroll
+                        for (int i = 0; i < Q; ++i) sc[q][c][i] = 0.f;
+                }
+#pragma unroll
+                for (int c = 0; c < 2; ++c) L.d[s - 1][r3(g)][c][t] = dens(sc[q][c]);
+            }
+            __syncthreads();
+            // collision of stage s at row h = r - 2(s - 1)
+            const int h = r - 2 * (s - 1);
+            const unsigned nbh = nbw[2 * s];
+            if (s < K) {
+                if (h >= y0 - 2 * (K - s) && t >= 2 * s - 1 && t < T - 2 * s + 1 && (OPEN || (nbh & 1u))) {
+                    float nr[2][Q], rc[2], oa[Q], ob[Q];
+                    nbr_dens(s - 1, h, nr, rc);
+                    collide2_core<float, UeqStage, FORCE>(a, UeqStage{A.u, s - 1}, wr(h) * a.pitch + X, sp[q], rc, nr, OPEN ? 0x1FFu : nbh, oa, ob);
+#pragma unroll
+                    for (int i = 1; i < Q; ++i) {
+                        pr(s - 1, h, 0, i, t) = ST::enc(oa[i]);
+                        pr(s - 1, h, 1, i, t) = ST::enc(ob[i]);
+                    }
+                    rest_now[s - 1][0] = ST::dec(ST::enc(oa[0]));
+                    rest_now[s - 1][1] = ST::dec(ST::enc(ob[0]));
+                }
+                __syncthreads();
+            } else if (h >= y0 && h < y1 && out_lane && (OPEN || (nbh & 1u))) {
+                float nr[2][Q], rc[2], oa[Q], ob[Q];
+                nbr_dens(s - 1, h, nr, rc);
+                collide2_core<float, UeqStage, FORCE>(a, UeqStage{A.u, s - 1}, h * a.pitch + X, sp[q], rc, nr, OPEN ? 0x1FFu : nbh, oa, ob);
+                const OffT home = (OffT)h * pitch_b + (OffT)X * ES;
+#pragma unroll
+                for (int i = 0; i < Q; ++i) {
+                    ST::stc((S*)((char*)A.dst[0] + (size_t)i * plane_b + home), oa[i]);
+                    ST::stc((S*)((char*)A.dst[1] + (size_t)i * plane_b + home), ob[i]);
+                }
+            }
+        }
+        // rotate (the S_n populations rotate by name)
+#pragma unroll
+        for (int q = 0; q < K - 1; ++q)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+#pragma unroll
+                for (int i = 0; i < Q; ++i) sp[q][c][i] = sc[q][c][i];
+                rest_prev[q][c] = rest_now[q][c];
+            }
+#pragma unroll
+        for (int j = 2 * K; j >= 1; --j) nbw[j] = nbw[j - 1];
+    };
+    // rows r_first .. r_last; the bounds are uniform over the workgroup, so
+    // every thread reaches the same barriers.  The row index is read from
+    // lane 0 to tell the compiler so: otherwise it keeps the loop counter
+    // and every row computation in vector registers (a divergent loop).
+    for (int r0 = r_first;; r0 += 3) {
+        const int r = __builtin_amdgcn_readfirstlane(r0);
+        row(r, pX, pY, pZ);
+        if (r + 1 > r_last) break;
+        row(r + 1, pY, pZ, pX);
+        if (r + 2 > r_last) break;
+        row(r + 2, pZ, pX, pY);
+        if (r + 3 > r_last) break;
+    }
+}
+
+// The lattice in one launch: strips of xk_out columns (A.a.gx of them) times
+// bands of A.a.rows rows, workgroup ids remapped so that each XCD owns a
+// contiguous range of tiles.
+template <class ST, bool OPEN, int K, int T, bool FORCE>
+__global__ __launch_bounds__(T) void k_stepk(const XkArgs<K> A) {
+    __shared__ XkLds<ST, OPEN, K, T> L;
+    const int nbx = A.a.gx;
+    const int nby = (A.a.ny + A.a.rows - 1) / A.a.rows;
+    int bid = blockIdx.x;
+    {
+        const int nblk = nbx * nby;
+        const int xcd = bid & 7, k = bid >> 3;
+        const int per = (nblk + 7) >> 3;
+        const int full = nblk - 8 * (per - 1);
+        bid = xcd < full ? xcd * per + k : full * per + (xcd - full) * (per - 1) + k;
+    }
+    const int bx = bid % nbx, by = bid / nbx;
+    const int y0 = by * A.a.rows;
+    stepk_tile<ST, OPEN, K, T, FORCE>(A, bx, y0, min(y0 + A.a.rows, A.a.ny), L);
+}
+
+}  // namespace scl

@@ -147,8 +147,8 @@ class Geometry:
         check(load().scl_set_lookahead(self.h, int(rows_ahead)))
 
     def set_step_fusion(self, steps_per_launch, rows=0):
-        """Time steps per launch (0 automatic, 1 or 2) and rows per band of the
-        two-step kernel (0 automatic)."""
+        """Time steps per launch (0 automatic, 1, 2 or 3) and rows per band of
+        the multi-step kernels (0 automatic)."""
         check(load().scl_set_step_fusion(self.h, int(steps_per_launch), int(rows)))
 
     def step_fusion(self, precision=FP32_DEV):

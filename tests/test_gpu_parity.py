@@ -305,9 +305,10 @@ def _dry_case(nx=320, ny=96):
 @pytest.mark.parametrize("prec", [nat.FP32_DEV, nat.FP16_DEV])
 @pytest.mark.parametrize("name", ["open", "porous", "dry"])
 def test_two_steps_per_launch_equal_single_steps(prec, name):
-    """The two-step kernel (step n stays on chip) gives the populations of
-    two single-step launches bit for bit, on lattices whose width is not a
-    multiple of the strip, with solids, adhesion, a force and dry sites."""
+    """The two- and three-step kernels (the intermediate steps stay on chip)
+    give the populations of single-step launches bit for bit, on lattices
+    whose width is not a multiple of the strip, with solids, adhesion, a
+    force and dry sites."""
     if name == "open":
         case = wl.spinodal(300, G=1.2, seed=4)
         case["force"] = [None, None]
@@ -316,15 +317,16 @@ def test_two_steps_per_launch_equal_single_steps(prec, name):
     else:
         case = _dry_case()
     out = []
-    for fusion in (1, 2):
+    for fusion in (1, 2, 3):
         g = nat.Geometry(case["mask"])
         g.set_step_fusion(fusion, 24)
+        assert g.step_fusion(prec)[0] == fusion
         fl = [nat.Field(g, case["tau"][c], prec, 1.0, g_solid=case["g_solid"][c], g_fluid=case["g_fluid"][c])
               for c in range(2)]
         for c in range(2):
             fl[c].set_f(case["f"][c])
         nat.solid_surface_force(g, fl[0], fl[1])
-        for n in (1, 6, 9, 2):   # bootstrap step, then even and odd chunks
+        for n in (1, 6, 9, 2, 5, 4):   # bootstrap step, then chunks of every remainder
             nat.step2(g, fl[0], fl[1], case["force"][0], case["force"][1], n)
         out.append([fl[c].get_f() for c in range(2)])
         for f in fl:
@@ -333,3 +335,4 @@ def test_two_steps_per_launch_equal_single_steps(prec, name):
     for c in range(2):
         assert np.all(np.isfinite(out[0][c]))
         assert np.array_equal(out[0][c], out[1][c])
+        assert np.array_equal(out[0][c], out[2][c])
