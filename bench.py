@@ -245,10 +245,12 @@ def main():
             % (form, res[form]["mlups"], res[form]["kernel"], res[form]["kernel_avg_ms"],
                res[form]["steps_per_launch"], res[form]["achieved_gbs"] or 0.0))
     head = res.get("fp32dev") or next(iter(res.values()))
-    traffic = None
+    traffic = valu_busy = None
     tpath = os.path.join(ROOT, "profiles", "traffic_%s_%s_%d.json" % (head["kernel"], head["form"], a.n))
     if os.path.exists(tpath):
-        traffic = json.load(open(tpath)).get("bytes_per_launch")
+        tj = json.load(open(tpath))
+        traffic = tj.get("bytes_per_launch")
+        valu_busy = tj.get("valu_busy_pct")   # the multi-step kernels are issue-bound, not HBM-bound
     achieved = head["achieved_gbs"]
     out = {
         "metric": "MLUPS + %HBM peak (FP32-dev vs FP64), 2-comp SC; p50/p99 served-job latency",
@@ -269,7 +271,8 @@ def main():
                    "tile": head["tile"], "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
-                     "kernel": head["kernel"], "steps_per_launch": head["steps_per_launch"]},
+                     "kernel": head["kernel"], "steps_per_launch": head["steps_per_launch"],
+                     "valu_busy_pct": valu_busy},
         "forms": {k: {kk: v[kk] for kk in ("mlups", "effective_pct_hbm", "kernel", "steps_per_launch",
                                            "band_rows", "kernel_avg_ms", "achieved_gbs", "tile")}
                   for k, v in res.items()},
