@@ -451,22 +451,30 @@ bool x2_ok(const scl_geometry* g, scl_field* const* fl, int nc) {
     return nc == 2 && fl[0]->phase == PH_P && fl[1]->phase == PH_P && x2_wanted(g, fl[0]->precision);
 }
 
+// Arguments of a multi-step launch (k_step2x, k_stepk): parameters and
+// P_{n-1} of both fields, P_{n+K-1} into the other buffer, the
+// equilibrium-velocity sets from the fields' current ones.
+template <int K>
+XkArgs<K> xk_args(const scl_geometry* g, scl_field* const* fl, const double* const* forces, int rows, int out) {
+    XkArgs<K> A{};
+    A.a = make_args<float>(g, rows, 4);
+    A.a.gx = (g->nx + out - 1) / out;
+    for (int c = 0; c < 2; ++c) {
+        const scl_field* f = fl[c];
+        fill_comp(A.a.c[c], f, f->cur, forces[c]);
+        A.dst[c] = f->f[f->cur ^ 1];
+        A.u.base[c] = f->ueq_block;
+        A.u.cur[c] = f->ucur;
+    }
+    A.u.n = (long long)n_sites(g);
+    return A;
+}
+
 template <class ST>
 int launch_x2(const scl_geometry* g, scl_field* const* fl, const double* const* forces) {
     const int rows = x2_rows_for(g, fl[0]->precision);
-    StepArgs<float> a1 = make_args<float>(g, rows, 4);
-    a1.gx = (g->nx + x2_out<ST>() - 1) / x2_out<ST>();
-    for (int c = 0; c < 2; ++c) fill_comp(a1.c[c], fl[c], fl[c]->cur, forces[c]);
-    StepArgs<float> a2 = a1;
-    for (int c = 0; c < 2; ++c) {
-        const scl_field* f = fl[c];
-        const int k1 = (f->ucur + 1) % kUeqRing, k2 = (f->ucur + 2) % kUeqRing;
-        a2.c[c].ueqx_in = f->ueqs[k1][0];
-        a2.c[c].ueqy_in = f->ueqs[k1][1];
-        a2.c[c].ueqx = f->ueqs[k2][0];
-        a2.c[c].ueqy = f->ueqs[k2][1];
-    }
-    const int nblk = a1.gx * ((g->ny + rows - 1) / rows);
+    const XkArgs<2> A = xk_args<2>(g, fl, forces, rows, x2_out<ST>());
+    const int nblk = A.a.gx * ((g->ny + rows - 1) / rows);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (g_lib.timing && g_lib.ev.size() < kMaxTimed) {
         HIPCHK(hipEventCreate(&e0));
@@ -477,11 +485,11 @@ int launch_x2(const scl_geometry* g, scl_field* const* fl, const double* const* 
     const bool force = forces[0] || forces[1];
     const dim3 grid(nblk), block(kX2Threads);
     if (g->open) {
-        if (force) hipLaunchKernelGGL((k_step2x<ST, true, true>), grid, block, 0, g_lib.stream, a1, a2);
-        else hipLaunchKernelGGL((k_step2x<ST, true, false>), grid, block, 0, g_lib.stream, a1, a2);
+        if (force) hipLaunchKernelGGL((k_step2x<ST, true, true>), grid, block, 0, g_lib.stream, A);
+        else hipLaunchKernelGGL((k_step2x<ST, true, false>), grid, block, 0, g_lib.stream, A);
     } else {
-        if (force) hipLaunchKernelGGL((k_step2x<ST, false, true>), grid, block, 0, g_lib.stream, a1, a2);
-        else hipLaunchKernelGGL((k_step2x<ST, false, false>), grid, block, 0, g_lib.stream, a1, a2);
+        if (force) hipLaunchKernelGGL((k_step2x<ST, false, true>), grid, block, 0, g_lib.stream, A);
+        else hipLaunchKernelGGL((k_step2x<ST, false, false>), grid, block, 0, g_lib.stream, A);
     }
     HIPCHK(hipGetLastError());
     if (e0) {
@@ -546,17 +554,7 @@ int launch_x3_t(const scl_geometry* g, scl_field* const* fl, const double* const
     constexpr int T = x3_threads<ST, OPEN>();
     constexpr int OUT = xk_out<ST, 3, T>();
     const int rows = x3_rows_for(g, fl[0]->precision);
-    XkArgs<3> A{};
-    A.a = make_args<float>(g, rows, 4);
-    A.a.gx = (g->nx + OUT - 1) / OUT;
-    for (int c = 0; c < 2; ++c) {
-        const scl_field* f = fl[c];
-        fill_comp(A.a.c[c], f, f->cur, forces[c]);
-        A.dst[c] = f->f[f->cur ^ 1];
-        A.u.base[c] = f->ueq_block;
-        A.u.cur[c] = f->ucur;
-    }
-    A.u.n = (long long)n_sites(g);
+    const XkArgs<3> A = xk_args<3>(g, fl, forces, rows, OUT);
     const int nblk = A.a.gx * ((g->ny + rows - 1) / rows);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (g_lib.timing && g_lib.ev.size() < kMaxTimed) {

@@ -1,6 +1,11 @@
 // step2x.h -- two fused time steps per launch for the two-component
 // deviation forms (FP32 arithmetic; FP32 or FP16 storage).
 //
+// step2x_tile is the two-step tile of the server's persistent kernel, whose
+// per-job arguments carry separate equilibrium-velocity pointers.  Fields
+// launch k_step2x (stepk.h), the K = 2 case of the K-step tile, with the
+// same strips, the same LDS layout and the same results.
+//
 // A launch reads P_{n-1} and writes P_{n+1}; the intermediate post-collision
 // populations P_n never leave the workgroup, so a population crosses HBM
 // once per two steps.  The per-site arithmetic is that of the single-step
@@ -296,27 +301,6 @@ __device__ inline void step2x_tile(const StepArgs<float>& a1, const StepArgs<flo
         row(r + 2, pZ, pX, pY);
         if (r + 3 > y1 + 1) break;
     }
-}
-
-// The lattice in one launch: strips of x2_out<ST>() columns (a1.gx of them)
-// times bands of a1.rows rows; workgroup ids remapped so that each XCD owns
-// a contiguous range of tiles, as in k_step.
-template <class ST, bool OPEN, bool FORCE>
-__global__ __launch_bounds__(kX2Threads) void k_step2x(const StepArgs<float> a1, const StepArgs<float> a2) {
-    __shared__ X2Lds<ST, OPEN> L;
-    const int nbx = a1.gx;
-    const int nby = (a1.ny + a1.rows - 1) / a1.rows;
-    int bid = blockIdx.x;
-    {
-        const int nblk = nbx * nby;
-        const int xcd = bid & 7, k = bid >> 3;
-        const int per = (nblk + 7) >> 3;
-        const int full = nblk - 8 * (per - 1);
-        bid = xcd < full ? xcd * per + k : full * per + (xcd - full) * (per - 1) + k;
-    }
-    const int bx = bid % nbx, by = bid / nbx;
-    const int y0 = by * a1.rows;
-    step2x_tile<ST, OPEN, FORCE>(a1, a2, bx, y0, min(y0 + a1.rows, a1.ny), L);
 }
 
 }  // namespace scl

@@ -332,8 +332,7 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
 // bands of A.a.rows rows, workgroup ids remapped so that each XCD owns a
 // contiguous range of tiles.
 template <class ST, bool OPEN, int K, int T, bool FORCE>
-__global__ __launch_bounds__(T) void k_stepk(const XkArgs<K> A) {
-    __shared__ XkLds<ST, OPEN, K, T> L;
+__device__ inline void stepk_grid(const XkArgs<K>& A, XkLds<ST, OPEN, K, T>& L) {
     const int nbx = A.a.gx;
     const int nby = (A.a.ny + A.a.rows - 1) / A.a.rows;
     int bid = blockIdx.x;
@@ -347,6 +346,27 @@ __global__ __launch_bounds__(T) void k_stepk(const XkArgs<K> A) {
     const int bx = bid % nbx, by = bid / nbx;
     const int y0 = by * A.a.rows;
     stepk_tile<ST, OPEN, K, T, FORCE>(A, bx, y0, min(y0 + A.a.rows, A.a.ny), L);
+}
+
+template <class ST, bool OPEN, int K, int T, bool FORCE>
+__global__ __launch_bounds__(T) void k_stepk(const XkArgs<K> A) {
+    __shared__ XkLds<ST, OPEN, K, T> L;
+    stepk_grid<ST, OPEN, K, T, FORCE>(A, L);
+}
+
+// Two steps per launch on a field: the K = 2 tile, with the strips of
+// step2x_tile (x2_out columns).  The server's persistent kernel keeps
+// step2x_tile, whose equilibrium-velocity buffers come per job; here the
+// buffer sets of the ring take 8 scalar registers instead of 32, which spared
+// the general (solid) variant most of its scalar spills (9% fewer vector
+// instructions).
+static_assert(xk_out<StoreF32, 2, kX2Threads>() == x2_out<StoreF32>() &&
+                  xk_out<StoreF16, 2, kX2Threads>() == x2_out<StoreF16>(),
+              "two-step strips");
+template <class ST, bool OPEN, bool FORCE>
+__global__ __launch_bounds__(kX2Threads) void k_step2x(const XkArgs<2> A) {
+    __shared__ XkLds<ST, OPEN, 2, kX2Threads> L;
+    stepk_grid<ST, OPEN, 2, kX2Threads, FORCE>(A, L);
 }
 
 }  // namespace scl

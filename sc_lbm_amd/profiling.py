@@ -30,8 +30,8 @@ def main(argv=None):
     s.add_argument("--steps", type=int, default=20)
     s.add_argument("--warmup", type=int, default=2)
     s.add_argument("--tile", default="")
-    s.add_argument("--fusion", type=int, default=0, help="time steps per launch: 0 automatic, 1 or 2")
-    s.add_argument("--band", type=int, default=0, help="rows per band of the two-step kernel (0 automatic)")
+    s.add_argument("--fusion", type=int, default=0, help="time steps per launch: 0 automatic, 1, 2 or 3")
+    s.add_argument("--band", type=int, default=0, help="rows per band of the multi-step kernels (0 automatic)")
     s.add_argument("--porosity", type=float, default=1.0,
                    help="below 1: seeded disk obstacles (radii 8..20) to this fluid fraction")
     c = sub.add_parser("copy")
@@ -76,7 +76,7 @@ def main(argv=None):
     avg, cnt = nat.timing(0)            # per time step
     spl, band = g.step_fusion(nat.PRECISION[a.form])
     print(json.dumps({"form": a.form, "n": a.n, "porosity": a.porosity, "steps": a.steps,
-                      "kernel": "k_step2x" if spl == 2 else "k_step", "steps_per_launch": spl, "band_rows": band,
+                      "kernel": {1: "k_step", 2: "k_step2x", 3: "k_stepk"}[spl], "steps_per_launch": spl, "band_rows": band,
                       "kernel_avg_ms": avg * a.steps / cnt if cnt else None, "launches": cnt,
                       "tile": list(g.tile(2, nat.PRECISION[a.form])),
                       "mlups": a.n * a.n / (avg * 1e-3) / 1e6 if avg else None}))
