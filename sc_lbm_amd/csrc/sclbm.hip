@@ -467,6 +467,8 @@ XkArgs<K> xk_args(const scl_geometry* g, scl_field* const* fl, const double* con
         A.u.cur[c] = f->ucur;
     }
     A.u.n = (long long)n_sites(g);
+    A.u.set_stride = 2 * A.u.n;
+    A.u.nsets = kUeqRing;
     return A;
 }
 

@@ -18,16 +18,17 @@
 //
 // Two steps per launch (FP32 deviations, jobs at least 256 columns wide with
 // two or more steps in the round): k_serve2x takes tiles of a second list
-// (strips of x2_out columns) and computes each with step2x_tile, the body of
-// the single-lattice two-step kernel; launch q of a round advances steps 2q
-// and 2q+1 of such a job.  An odd last step goes through k_serve.  A pair
-// reads one buffer and writes the other, so after q pairs the job's
-// populations are in buffer cur ^ (q & 1).
+// (strips of x2_out columns) and computes each with the K = 2 tile of
+// stepk_tile, the body of the single-lattice two-step kernel; launch q of a
+// round advances steps 2q and 2q+1 of such a job.  An odd last step goes
+// through k_serve.  A pair reads one buffer and writes the other, so after q
+// pairs the job's populations are in buffer cur ^ (q & 1).  Rounds in which
+// no two-step job has a body force launch the force-free variant.
 #pragma once
 #include "../host/serve_ic.h"
 #include "convert.h"
 #include "step.h"
-#include "step2x.h"
+#include "stepk.h"
 
 namespace scl {
 
@@ -47,6 +48,7 @@ struct JobArgs {
     // [x/y]; step s reads buffer (ucur + s) % 3 and writes the next
     double* ueq[3][2][2];
     int ucur;
+    UeqRing ring;               // the same buffers as a ring (cur = ucur)
 };
 
 struct ServeHdr {
@@ -132,12 +134,13 @@ k_serve(const ServeHdr* __restrict__ hdr, int s, unsigned* __restrict__ ctr) {
 }
 
 // Launch q of a round: steps 2q and 2q+1 of every two-step job that has them,
-// one tile of the second list per ticket (FP32 storage; the general variant,
-// since the jobs carry masks).
-template <class ST>
+// one tile of the second list per ticket (the general variant, since the
+// jobs carry masks).  FORCE = false: no two-step job of the round has a body
+// force.
+template <class ST, bool FORCE>
 __global__ __launch_bounds__(kX2Threads) void k_serve2x(const ServeHdr* __restrict__ hdr, int q,
                                                         unsigned* __restrict__ ctr) {
-    __shared__ X2Lds<ST, false> lds;
+    __shared__ XkLds<ST, false, 2, kX2Threads> lds;
     __shared__ int ticket;
     const int n = hdr->n_items2;
     const SCL_AS_CONST JobArgs<float>* jobs = K((const JobArgs<float>*)hdr->jobs);
@@ -157,24 +160,18 @@ __global__ __launch_bounds__(kX2Threads) void k_serve2x(const ServeHdr* __restri
         y1 = __builtin_amdgcn_readfirstlane(y1);
         const SCL_AS_CONST JobArgs<float>& J = jobs[j];
         if (2 * q + 1 >= J.steps) continue;
-        StepArgs<float> a1 = J.a, a2 = J.a;
-        const int s = 2 * q, rb = (J.cur ^ q) & 1;
+        XkArgs<2> A;
+        A.a = J.a;
+        A.u = J.ring;
+        const int rb = (J.cur ^ q) & 1;
+        const int cur = (J.ucur + 2 * q) % J.ring.nsets;
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
-            const void* src = rb ? J.a.c[c].dst : J.a.c[c].src;
-            void* dst = rb ? const_cast<void*>(J.a.c[c].src) : J.a.c[c].dst;
-            a1.c[c].src = src;
-            a2.c[c].dst = dst;
-            a1.c[c].ueqx_in = J.ueq[(J.ucur + s) % 3][c][0];
-            a1.c[c].ueqy_in = J.ueq[(J.ucur + s) % 3][c][1];
-            a1.c[c].ueqx = J.ueq[(J.ucur + s + 1) % 3][c][0];
-            a1.c[c].ueqy = J.ueq[(J.ucur + s + 1) % 3][c][1];
-            a2.c[c].ueqx_in = a1.c[c].ueqx;
-            a2.c[c].ueqy_in = a1.c[c].ueqy;
-            a2.c[c].ueqx = J.ueq[(J.ucur + s + 2) % 3][c][0];
-            a2.c[c].ueqy = J.ueq[(J.ucur + s + 2) % 3][c][1];
+            A.a.c[c].src = rb ? J.a.c[c].dst : J.a.c[c].src;
+            A.dst[c] = rb ? const_cast<void*>(J.a.c[c].src) : J.a.c[c].dst;
+            A.u.cur[c] = cur;
         }
-        step2x_tile<ST, false>(a1, a2, bx, y0, y1, lds);
+        stepk_tile<ST, false, 2, kX2Threads, FORCE>(A, bx, y0, y1, lds);
     }
     if (threadIdx.x == 0) {
         const unsigned d = __hip_atomic_fetch_add(ctr + 1, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);

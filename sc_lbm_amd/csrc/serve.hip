@@ -188,7 +188,7 @@ class GpuServe final : public serve::Backend {
             x2rows_ = cfg_.band_rows ? cfg_.band_rows : 32;
             int per_cu = 0;
             SHIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &per_cu, reinterpret_cast<const void*>(&k_serve2x<ST>), kX2Threads, 0));
+                &per_cu, reinterpret_cast<const void*>(&k_serve2x<ST, true>), kX2Threads, 0));
             grid2_ = std::max(1, per_cu) * cus_;
         }
         SHIP(hipStreamSynchronize(stream_));
@@ -238,6 +238,7 @@ class GpuServe final : public serve::Backend {
         if (rc != SCL_OK) return rc;
         JobArgs<T>* table = (JobArgs<T>*)table_h_;
         int n = 0, n_step = 0, n2 = 0;
+        bool force2 = false;   // a two-step job of the round has a body force
         for (int k : order) {
             const GJob* g = (const GJob*)jobs[k]->dev;
             JobArgs<T>& ja = table[k];
@@ -251,7 +252,10 @@ class GpuServe final : public serve::Backend {
             n = emit_tiles(k, g->nx, g->ny, nw_, rows_, items_h_, n);
             ja.item_end = n;
             if (single(k)) n_step = n;
-            if (two[k]) n2 = emit_tiles2(k, g->nx, g->ny, x2rows_, items2_h_, n2);
+            if (two[k]) {
+                n2 = emit_tiles2(k, g->nx, g->ny, x2rows_, items2_h_, n2);
+                force2 = force2 || jobs[k]->spec.has_force[0] || jobs[k]->spec.has_force[1];
+            }
         }
         *hdr_h_ = ServeHdr{table_d_, items_d_, partial_d_, mass_d_, n, (int)jobs.size(), 64 * nw_, n_step,
                            items2_d_, n2};
@@ -261,7 +265,7 @@ class GpuServe final : public serve::Backend {
                                     stream_));
         SHIP(hipMemcpyAsync(hdr_d_, hdr_h_, sizeof(ServeHdr), hipMemcpyHostToDevice, stream_));
         hipGraphExec_t ge = nullptr;
-        rc = graph(rsteps, n2 > 0, &ge, err);
+        rc = graph(rsteps, n2 > 0, force2, &ge, err);
         if (rc != SCL_OK) return rc;
         SHIP(hipEventRecord(ev_[0], stream_));
         SHIP(hipGraphLaunch(ge, stream_));
@@ -330,6 +334,18 @@ class GpuServe final : public serve::Backend {
             for (int c = 0; c < 2; ++c)
                 for (int d = 0; d < 2; ++d) ja.ueq[k][c][d] = g.ueq(k < g.L.nueq ? k : 0, c, d);
         ja.ucur = g.ucur;
+        // The ring view (two-step tiles).  layout_of takes the buffers in
+        // [k][c][d] order, each plane * 8 bytes, a multiple of 512 (the pitch
+        // is a multiple of 64), so they are contiguous: component c's set k,
+        // direction d sits at ueq(0, c, 0) + (4k + d) * plane doubles.
+        UeqRing& r = ja.ring;
+        r.n = g.L.plane;
+        r.set_stride = 4 * g.L.plane;
+        r.nsets = g.L.nueq;
+        for (int c = 0; c < 2; ++c) {
+            r.base[c] = g.ueq(0, c, 0);
+            r.cur[c] = g.ucur;
+        }
     }
 
     static size_t tiles_of(int nx, int ny, int nw, int rows) {
@@ -501,9 +517,10 @@ class GpuServe final : public serve::Backend {
     }
 
     // The round as a graph: rsteps single-step launches, or (two) per pair q
-    // a two-step launch then the single-step launches of steps 2q and 2q+1.
-    int graph(int rsteps, bool two, hipGraphExec_t* out, std::string* err) {
-        const int key = 2 * rsteps + (two ? 1 : 0);
+    // a two-step launch then the single-step launches of steps 2q and 2q+1;
+    // force: the two-step launches take the body-force variant.
+    int graph(int rsteps, bool two, bool force, hipGraphExec_t* out, std::string* err) {
+        const int key = 4 * rsteps + (two ? 1 : 0) + (two && force ? 2 : 0);
         auto it = graphs_.find(key);
         if (it != graphs_.end()) {
             *out = it->second;
@@ -511,7 +528,7 @@ class GpuServe final : public serve::Backend {
         }
         SHIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
         for (int s = 0; s < rsteps; ++s) {
-            if (two && !(s & 1) && s + 1 < rsteps) launch_pair(s / 2);
+            if (two && !(s & 1) && s + 1 < rsteps) launch_pair(s / 2, force);
             launch_step(nw_, la_, grid_, s);
         }
         hipLaunchKernelGGL(k_serve_mass<ST>, dim3(mass_grid_), dim3(256), 0, stream_, hdr_d_);
@@ -534,9 +551,15 @@ class GpuServe final : public serve::Backend {
         return SCL_OK;
     }
 
-    void launch_pair(int q) {
-        if constexpr (std::is_same<ST, StoreF32>::value)
-            hipLaunchKernelGGL(k_serve2x<ST>, dim3(grid2_), dim3(kX2Threads), 0, stream_, hdr_d_, q, ctr_d_);
+    void launch_pair(int q, bool force) {
+        if constexpr (std::is_same<ST, StoreF32>::value) {
+            if (force)
+                hipLaunchKernelGGL((k_serve2x<ST, true>), dim3(grid2_), dim3(kX2Threads), 0, stream_, hdr_d_, q,
+                                   ctr_d_);
+            else
+                hipLaunchKernelGGL((k_serve2x<ST, false>), dim3(grid2_), dim3(kX2Threads), 0, stream_, hdr_d_, q,
+                                   ctr_d_);
+        }
     }
 
     // Tile shape by measurement: two resident 2048 x 2048 porous jobs

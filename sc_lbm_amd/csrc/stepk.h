@@ -58,24 +58,28 @@ struct XkLds {
     typename ST::S pd[K - 1][OPEN ? 1 : 2][2][3][T];     // c_y = -1 (4, 7, 8)
 };
 
-// Equilibrium-velocity buffers of a deviation-form field: one block of
-// kUeqRing sets of (x, y) buffers of n doubles each; set k, direction d at
-// base + (2k + d) n.  A step reads set cur and writes set cur + 1 (mod
-// kUeqRing).  Eight scalar registers describe every set of both components,
-// where separate pointers for K stages would take 16 K and spill.
+// Equilibrium-velocity buffers of a deviation-form field: nsets sets of
+// (x, y) buffers of n doubles each, component c's set k direction d at
+// base[c] + k * set_stride + d * n.  A step reads set cur and writes set
+// cur + 1 (mod nsets).  A few scalar registers describe every set of both
+// components, where separate pointers for K stages would take 16 K and
+// spill.  Fields hold kUeqRing sets (one more than the most steps a launch
+// fuses), the server three.
 constexpr int kUeqRing = 4;
 struct UeqRing {
     double* base[2];
-    long long n;
-    int cur[2];
+    long long n, set_stride;
+    int nsets;
+    int cur[2];   // < nsets
 };
 // The buffers stage s (0-based) of a launch reads and writes.
 struct UeqStage {
     const UeqRing& r;
     int s;
     __device__ double* buf(int c, int next, int d) const {
-        const int k = (r.cur[c] + s + next) & (kUeqRing - 1);
-        return r.base[c] + (long long)(2 * k + d) * r.n;
+        int k = r.cur[c] + s + next;   // < 2 nsets: cur < nsets, s + next <= K < nsets
+        k -= k >= r.nsets ? r.nsets : 0;
+        return r.base[c] + k * r.set_stride + d * r.n;
     }
     __device__ const double* ix(int c) const { return buf(c, 0, 0); }
     __device__ const double* iy(int c) const { return buf(c, 0, 1); }
