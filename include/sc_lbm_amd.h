@@ -157,6 +157,32 @@ int scl_step2(const scl_geometry* g, scl_field* a, scl_field* b,
  * device in a fixed order: out = {mass, momentum_x, momentum_y}. */
 int scl_moments(const scl_geometry* g, scl_field* f, double out[3]);
 
+/* ---- Droplet observables on the device.  The reference evaluates these in
+ *      Python from the density files its drivers write; these entry points
+ *      replace those scripts' arithmetic without downloading the fields.
+ *      Densities are those of the fields' current streamed states (what
+ *      Fluid::compute_density would give now), any storage form. ------- */
+
+/* Laplace law, benchmarks/two_phase_two_components_flows/laplace_law/
+ * compute_surface_tension.py:63-70 (G12, p_in, p_out, rho_bulk, rho_disp of
+ * lines 34-43).  out[0] = p(x_in, y_in) - p(x_out, y_out) with
+ * p = 1/3 (rho_b + rho_d) + G/3 rho_b rho_d; out[1] = the droplet radius
+ * 0.5 (Rx + Ry), half the y and x extents of the sites with rho_d < rho_b,
+ * rho_disp < rho_b <= rho_bulk.  SCL_ERR_RANGE for a site outside the
+ * lattice or an empty set. */
+int scl_laplace_observables(const scl_geometry* g, scl_field* droplet, scl_field* bulk, double G, int x_in,
+                            int y_in, int x_out, int y_out, double rho_bulk, double rho_disp, double out[2]);
+
+/* Contact angle of a droplet resting on a solid below it,
+ * benchmarks/two_phase_two_components_flows/contact_angles/
+ * compute_contact_angles.py:46-100: interface sites rho_b <= rho_d,
+ * rho_d <= rho_bulk, rho_b > rho_disp; base from its two lowest rows, height
+ * from its y extent.  out = {theta in degrees, base length L, height H,
+ * radius R}.  SCL_ERR_RANGE if the interface is empty or has no site in the
+ * row above its lowest. */
+int scl_contact_angle(const scl_geometry* g, scl_field* droplet, scl_field* bulk, double rho_bulk, double rho_disp,
+                      double out[4]);
+
 /* Launch shape of the fused kernels for geometry g: columns per workgroup
  * (64, 128, 256 or 512) and rows per workgroup (>= 1).  0 = automatic.
  * Results do not depend on the shape (per-site arithmetic is fixed). */

@@ -230,6 +230,79 @@ __global__ void k_moments_final(int rows, const double* __restrict__ part, doubl
     }
 }
 
+// Droplet observables (the reference evaluates them in Python from the
+// density files its drivers write).  Per row y of the FP64 populations of
+// the droplet (fd) and bulk (fb) components, the number of sites and the
+// least and greatest x of two site sets:
+//   out[6y + 0..2]: the Laplace-law set, rho_d - rho_b < 0, rho_b <= rho_bulk,
+//     rho_b > rho_disp (laplace_law/compute_surface_tension.py:66);
+//   out[6y + 3..5]: the contact-angle interface, rho_b - rho_d <= 0,
+//     rho_d <= rho_bulk, rho_b > rho_disp (contact_angles/
+//     compute_contact_angles.py:47).
+// Densities in the reference's summation order (dens).  Empty sets leave
+// min x = nx, max x = -1.
+__global__ void k_obs_rows(Grid2 g, const double* __restrict__ fd, const double* __restrict__ fb, double rho_bulk,
+                           double rho_disp, int* __restrict__ out) {
+    const int y = blockIdx.x;
+    int n[2] = {0, 0}, lo[2] = {g.nx, g.nx}, hi[2] = {-1, -1};
+    for (int x = threadIdx.x; x < g.nx; x += blockDim.x) {
+        const int s = y * g.pitch + x;
+        double pd[Q], pb[Q];
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            pd[i] = fd[i * g.plane + s];
+            pb[i] = fb[i * g.plane + s];
+        }
+        const double rd = dens(pd), rb = dens(pb);
+        const bool in[2] = {rd - rb < 0.0 && rb <= rho_bulk && rb > rho_disp,
+                            rb - rd <= 0.0 && rd <= rho_bulk && rb > rho_disp};
+#pragma unroll
+        for (int k = 0; k < 2; ++k)
+            if (in[k]) {
+                ++n[k];
+                lo[k] = min(lo[k], x);
+                hi[k] = max(hi[k], x);
+            }
+    }
+    __shared__ int red[6][256];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        red[3 * k][threadIdx.x] = n[k];
+        red[3 * k + 1][threadIdx.x] = lo[k];
+        red[3 * k + 2][threadIdx.x] = hi[k];
+    }
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                red[3 * k][threadIdx.x] += red[3 * k][threadIdx.x + w];
+                red[3 * k + 1][threadIdx.x] = min(red[3 * k + 1][threadIdx.x], red[3 * k + 1][threadIdx.x + w]);
+                red[3 * k + 2][threadIdx.x] = max(red[3 * k + 2][threadIdx.x], red[3 * k + 2][threadIdx.x + w]);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) out[6 * y + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// Densities of both components at sites s0 and s1: out = {rho_d(s0),
+// rho_b(s0), rho_d(s1), rho_b(s1)}.  One thread.
+__global__ void k_obs_points(Grid2 g, const double* __restrict__ fd, const double* __restrict__ fb, int s0, int s1,
+                             double* __restrict__ out) {
+    const int ss[2] = {s0, s1};
+    for (int k = 0; k < 2; ++k) {
+        double pd[Q], pb[Q];
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            pd[i] = fd[i * g.plane + ss[k]];
+            pb[i] = fb[i * g.plane + ss[k]];
+        }
+        out[2 * k] = dens(pd);
+        out[2 * k + 1] = dens(pb);
+    }
+}
+
 // Populations of a fluid at rest with density rho everywhere (f_i = w_i*rho),
 // in the storage form; used for the autotuner's scratch fields.
 template <class ST>

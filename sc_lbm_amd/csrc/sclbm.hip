@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -126,6 +127,7 @@ struct scl_geometry {
     int x2_tuned[3] = {};               // [precision]: autotuned band rows; -1: single steps were faster
     int x3_tuned[3] = {};               // [precision]: the same for three steps per launch; -1: slower
     double* part = nullptr;             // moment partials (3 per row + 3)
+    int* obs = nullptr;                 // droplet-observable row extents (6 per row), allocated on first use
     // Fields keep their geometry alive: destroying a geometry that fields
     // still use only marks it, and the last field destroyed frees it.
     int refs = 0;
@@ -820,6 +822,7 @@ static void geometry_free(scl_geometry* g) {
     if (g_lib.ready) (void)hipStreamSynchronize(g_lib.stream);
     dfree(g->mask);
     dfree(g->part);
+    dfree(g->obs);
     delete g;
 }
 
@@ -1259,9 +1262,10 @@ int scl_step2(const scl_geometry* g, scl_field* a, scl_field* b, const double* f
 
 // ---- diagnostics ----------------------------------------------------------
 
-int scl_moments(const scl_geometry* g, scl_field* f, double out[3]) {
-    CHK(check_geom(g, f));
-    if (!out) return fail(SCL_ERR_ARG, "null output");
+namespace {
+// FP64 populations of f's streamed state in f->stage (gathered with the mask
+// that produced them if f holds post-collision populations); *gr: that grid.
+int stage_f64(const scl_geometry* g, scl_field* f, Grid2* gr_out) {
     CHK(ensure_stage(f));
     const void* s = f->f[f->cur];
     const bool gth = f->phase == PH_P;
@@ -1281,12 +1285,116 @@ int scl_moments(const scl_geometry* g, scl_field* f, double out[3]) {
             break;
     }
     HIPCHK(hipGetLastError());
+    if (gr_out) *gr_out = gr;
+    return SCL_OK;
+}
+
+// Row extents of the two droplet-observable site sets (k_obs_rows) and the
+// densities at sites s0, s1, on the host.
+int droplet_rows(const scl_geometry* g, scl_field* d, scl_field* b, double rho_bulk, double rho_disp, int s0,
+                 int s1, std::vector<int>* rows, double pts[4]) {
+    if (!d || !b) return fail(SCL_ERR_ARG, "null field");
+    if (d == b) return fail(SCL_ERR_ARG, "the droplet and bulk components must be different fields");
+    CHK(check_geom(g, d));
+    CHK(check_geom(g, b));
+    Grid2 gr{};
+    CHK(stage_f64(g, d, &gr));
+    CHK(stage_f64(g, b, nullptr));
+    if (!g->obs) CHK(dalloc(&const_cast<scl_geometry*>(g)->obs, (size_t)6 * g->ny));
+    hipLaunchKernelGGL(k_obs_rows, dim3(g->ny), dim3(256), 0, g_lib.stream, gr, (const double*)d->stage,
+                       (const double*)b->stage, rho_bulk, rho_disp, g->obs);
+    hipLaunchKernelGGL(k_obs_points, dim3(1), dim3(1), 0, g_lib.stream, gr, (const double*)d->stage,
+                       (const double*)b->stage, s0, s1, g->part);
+    HIPCHK(hipGetLastError());
+    rows->assign((size_t)6 * g->ny, 0);
+    HIPCHK(hipMemcpyAsync(rows->data(), g->obs, rows->size() * sizeof(int), hipMemcpyDeviceToHost, g_lib.stream));
+    HIPCHK(hipMemcpyAsync(pts, g->part, 4 * sizeof(double), hipMemcpyDeviceToHost, g_lib.stream));
+    HIPCHK(hipStreamSynchronize(g_lib.stream));
+    return SCL_OK;
+}
+}  // namespace
+
+int scl_moments(const scl_geometry* g, scl_field* f, double out[3]) {
+    CHK(check_geom(g, f));
+    if (!out) return fail(SCL_ERR_ARG, "null output");
+    Grid2 gr{};
+    CHK(stage_f64(g, f, &gr));
     hipLaunchKernelGGL(k_moments_rows, dim3(g->ny), dim3(256), 0, g_lib.stream, gr, (const double*)f->stage, g->part);
     hipLaunchKernelGGL(k_moments_final, dim3(1), dim3(256), 0, g_lib.stream, g->ny, (const double*)g->part,
                        g->part + 3 * (size_t)g->ny);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(out, g->part + 3 * (size_t)g->ny, 3 * sizeof(double), hipMemcpyDeviceToHost, g_lib.stream));
     HIPCHK(hipStreamSynchronize(g_lib.stream));
+    return SCL_OK;
+}
+
+// Laplace law, as laplace_law/compute_surface_tension.py:63-70 evaluates it
+// from the two density fields: the pressure p = (rho_b + rho_d)/3 +
+// G/3 rho_b rho_d at two sites, and the radius of the droplet set.
+int scl_laplace_observables(const scl_geometry* g, scl_field* droplet, scl_field* bulk, double G, int x_in,
+                            int y_in, int x_out, int y_out, double rho_bulk, double rho_disp, double out[2]) {
+    if (!g || !out) return fail(SCL_ERR_ARG, "null argument");
+    if (x_in < 0 || x_in >= g->nx || y_in < 0 || y_in >= g->ny || x_out < 0 || x_out >= g->nx || y_out < 0 ||
+        y_out >= g->ny)
+        return fail(SCL_ERR_RANGE, "pressure sites outside the %dx%d lattice", g->nx, g->ny);
+    std::vector<int> rows;
+    double pts[4];
+    CHK(droplet_rows(g, droplet, bulk, rho_bulk, rho_disp, y_in * g->pitch + x_in, y_out * g->pitch + x_out, &rows,
+                     pts));
+    int ylo = -1, yhi = -1, xlo = g->nx, xhi = -1;
+    for (int y = 0; y < g->ny; ++y) {
+        if (!rows[6 * y]) continue;
+        if (ylo < 0) ylo = y;
+        yhi = y;
+        xlo = std::min(xlo, rows[6 * y + 1]);
+        xhi = std::max(xhi, rows[6 * y + 2]);
+    }
+    if (ylo < 0) return fail(SCL_ERR_RANGE, "no site of the droplet set (rho_d < rho_b, rho_disp < rho_b <= rho_bulk)");
+    auto p = [&](double rd, double rb) { return 1.0 / 3.0 * (rb + rd) + G / 3.0 * rb * rd; };
+    out[0] = p(pts[0], pts[1]) - p(pts[2], pts[3]);
+    // the script's [0] index is the row (y), [1] the column (x)
+    const double ry = (yhi - ylo) / 2.0, rx = (xhi - xlo) / 2.0;
+    out[1] = 0.5 * (ry + rx);
+    return SCL_OK;
+}
+
+// Contact angle of a droplet on a solid below it, as
+// contact_angles/compute_contact_angles.py:46-100 evaluates it: the base
+// from the interface's two lowest rows (extrapolated half a row down), the
+// height from its extent in y, the circle through them, and the angle.
+int scl_contact_angle(const scl_geometry* g, scl_field* droplet, scl_field* bulk, double rho_bulk, double rho_disp,
+                      double out[4]) {
+    if (!g || !out) return fail(SCL_ERR_ARG, "null argument");
+    std::vector<int> rows;
+    double pts[4];
+    CHK(droplet_rows(g, droplet, bulk, rho_bulk, rho_disp, 0, 0, &rows, pts));
+    int ylo = -1, yhi = -1;
+    for (int y = 0; y < g->ny; ++y)
+        if (rows[6 * y + 3]) {
+            if (ylo < 0) ylo = y;
+            yhi = y;
+        }
+    if (ylo < 0) return fail(SCL_ERR_RANGE, "no interface site (rho_b <= rho_d, rho_d <= rho_bulk, rho_b > rho_disp)");
+    if (ylo + 1 >= g->ny || !rows[6 * (ylo + 1) + 3])
+        return fail(SCL_ERR_RANGE, "the interface has no site in the row above its lowest one");
+    const double y2 = ylo, y1 = ylo + 1;                  // yL2 = yR2, yL1 = yR1
+    double xl1 = rows[6 * (ylo + 1) + 4], xl2 = rows[6 * ylo + 4];
+    double xr1 = rows[6 * (ylo + 1) + 5], xr2 = rows[6 * ylo + 5];
+    const double yb = ylo - 0.5;                          // yL = yR
+    if (xl1 == xl2) xl1 = xl1 - 1;
+    if (xr1 == xr2) xr1 = xr1 - 1;
+    const double sl = (y1 - y2) / (xl1 - xl2), sr = (y1 - y2) / (xr1 - xr2);
+    const double il = y2 - sl * xl2, ir = y2 - sr * xr2;
+    const double xl = 1 / sl * (yb - il), xr = 1 / sr * (yb - ir);
+    const double L = xr - xl;
+    const double H = (yhi - ylo) + 0.5;
+    const double R = (4 * H * H + L * L) / (8 * H);
+    double theta = std::atan(L / (2 * (R - H))) * (180.0 / M_PI);
+    if (theta < 0) theta += 180;
+    out[0] = theta;
+    out[1] = L;
+    out[2] = H;
+    out[3] = R;
     return SCL_OK;
 }
 

@@ -28,6 +28,7 @@ EXPORTS = [
     "scl_set_tile", "scl_set_lookahead", "scl_get_tile", "scl_autotune", "scl_timing", "scl_probe_copy",
     "scl_set_step_fusion",
     "scl_get_step_fusion",
+    "scl_laplace_observables", "scl_contact_angle",
     # serving layer (bound in serve.py)
     "scl_server_default_config", "scl_server_create", "scl_server_submit", "scl_server_status",
     "scl_server_wait", "scl_server_cancel", "scl_server_result", "scl_server_release", "scl_server_drain",
@@ -84,6 +85,9 @@ def load():
         "scl_step1": [_vp, _vp, _dp, ctypes.c_int],
         "scl_step2": [_vp, _vp, _vp, _dp, _dp, ctypes.c_int],
         "scl_moments": [_vp, _vp, _dp],
+        "scl_laplace_observables": [_vp, _vp, _vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp],
+        "scl_contact_angle": [_vp, _vp, _vp, ctypes.c_double, ctypes.c_double, _dp],
         "scl_set_tile": [_vp, ctypes.c_int, ctypes.c_int],
         "scl_set_lookahead": [_vp, ctypes.c_int],
         "scl_set_step_fusion": [_vp, ctypes.c_int, ctypes.c_int],
@@ -324,3 +328,25 @@ def gpu_available():
     except ImportError:
         return False
     return lib.scl_init(0) == SCL_OK
+
+
+def laplace_observables(geom, droplet, bulk, G, p_in, p_out, rho_bulk, rho_disp):
+    """(pressure difference, radius) of a droplet, on the device: the
+    arithmetic of the reference's laplace_law/compute_surface_tension.py
+    (lines 63-70).  p_in / p_out are (row, column) = (y, x) as the script's
+    p_tot[p_in[0], p_in[1]]."""
+    out = np.zeros(2)
+    check(load().scl_laplace_observables(geom.h, droplet.h, bulk.h, float(G), int(p_in[1]), int(p_in[0]),
+                                         int(p_out[1]), int(p_out[0]), float(rho_bulk), float(rho_disp),
+                                         out.ctypes.data_as(_dp)))
+    return float(out[0]), float(out[1])
+
+
+def contact_angle(geom, droplet, bulk, rho_bulk, rho_disp):
+    """(theta in degrees, base length, height, radius) of a droplet on a solid
+    below it, on the device: the arithmetic of the reference's
+    contact_angles/compute_contact_angles.py (lines 46-100)."""
+    out = np.zeros(4)
+    check(load().scl_contact_angle(geom.h, droplet.h, bulk.h, float(rho_bulk), float(rho_disp),
+                                   out.ctypes.data_as(_dp)))
+    return tuple(float(v) for v in out)

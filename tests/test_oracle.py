@@ -291,3 +291,67 @@ def test_poiseuille_profile_closed_form():
     want = wl.poiseuille_profile(ny, g, tau)
     inner = slice(1, ny - 1)
     assert np.max(np.abs(ux[inner] - want[inner])) < 2e-2 * want.max()
+
+
+# ---- droplet observables (restatement of the reference's post-processing
+#      scripts, oracle/observables.py), pinned by closed forms --------------
+
+from oracle import observables as ob
+
+
+def test_observables_pressure_of_uniform_states():
+    """p = (rho_b + rho_d)/3 + G rho_b rho_d / 3 on uniform states with exact
+    values: 3/3 = 1; (1 + 1)/3 + 3*1*1/3 = 5/3; and the Laplace difference of
+    two such states."""
+    assert ob.pressure(np.array(3.0), np.array(0.0), 0.9) == 1.0
+    assert abs(ob.pressure(np.array(1.0), np.array(1.0), 3.0) - 5.0 / 3.0) < 1e-15
+    n = 16
+    rho_b = np.full((n, n), 1.5)
+    rho_d = np.full((n, n), 0.25)
+    rho_b[4, 5], rho_d[4, 5] = 3.0, 0.0      # p = 1 at (y 4, x 5)
+    rho_b[9, 9], rho_d[9, 9] = 1.0, 1.0      # p = 5/3 at (y 9, x 9) with G = 3
+    dp, _ = ob.laplace(rho_d, rho_b, 3.0, (9, 9), (4, 5), 1.9, 0.06)
+    assert abs(dp - 2.0 / 3.0) < 1e-15
+
+
+def test_observables_radius_of_a_known_set():
+    """The Laplace set (rho_d < rho_b, rho_disp < rho_b <= rho_bulk) placed on
+    a ring of known extents: rows 7..31 and columns 12..40 give
+    R = 0.5 * ((31 - 7)/2 + (40 - 12)/2) = 13; sites failing any one of the
+    three conditions do not count."""
+    ny, nx = 48, 64
+    rho_b = np.full((ny, nx), 2.0)           # bulk: above rho_bulk, not in the set
+    rho_d = np.full((ny, nx), 0.01)
+    y, x = np.mgrid[0:ny, 0:nx]
+    ring = ((y == 7) | (y == 31)) & (x >= 12) & (x <= 40) | ((x == 12) | (x == 40)) & (y >= 7) & (y <= 31)
+    rho_b[ring], rho_d[ring] = 1.0, 0.5
+    rho_b[20, 2], rho_d[20, 2] = 1.0, 1.5    # rho_d > rho_b
+    rho_b[2, 20], rho_d[2, 20] = 0.05, 0.01  # rho_b below rho_disp
+    rho_b[45, 60], rho_d[45, 60] = 1.95, 0.5  # rho_b above rho_bulk
+    _, R = ob.laplace(rho_d, rho_b, 0.9, (20, 26), (2, 2), 1.9, 0.06)
+    assert R == 13.0
+
+
+@pytest.mark.parametrize("h,half_base", [(5, 5), (3, 9), (8, 4), (6, 6)])
+def test_observables_contact_angle_of_a_constructed_cap(h, half_base):
+    """An interface whose lowest row spans x = a..b and the next row
+    a+1..b-1, top row h above: the script extrapolates the base to
+    L = b - a + 1 and the height to H = h + 0.5, and the circle through them
+    meets the base at theta = 2 atan(2H / L) (the cap angle of elementary
+    geometry), 90 degrees for L = 2H."""
+    ny, nx = 40, 64
+    rho_b = np.full((ny, nx), 2.0)
+    rho_d = np.full((ny, nx), 0.01)
+    y0, a = 3, 20
+    b = a + 2 * half_base
+    iface = np.zeros((ny, nx), bool)
+    iface[y0, a:b + 1] = True
+    iface[y0 + 1, a + 1:b] = True
+    iface[y0 + h, (a + b) // 2] = True
+    rho_b[iface], rho_d[iface] = 0.5, 1.0
+    theta, L, H, R = ob.contact_angle(rho_d, rho_b, 2.0, 0.06)
+    assert L == b - a + 1 and H == h + 0.5
+    assert abs(R - (4 * H * H + L * L) / (8 * H)) < 1e-12
+    assert abs(theta - np.degrees(2 * np.arctan(2 * H / L))) < 1e-9
+    if L == 2 * H:
+        assert theta == 90.0
