@@ -105,6 +105,19 @@ int scl_field_set_interactions(scl_field* f, double g_solid, double g_fluid);
 int scl_field_upload(scl_field* f, int which, const double* src, size_t n);
 int scl_field_download(scl_field* f, int which, double* dst, size_t n);
 
+/* Exact checkpoint of a field (no reference counterpart: the reference's
+ * Fluid::restart_from_file re-reads 6-digit text, include/fluid.h:249-250,
+ * src/fluid.cpp:285-300).  It holds the populations as stored (any form),
+ * their phase and the equilibrium velocities of the zero-density rule, so a
+ * field restored from it continues bit for bit as the original would.  The
+ * fluid-solid force is not part of it: call scl_solid_surface_force on the
+ * restored fields as after creation.  scl_field_restore needs a field of the
+ * same size, precision and rho0 (SCL_ERR_ARG otherwise); bytes must equal
+ * scl_field_checkpoint_size. */
+int scl_field_checkpoint_size(const scl_field* f, size_t* bytes);
+int scl_field_checkpoint(scl_field* f, void* dst, size_t bytes);
+int scl_field_restore(scl_field* f, const void* src, size_t bytes);
+
 /* ---- Per-operator entry points, reference semantics (SCL_FP64 only). --- */
 
 /* Fluid::compute_density, src/fluid.cpp:175-183 */

@@ -15,6 +15,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
+#include <cstdint>
 #include <cstring>
 #include <memory>
 #include <type_traits>
@@ -1106,6 +1107,86 @@ int scl_field_download(scl_field* f, int which, double* dst, size_t n) {
     }
     CHK(materialize(f));
     CHK(copy_planes(g, aux_ptr(f, which), dst, which == SCL_FEQ ? 9 : 1, false));
+    HIPCHK(hipStreamSynchronize(g_lib.stream));
+    return SCL_OK;
+}
+
+// ---- exact checkpoint -------------------------------------------------------
+
+namespace {
+// Checkpoint layout: this header, the populations as stored on the device
+// (9 pitched planes in the storage form), then the equilibrium velocities
+// the next step reads (x, then y; pitched FP64).
+struct CkptHeader {
+    char magic[8];
+    int32_t precision, nx, ny, phase;
+    double rho0;
+    uint64_t pop_bytes, ueq_bytes;
+};
+constexpr char kCkptMagic[8] = {'S', 'C', 'L', 'C', 'K', 'P', 'T', '1'};
+
+size_t ckpt_pop_bytes(const scl_field* f) { return n_pop(f->g) * f->esize(); }
+size_t ckpt_ueq_bytes(const scl_field* f) { return 2 * n_sites(f->g) * sizeof(double); }
+size_t ckpt_bytes(const scl_field* f) { return sizeof(CkptHeader) + ckpt_pop_bytes(f) + ckpt_ueq_bytes(f); }
+}  // namespace
+
+int scl_field_checkpoint_size(const scl_field* f, size_t* bytes) {
+    if (!f || !bytes) return fail(SCL_ERR_ARG, "null argument");
+    *bytes = ckpt_bytes(f);
+    return SCL_OK;
+}
+
+int scl_field_checkpoint(scl_field* f, void* dst, size_t bytes) {
+    if (!f || !dst) return fail(SCL_ERR_ARG, "null argument");
+    if (bytes != ckpt_bytes(f)) return fail(SCL_ERR_ARG, "checkpoint needs %zu bytes, got %zu", ckpt_bytes(f), bytes);
+    CHK(materialize(f));
+    // post-collision populations of another mask: stream them with it first
+    if (f->phase == PH_P && f->pg && f->pg != f->g) CHK(ensure_s(f));
+    CkptHeader h{};
+    std::memcpy(h.magic, kCkptMagic, sizeof h.magic);
+    h.precision = f->precision;
+    h.nx = f->g->nx;
+    h.ny = f->g->ny;
+    h.phase = f->phase;
+    h.rho0 = f->rho0;
+    h.pop_bytes = ckpt_pop_bytes(f);
+    h.ueq_bytes = ckpt_ueq_bytes(f);
+    char* out = (char*)dst;
+    std::memcpy(out, &h, sizeof h);
+    out += sizeof h;
+    HIPCHK(hipMemcpyAsync(out, f->f[f->cur], h.pop_bytes, hipMemcpyDeviceToHost, g_lib.stream));
+    out += h.pop_bytes;
+    const size_t half = h.ueq_bytes / 2;
+    HIPCHK(hipMemcpyAsync(out, f->ueqx, half, hipMemcpyDeviceToHost, g_lib.stream));
+    HIPCHK(hipMemcpyAsync(out + half, f->ueqy, half, hipMemcpyDeviceToHost, g_lib.stream));
+    HIPCHK(hipStreamSynchronize(g_lib.stream));
+    return SCL_OK;
+}
+
+int scl_field_restore(scl_field* f, const void* src, size_t bytes) {
+    if (!f || !src) return fail(SCL_ERR_ARG, "null argument");
+    if (bytes != ckpt_bytes(f)) return fail(SCL_ERR_ARG, "checkpoint of %zu bytes for a field that needs %zu", bytes,
+                                            ckpt_bytes(f));
+    CkptHeader h;
+    std::memcpy(&h, src, sizeof h);
+    if (std::memcmp(h.magic, kCkptMagic, sizeof h.magic) != 0) return fail(SCL_ERR_ARG, "not a field checkpoint");
+    if (h.precision != f->precision || h.nx != f->g->nx || h.ny != f->g->ny)
+        return fail(SCL_ERR_ARG, "checkpoint of a %dx%d field of precision %d, field is %dx%d precision %d", h.nx, h.ny,
+                    h.precision, f->g->nx, f->g->ny, f->precision);
+    if (std::memcmp(&h.rho0, &f->rho0, sizeof(double)) != 0)
+        return fail(SCL_ERR_ARG, "checkpoint deviations are about rho0 = %.17g, the field's about %.17g", h.rho0,
+                    f->rho0);
+    if (h.pop_bytes != ckpt_pop_bytes(f) || h.ueq_bytes != ckpt_ueq_bytes(f) || (h.phase != PH_S && h.phase != PH_P))
+        return fail(SCL_ERR_ARG, "corrupt checkpoint header");
+    CHK(materialize(f));
+    const char* in = (const char*)src + sizeof h;
+    HIPCHK(hipMemcpyAsync(f->f[f->cur], in, h.pop_bytes, hipMemcpyHostToDevice, g_lib.stream));
+    in += h.pop_bytes;
+    const size_t half = h.ueq_bytes / 2;
+    HIPCHK(hipMemcpyAsync(f->ueqx, in, half, hipMemcpyHostToDevice, g_lib.stream));
+    HIPCHK(hipMemcpyAsync(f->ueqy, in + half, half, hipMemcpyHostToDevice, g_lib.stream));
+    f->phase = h.phase;
+    if (f->phase == PH_P) set_pgeom(f, f->g);
     HIPCHK(hipStreamSynchronize(g_lib.stream));
     return SCL_OK;
 }

@@ -29,6 +29,7 @@ EXPORTS = [
     "scl_set_step_fusion",
     "scl_get_step_fusion",
     "scl_laplace_observables", "scl_contact_angle",
+    "scl_field_checkpoint_size", "scl_field_checkpoint", "scl_field_restore",
     # serving layer (bound in serve.py)
     "scl_server_default_config", "scl_server_create", "scl_server_submit", "scl_server_status",
     "scl_server_wait", "scl_server_cancel", "scl_server_result", "scl_server_release", "scl_server_drain",
@@ -88,6 +89,9 @@ def load():
         "scl_laplace_observables": [_vp, _vp, _vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp],
         "scl_contact_angle": [_vp, _vp, _vp, ctypes.c_double, ctypes.c_double, _dp],
+        "scl_field_checkpoint_size": [_vp, ctypes.POINTER(ctypes.c_size_t)],
+        "scl_field_checkpoint": [_vp, ctypes.c_void_p, ctypes.c_size_t],
+        "scl_field_restore": [_vp, ctypes.c_void_p, ctypes.c_size_t],
         "scl_set_tile": [_vp, ctypes.c_int, ctypes.c_int],
         "scl_set_lookahead": [_vp, ctypes.c_int],
         "scl_set_step_fusion": [_vp, ctypes.c_int, ctypes.c_int],
@@ -226,6 +230,21 @@ class Field:
 
     def get_f(self):
         return self.download(F)
+
+    def checkpoint(self):
+        """Exact checkpoint of the field (populations as stored, their phase,
+        the zero-density rule's equilibrium velocities) as bytes."""
+        n = ctypes.c_size_t()
+        check(load().scl_field_checkpoint_size(self.h, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        check(load().scl_field_checkpoint(self.h, buf, n.value))
+        return buf.raw
+
+    def restore(self, data):
+        """Continue from a checkpoint of a field of the same size, precision
+        and rho0."""
+        buf = ctypes.create_string_buffer(bytes(data), len(data))
+        check(load().scl_field_restore(self.h, buf, len(data)))
 
     def moments(self):
         out = np.zeros(3)

@@ -336,3 +336,47 @@ def test_two_steps_per_launch_equal_single_steps(prec, name):
         assert np.all(np.isfinite(out[0][c]))
         assert np.array_equal(out[0][c], out[1][c])
         assert np.array_equal(out[0][c], out[2][c])
+
+
+@pytest.mark.parametrize("prec", [nat.FP64, nat.FP32_DEV, nat.FP16_DEV])
+@pytest.mark.parametrize("name", ["porous", "dry"])
+def test_checkpoint_restore_continues_bit_for_bit(prec, name):
+    """A checkpoint taken after 31 steps (post-collision phase, with solids,
+    adhesion, a force and dry sites) restored into fresh fields continues
+    bit for bit as the original: 26 more steps, in chunks that mix the
+    multi-step kernels with single steps."""
+    case = wl.porous(nx=300, ny=90, porosity=0.75, seed=3, rmin=5, rmax=10) if name == "porous" else _dry_case()
+
+    def fields(g):
+        fl = [nat.Field(g, case["tau"][c], prec, 1.0, g_solid=case["g_solid"][c], g_fluid=case["g_fluid"][c])
+              for c in range(2)]
+        nat.solid_surface_force(g, fl[0], fl[1])
+        return fl
+
+    g = nat.Geometry(case["mask"])
+    fl = fields(g)
+    for c in range(2):
+        fl[c].set_f(case["f"][c])
+    nat.step2(g, fl[0], fl[1], case["force"][0], case["force"][1], 31)
+    ck = [f.checkpoint() for f in fl]
+    for n in (7, 19):
+        nat.step2(g, fl[0], fl[1], case["force"][0], case["force"][1], n)
+    want = [f.get_f() for f in fl]
+    g2 = nat.Geometry(case["mask"])
+    fl2 = fields(g2)
+    for c in range(2):
+        fl2[c].restore(ck[c])
+    for n in (7, 19):
+        nat.step2(g2, fl2[0], fl2[1], case["force"][0], case["force"][1], n)
+    for c in range(2):
+        got = fl2[c].get_f()
+        assert np.all(np.isfinite(got))
+        assert np.array_equal(got.view(np.uint64), want[c].view(np.uint64))
+    # a checkpoint refuses a field of another precision or size
+    other = nat.Field(g2, 1.0, nat.FP32_DEV if prec != nat.FP32_DEV else nat.FP64, 1.0)
+    with pytest.raises(nat.SclError):
+        other.restore(ck[0])
+    for f in fl + fl2 + [other]:
+        f.close()
+    g.close()
+    g2.close()
