@@ -380,3 +380,37 @@ def test_checkpoint_restore_continues_bit_for_bit(prec, name):
         f.close()
     g.close()
     g2.close()
+
+
+@pytest.mark.parametrize("prec", [nat.FP32_DEV, nat.FP16_DEV])
+@pytest.mark.parametrize("shape", [(300, 5), (256, 40), (257, 33)])
+def test_multi_step_kernels_on_edge_shapes(prec, shape):
+    """Lattices where the multi-step kernels' halos wrap: 5 rows (the
+    three-step band halo of 9 rows wraps the lattice more than once), 256
+    columns (one two-step strip exactly) and 257 (a strip of one column).
+    Solids and adhesion included; fusion 1, 2 and 3 give the same bits."""
+    nx, ny = shape
+    rng = np.random.default_rng(nx * 1000 + ny)
+    m = np.ones((ny, nx), dtype=np.int32)
+    m[rng.random((ny, nx)) < 0.05] = 0
+    ra = 1.0 + 0.02 * rng.standard_normal((ny, nx))
+    rb = np.where(rng.random((ny, nx)) < 0.5, 0.3, 0.9)
+    out = []
+    for fusion in (1, 2, 3):
+        g = nat.Geometry(m)
+        g.set_step_fusion(fusion, 0)
+        assert g.step_fusion(prec)[0] == fusion
+        fl = [nat.Field(g, (1.0, 0.8)[c], prec, 1.0, g_solid=(0.1, -0.1)[c], g_fluid=0.9) for c in range(2)]
+        fl[0].set_f(wl.f_from_rho(ra, m))
+        fl[1].set_f(wl.f_from_rho(rb, m))
+        nat.solid_surface_force(g, fl[0], fl[1])
+        for n in (1, 8, 6, 5):
+            nat.step2(g, fl[0], fl[1], None, None, n)
+        out.append([f.get_f() for f in fl])
+        for f in fl:
+            f.close()
+        g.close()
+    for c in range(2):
+        assert np.all(np.isfinite(out[0][c]))
+        assert np.array_equal(out[0][c], out[1][c])
+        assert np.array_equal(out[0][c], out[2][c])
