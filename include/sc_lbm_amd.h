@@ -326,8 +326,9 @@ typedef struct scl_server_config {
     double mass_tol;            /* watchdog: |mass - mass0| > mass_tol*|mass0| fails a job;
                                    0 = by precision (1e-6 FP64, 1e-3 FP32, 0.1 FP16) */
     int max_bypass;             /* later jobs admitted past a waiting one before it blocks admission */
-    int steps_per_launch;       /* 0 = automatic: with FP32 storage, jobs at least 256 columns wide
-                                   advance two steps per launch (same bits as one); 1 = one */
+    int steps_per_launch;       /* jobs at least 256 columns wide advance 2 steps per launch
+                                   (same bits as 1) with a deviation form; 0 = automatic: 2 with
+                                   FP32 storage, 1 with FP16 (two measured slower there) */
     int band_rows;              /* rows per band of the two-step tiles; 0 = 32            */
 } scl_server_config;
 

@@ -181,10 +181,9 @@ class GpuServe final : public serve::Backend {
             *err = "steps_per_launch is 0, 1 or 2 and band_rows 0..4096";
             return SCL_ERR_ARG;
         }
-        if constexpr (std::is_same<ST, StoreF32>::value) {
-            // two steps per launch: FP32 storage only (with FP16 storage the
-            // general two-step variant is slower than single steps)
-            x2_ = cfg_.steps_per_launch != 1;
+        if constexpr (ST::kDev) {
+            // two steps per launch: the deviation forms (automatic: FP32)
+            x2_ = cfg_.steps_per_launch == 2 || (cfg_.steps_per_launch == 0 && std::is_same<ST, StoreF32>::value);
             x2rows_ = cfg_.band_rows ? cfg_.band_rows : 32;
             int per_cu = 0;
             SHIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
@@ -552,7 +551,7 @@ class GpuServe final : public serve::Backend {
     }
 
     void launch_pair(int q, bool force) {
-        if constexpr (std::is_same<ST, StoreF32>::value) {
+        if constexpr (ST::kDev) {
             if (force)
                 hipLaunchKernelGGL((k_serve2x<ST, true>), dim3(grid2_), dim3(kX2Threads), 0, stream_, hdr_d_, q,
                                    ctr_d_);
@@ -690,7 +689,7 @@ class GpuServe final : public serve::Backend {
     double* mass_d_ = nullptr;
     double* mass_h_ = nullptr;
     int nw_ = 4, rows_ = 32, la_ = 1, grid_ = 0, mass_grid_ = 1024;
-    bool x2_ = false;                  // two steps per launch for wide jobs (FP32 storage)
+    bool x2_ = false;                  // two steps per launch for wide jobs (deviation forms)
     int x2rows_ = 32, grid2_ = 0;      // band rows of the two-step tiles; its persistent grid
     uint64_t* items2_d_ = nullptr;
     uint64_t* items2_h_ = nullptr;

@@ -215,19 +215,20 @@ def test_autotuned_server_runs_and_reports_its_shape():
         assert np.array_equal(tuned[c], fixed[c])
 
 
-def test_two_steps_per_launch_in_the_server_keep_results():
-    """FP32 storage: jobs at least 256 columns wide advance two steps per
+@pytest.mark.parametrize("form", ["fp32dev", "fp16dev"])
+def test_two_steps_per_launch_in_the_server_keep_results(form):
+    """Deviation forms: jobs at least 256 columns wide advance two steps per
     launch of the persistent kernel; an odd step count, a round shorter than
     two steps and narrow jobs take single steps.  The result is the same bits
     with one step per launch, in a mix, and on the single-lattice path."""
     wide = dict(POROUS, nx=300, ny=120, steps=151, geom_seed=4, ic_seed=6, deliver=serve.DELIVER_F)
-    two = run_alone(wide, "fp32dev", round_steps=16)
-    one = run_alone(wide, "fp32dev", round_steps=16, steps_per_launch=1)
-    odd = run_alone(wide, "fp32dev", round_steps=5, band_rows=17)
+    two = run_alone(wide, form, round_steps=16, steps_per_launch=2)
+    one = run_alone(wide, form, round_steps=16, steps_per_launch=1)
+    odd = run_alone(wide, form, round_steps=5, band_rows=17, steps_per_launch=2)
     for c in range(2):
         assert np.array_equal(two[c], one[c])
         assert np.array_equal(two[c], odd[c])
-    with gpu_server("fp32dev") as srv:
+    with gpu_server(form, steps_per_launch=2) as srv:
         assert srv.stats()["steps_per_launch"] == 2
         t0 = srv.stats()["clock_ms"]
         others = [dict(nx=257, ny=64, steps=90, ic_seed=3, radius=12, **DROP),
@@ -244,7 +245,7 @@ def test_two_steps_per_launch_in_the_server_keep_results():
         assert np.array_equal(two[c], mixed[c])
     case = wl.served_case(**{k: v for k, v in wide.items() if k not in ("steps", "deliver")})
     g = nat.Geometry(case["mask"])
-    fl = [nat.Field(g, case["tau"][c], nat.FP32_DEV, served_rho0(wide), g_solid=case["g_solid"][c],
+    fl = [nat.Field(g, case["tau"][c], nat.PRECISION[form], served_rho0(wide), g_solid=case["g_solid"][c],
                     g_fluid=case["g_fluid"][c]) for c in range(2)]
     for c in range(2):
         fl[c].set_f(case["f"][c])
