@@ -29,9 +29,9 @@
 //
 // Zero-density rule: stage s reads the equilibrium velocities of the step
 // before from one buffer set and writes the next, which stage s + 1 reads;
-// the K + 1 sets of a launch are distinct (K < kUeqRing), so no buffer is
-// read by one workgroup while another writes it (columns computed by two
-// strips write identical values).
+// the K + 1 sets of a launch are distinct (K < the ring's set count), so no
+// buffer is read by one workgroup while another writes it (columns computed
+// by two strips write identical values).
 #pragma once
 #include "step2x.h"
 
