@@ -7,12 +7,16 @@ synthetic and resident in HBM before the timed region.  The headline value
 is the FP32-deviation form; the FP64 form (bit-exact with the reference) and
 the FP16-deviation storage form are timed beside it.  The deviation forms
 advance three (k_stepk) or two (k_step2x) steps per launch, as the
-autotuner measures fastest, FP64 one (k_step).  The roofline object is per
-launch of the dominant kernel: algorithmic bytes (every population of both
-components read once and written once, 144 B per site in FP32) over the
-HIP-event average launch duration; `effective_pct_hbm` is MLUPS x those
-bytes per site over the peak, the share a one-step-per-launch kernel would
-need.
+autotuner measures fastest, FP64 one (k_step).  The roofline object is for
+the dominant kernel: `achieved` = the algorithmic bytes of the lattice
+updates it performs (every population of both components read once and
+written once per update, 144 B per site and step in FP32) over its
+HIP-event time, i.e. kernel MLUPS x 144 B; a launch that fuses K steps
+performs K updates per site while moving the populations once, so this
+exceeds the one-step HBM roofline.  `launch_pass_gbs` (one pass over the
+populations per launch) and `hbm_moved_gbs` (measured traffic per launch)
+give the bytes it actually moves.  `effective_pct_hbm` is the bench's MLUPS
+x those bytes per update over the peak.
 
 Prints one JSON line (rank 0, N = 1).
 """
@@ -93,12 +97,16 @@ def time_form(nat, wl, case, form, steps, warmup, tile, fusion=""):
     n_sites = case["mask"].size
     mlups = n_sites * steps / dt / 1e6
     launch_ms = avg_ms * steps / cnt if cnt else 0.0
-    # one launch reads and writes every population once, whether it advances
-    # one step (k_step), two (k_step2x) or three (k_stepk)
+    # achieved: the algorithmic bytes of the lattice updates the kernel
+    # performs (BYTES_PER_LU per site and step) over its time, i.e. kernel
+    # MLUPS x bytes per update; launch_pass_gbs: one read and one write of
+    # every population, what a launch moves whatever the steps it fuses (one:
+    # k_step, two: k_step2x, three: k_stepk), over the launch time
     return dict(form=form, seconds=dt, ms_per_step=dt * 1e3 / steps, mlups=mlups,
                 kernel={1: "k_step", 2: "k_step2x", 3: "k_stepk"}[spl], steps_per_launch=spl, band_rows=band,
-                kernel_avg_ms=launch_ms, kernel_launches=cnt, tile=list(tile_used),
-                achieved_gbs=n_sites * BYTES_PER_LU[form] / (launch_ms * 1e-3) / 1e9 if launch_ms > 0 else None,
+                kernel_avg_ms=launch_ms, kernel_step_ms=avg_ms, kernel_launches=cnt, tile=list(tile_used),
+                achieved_gbs=n_sites * BYTES_PER_LU[form] / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else None,
+                launch_pass_gbs=n_sites * BYTES_PER_LU[form] / (launch_ms * 1e-3) / 1e9 if launch_ms > 0 else None,
                 effective_pct_hbm=100.0 * mlups * 1e6 * BYTES_PER_LU[form] / (HBM_PEAK_GBS * 1e9), mass=mass)
 
 
@@ -241,9 +249,9 @@ def main():
     for form in a.forms.split(","):
         log("timing %s: %d warmup + %d steps" % (form, a.warmup, a.steps))
         res[form] = time_form(nat, wl, case, form, a.steps, a.warmup, a.tile, a.fusion)
-        log("  %s: %.1f MLUPS, %s %.4f ms per launch (%d steps), %.0f GB/s algorithmic"
+        log("  %s: %.1f MLUPS, %s %.4f ms per launch (%d steps), %.0f GB/s algorithmic (%.0f GB/s per launch pass)"
             % (form, res[form]["mlups"], res[form]["kernel"], res[form]["kernel_avg_ms"],
-               res[form]["steps_per_launch"], res[form]["achieved_gbs"] or 0.0))
+               res[form]["steps_per_launch"], res[form]["achieved_gbs"] or 0.0, res[form]["launch_pass_gbs"] or 0.0))
     head = res.get("fp32dev") or next(iter(res.values()))
     traffic = valu_busy = None
     tpath = os.path.join(ROOT, "profiles", "traffic_%s_%s_%d.json" % (head["kernel"], head["form"], a.n))
@@ -271,10 +279,19 @@ def main():
                    "tile": head["tile"], "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
+                     "bytes_per_update": BYTES_PER_LU[head["form"]],
+                     "launch_pass_gbs": head["launch_pass_gbs"],
+                     "hbm_moved_gbs": traffic / (head["kernel_avg_ms"] * 1e-3) / 1e9 if traffic else None,
+                     "note": "achieved = the kernel's lattice updates x %d B (read and write every population "
+                             "once per step) over its time; a launch fusing %d steps performs %d updates per "
+                             "site but moves the populations once (launch_pass_gbs, hbm_moved_gbs = measured "
+                             "traffic per launch over the launch time)"
+                             % (BYTES_PER_LU[head["form"]], head["steps_per_launch"], head["steps_per_launch"]),
                      "kernel": head["kernel"], "steps_per_launch": head["steps_per_launch"],
                      "valu_busy_pct": valu_busy},
         "forms": {k: {kk: v[kk] for kk in ("mlups", "effective_pct_hbm", "kernel", "steps_per_launch",
-                                           "band_rows", "kernel_avg_ms", "achieved_gbs", "tile")}
+                                           "band_rows", "kernel_avg_ms", "achieved_gbs", "launch_pass_gbs",
+                                           "tile")}
                   for k, v in res.items()},
     }
     if "fp64" in res and "fp32dev" in res:
