@@ -354,7 +354,8 @@ class GpuServe final : public serve::Backend {
     // Launches that swap a job's buffers in a round of `steps` steps.
     static int launches(int steps, bool two) { return two ? steps / 2 + (steps & 1) : steps; }
 
-    static constexpr int kOut2 = x2_out<StoreF32>();
+    // output columns of a two-step strip in this storage form (the tile's)
+    static constexpr int kOut2 = x2_out<ST>();
     static size_t tiles2_of(int nx, int ny, int rows) {
         return (size_t)((nx + kOut2 - 1) / kOut2) * (size_t)((ny + rows - 1) / rows);
     }
