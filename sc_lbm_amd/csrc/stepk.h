@@ -1,6 +1,6 @@
 // stepk.h -- K fused time steps per launch for the two-component deviation
-// forms (FP32 arithmetic; FP32 or FP16 storage), the generalisation of
-// step2x.h to K stages.
+// forms (FP32 arithmetic; FP32 or FP16 storage): K = 2 (k_step2x, and the
+// server's k_serve2x) and K = 3 (k_stepk).
 //
 // A launch reads P_{n-1} and writes P_{n+K-1}; the post-collision
 // populations of the K-1 intermediate steps stay in LDS, so a population
@@ -358,12 +358,10 @@ __global__ __launch_bounds__(T) void k_stepk(const XkArgs<K> A) {
     stepk_grid<ST, OPEN, K, T, FORCE>(A, L);
 }
 
-// Two steps per launch on a field: the K = 2 tile, with the strips of
-// step2x_tile (x2_out columns).  The server's persistent kernel keeps
-// step2x_tile, whose equilibrium-velocity buffers come per job; here the
-// buffer sets of the ring take 8 scalar registers instead of 32, which spared
-// the general (solid) variant most of its scalar spills (9% fewer vector
-// instructions).
+// Two steps per launch on a field: the K = 2 tile, with strips of x2_out
+// columns.  (The ring of buffer sets takes 8 scalar registers where
+// separate pointers for the two steps took 32, which spared the general,
+// solid, variant most of its scalar spills: 9% fewer vector instructions.)
 static_assert(xk_out<StoreF32, 2, kX2Threads>() == x2_out<StoreF32>() &&
                   xk_out<StoreF16, 2, kX2Threads>() == x2_out<StoreF16>(),
               "two-step strips");
