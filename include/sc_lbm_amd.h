@@ -276,6 +276,7 @@ typedef struct scl_server scl_server;
 #define SCL_JOB_FAILED 3     /* setup failed, or the mass watchdog tripped            */
 #define SCL_JOB_CANCELLED 4
 #define SCL_JOB_TIMED_OUT 5
+#define SCL_JOB_SUSPENDED 6  /* pre-empted: exact state held in host memory           */
 
 typedef struct scl_job {
     int nx, ny;                 /* lattice, 1..32767 each                             */
@@ -364,6 +365,18 @@ int scl_server_status(scl_server* s, long long id, scl_job_status* st);
 int scl_server_wait(scl_server* s, long long id, double timeout_ms, scl_job_status* st);
 /* Queued jobs are dropped at once, resident ones at the next round boundary. */
 int scl_server_cancel(scl_server* s, long long id);
+/* Pre-emption (no reference counterpart): a queued job is set aside at once;
+ * a resident one at the next round boundary, where its exact state (the
+ * populations as stored and the zero-density rule's equilibrium velocities)
+ * is copied to host memory and its device memory freed.  The job then waits
+ * in SCL_JOB_SUSPENDED (scl_server_wait keeps waiting, scl_server_drain does
+ * not count it) until scl_server_resume queues it again in its release
+ * order; it continues bit for bit as if never interrupted.
+ * Suspending a suspended job does nothing; SCL_ERR_ARG for a finished job
+ * or, from resume, for a job that is not suspended (a resident job whose
+ * suspension is still pending just keeps running). */
+int scl_server_suspend(scl_server* s, long long id);
+int scl_server_resume(scl_server* s, long long id);
 /* Copies a deliverable of a finished job: which = SCL_RHO (nx*ny) or SCL_F
  * (9*nx*ny), component 0 or 1. */
 int scl_server_result(scl_server* s, long long id, int which, int comp, double* out, size_t n);
@@ -372,7 +385,8 @@ int scl_server_release(scl_server* s, long long id);
 /* Waits until no job is queued or resident. */
 int scl_server_drain(scl_server* s, double timeout_ms);
 int scl_server_get_stats(scl_server* s, scl_server_stats* st);
-/* Stops the loop (queued and resident jobs end CANCELLED) and frees the arena. */
+/* Stops the loop (queued, resident and suspended jobs end CANCELLED) and
+ * frees the arena. */
 int scl_server_destroy(scl_server* s);
 
 /* Host helper (no device needed): the mask a job would use, nx*ny ints. */

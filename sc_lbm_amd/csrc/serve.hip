@@ -462,6 +462,25 @@ class GpuServe final : public serve::Backend {
         }
         SHIP(hipMemsetAsync(base, 0, L.total, stream_));
         SHIP(hipMemcpyAsync(base + L.mask, j.bits.data(), j.bits.size() * 8, hipMemcpyHostToDevice, stream_));
+        if (!j.saved.empty()) {   // a resumed job: its saved state (save) into buffer 0 and set 0
+            if (j.saved.size() != saved_bytes(L)) {
+                *err = "saved state does not match the lattice";
+                return SCL_ERR_ARG;
+            }
+            const char* in = j.saved.data();
+            const size_t pop = (size_t)Q * L.plane * sizeof(S), ueq = (size_t)L.plane * 8;
+            for (int c = 0; c < 2; ++c, in += pop)
+                SHIP(hipMemcpyAsync(g->f(c, 0), in, pop, hipMemcpyHostToDevice, stream_));
+            for (int c = 0; c < 2; ++c)
+                for (int d = 0; d < 2; ++d, in += ueq)
+                    SHIP(hipMemcpyAsync(g->ueq(0, c, d), in, ueq, hipMemcpyHostToDevice, stream_));
+            SHIP(hipStreamSynchronize(stream_));
+            j.saved.clear();
+            g->cur = 0;
+            g->ucur = 0;
+            j.dev = g.release();
+            return SCL_OK;
+        }
         const Grid2 gr = g->grid();
         const dim3 sg = site_grid(s.nx, s.ny);
         for (int c = 0; c < 2; ++c) {
@@ -482,6 +501,29 @@ class GpuServe final : public serve::Backend {
         SHIP(hipStreamSynchronize(stream_));   // host vectors above must outlive the copies
         g->cur = 1;
         j.dev = g.release();
+        return SCL_OK;
+    }
+
+    // Pre-emption: the populations of buffer cur (post-collision, as stored)
+    // and the equilibrium-velocity set the next step reads, per component.
+    static size_t saved_bytes(const Layout& L) {
+        return 2 * (size_t)Q * L.plane * sizeof(S) + 4 * (size_t)L.plane * 8;
+    }
+    bool can_save() const override { return true; }
+    int save(Job& j, std::vector<char>* out, std::string* err) override {
+        SHIP(hipSetDevice(dev_));
+        const GJob* g = (const GJob*)j.dev;
+        const Layout& L = g->L;
+        const size_t pop = (size_t)Q * L.plane * sizeof(S), ueq = (size_t)L.plane * 8;
+        const int k = L.nueq > 1 ? g->ucur : 0;
+        out->resize(saved_bytes(L));
+        char* o = out->data();
+        for (int c = 0; c < 2; ++c, o += pop)
+            SHIP(hipMemcpyAsync(o, g->f(c, g->cur), pop, hipMemcpyDeviceToHost, stream_));
+        for (int c = 0; c < 2; ++c)
+            for (int d = 0; d < 2; ++d, o += ueq)
+                SHIP(hipMemcpyAsync(o, g->ueq(k, c, d), ueq, hipMemcpyDeviceToHost, stream_));
+        SHIP(hipStreamSynchronize(stream_));
         return SCL_OK;
     }
 

@@ -423,11 +423,7 @@ int Scheduler::submit(const scl_job* job, long long* id) {
     j->id = next_id_++;
     j->submit_ms = now_ms();
     Job* p = j.get();
-    auto pos = std::upper_bound(pending_.begin(), pending_.end(), p, [](const Job* a, const Job* b) {
-        return a->spec.release_ms < b->spec.release_ms ||
-               (a->spec.release_ms == b->spec.release_ms && a->id < b->id);
-    });
-    pending_.insert(pos, p);
+    enqueue_locked(p);
     jobs_[p->id] = std::move(j);
     prep_q_.emplace(p->spec.release_ms, p->id);
     ++n_submitted_;
@@ -435,6 +431,15 @@ int Scheduler::submit(const scl_job* job, long long* id) {
     cv_prep_.notify_one();
     cv_work_.notify_one();
     return SCL_OK;
+}
+
+// Must hold mu_.  Into the queue in (release, id) order.
+void Scheduler::enqueue_locked(Job* p) {
+    auto pos = std::upper_bound(pending_.begin(), pending_.end(), p, [](const Job* a, const Job* b) {
+        return a->spec.release_ms < b->spec.release_ms ||
+               (a->spec.release_ms == b->spec.release_ms && a->id < b->id);
+    });
+    pending_.insert(pos, p);
 }
 
 void Scheduler::fill_status(const Job& j, scl_job_status* st) const {
@@ -485,6 +490,11 @@ int Scheduler::cancel(long long id) {
     Job* j = it->second.get();
     if (finished(j->state)) return SCL_OK;
     j->cancel = true;
+    if (j->state == SCL_JOB_SUSPENDED) {
+        j->saved.clear();
+        retire(j, SCL_JOB_CANCELLED, "cancelled while suspended");
+        return SCL_OK;
+    }
     auto pos = std::find(pending_.begin(), pending_.end(), j);
     if (pos != pending_.end()) {
         pending_.erase(pos);
@@ -493,6 +503,45 @@ int Scheduler::cancel(long long id) {
         ++n_cancelled_;
         cv_done_.notify_all();
     }
+    cv_work_.notify_one();
+    return SCL_OK;
+}
+
+int Scheduler::suspend(long long id) {
+    std::lock_guard<std::mutex> lk(mu_);
+    auto it = jobs_.find(id);
+    if (it == jobs_.end()) return report(SCL_ERR_ARG, "unknown job id " + std::to_string(id));
+    Job* j = it->second.get();
+    if (finished(j->state)) return report(SCL_ERR_ARG, "job " + std::to_string(id) + " has finished");
+    if (j->state == SCL_JOB_SUSPENDED) return SCL_OK;
+    if (!be_->can_save()) return report(SCL_ERR_UNSUPPORTED, "this server's backend cannot pre-empt jobs");
+    auto pos = std::find(pending_.begin(), pending_.end(), j);
+    if (pos != pending_.end()) {   // queued: set aside now (a resumed job keeps its saved state)
+        pending_.erase(pos);
+        j->state = SCL_JOB_SUSPENDED;
+        cv_done_.notify_all();
+        return SCL_OK;
+    }
+    j->suspend = true;   // resident or being set up: at the next round boundary
+    cv_work_.notify_one();
+    return SCL_OK;
+}
+
+int Scheduler::resume(long long id) {
+    std::lock_guard<std::mutex> lk(mu_);
+    auto it = jobs_.find(id);
+    if (it == jobs_.end()) return report(SCL_ERR_ARG, "unknown job id " + std::to_string(id));
+    Job* j = it->second.get();
+    if (j->state != SCL_JOB_SUSPENDED) {
+        if (j->suspend && !finished(j->state)) {   // not yet off the device: keep running
+            j->suspend = false;
+            return SCL_OK;
+        }
+        return report(SCL_ERR_ARG, "job " + std::to_string(id) + " is not suspended");
+    }
+    if (stop_) return report(SCL_ERR_ARG, "server is shutting down");
+    j->state = SCL_JOB_QUEUED;
+    enqueue_locked(j);
     cv_work_.notify_one();
     return SCL_OK;
 }
@@ -633,15 +682,17 @@ void Scheduler::loop() {
             }
         }
         std::vector<std::pair<Job*, int>> evict;
+        std::vector<Job*> suspends;
         for (Job* j : active_) {
             if (j->cancel) evict.emplace_back(j, SCL_JOB_CANCELLED);
             else if (j->spec.timeout_ms > 0 && now > j->spec.release_ms + j->spec.timeout_ms)
                 evict.emplace_back(j, SCL_JOB_TIMED_OUT);
+            else if (j->suspend) suspends.push_back(j);
         }
         // 2. admission
         std::vector<Job*> admitted;
         admit_locked(now, &admitted);
-        if (admitted.empty() && active_.empty() && evict.empty()) {
+        if (admitted.empty() && active_.empty() && evict.empty() && suspends.empty()) {
             cv_done_.notify_all();   // drain() waiters: nothing resident
             // sleep until the next prepared job is released, or until a
             // submission, a finished preparation, a cancel or shutdown
@@ -662,6 +713,16 @@ void Scheduler::loop() {
         }
         lk.unlock();
         for (auto& e : evict) be_->drop(*e.first);
+        // pre-emption: the exact state to host memory, then off the device
+        std::vector<std::pair<Job*, std::string>> suspend_failed;
+        for (Job* j : suspends) {
+            std::string err;
+            std::vector<char> st;
+            const int rc = be_->save(*j, &st, &err);
+            be_->drop(*j);
+            if (rc != SCL_OK) suspend_failed.emplace_back(j, "suspension failed: " + err);
+            else j->saved.swap(st);
+        }
         // 3. new lattices
         std::vector<std::pair<Job*, std::string>> setup_failed;
         std::vector<Job*> ready;
@@ -683,6 +744,24 @@ void Scheduler::loop() {
             active_.erase(std::find(active_.begin(), active_.end(), e.first));
             retire(e.first, e.second, e.second == SCL_JOB_CANCELLED ? "cancelled" : "timed out while resident");
         }
+        for (Job* j : suspends) {
+            active_.erase(std::find(active_.begin(), active_.end(), j));
+            if (std::find_if(suspend_failed.begin(), suspend_failed.end(),
+                             [j](const std::pair<Job*, std::string>& e) { return e.first == j; }) !=
+                suspend_failed.end())
+                continue;
+            arena_.release(j->off);
+            j->has_block = false;
+            j->suspend = false;
+            if (j->cancel) {   // cancelled while its state was being saved
+                j->saved.clear();
+                retire(j, SCL_JOB_CANCELLED, "cancelled");
+            } else {
+                j->state = SCL_JOB_SUSPENDED;
+            }
+        }
+        for (auto& e : suspend_failed) retire(e.first, SCL_JOB_FAILED, e.second);
+        if (!suspends.empty()) cv_done_.notify_all();
         for (auto& e : setup_failed) {
             --setting_up_;
             retire(e.first, SCL_JOB_FAILED, e.second);
@@ -783,6 +862,8 @@ void Scheduler::loop() {
     active_.clear();
     for (Job* j : pending_) retire(j, SCL_JOB_CANCELLED, "server destroyed");
     pending_.clear();
+    for (auto& e : jobs_)
+        if (e.second->state == SCL_JOB_SUSPENDED) retire(e.second.get(), SCL_JOB_CANCELLED, "server destroyed");
 }
 
 }  // namespace serve
@@ -826,6 +907,16 @@ int scl_server_wait(scl_server* s, long long id, double timeout_ms, scl_job_stat
 int scl_server_cancel(scl_server* s, long long id) {
     if (!s) return report(SCL_ERR_ARG, "null server");
     return s->sched->cancel(id);
+}
+
+int scl_server_suspend(scl_server* s, long long id) {
+    if (!s) return report(SCL_ERR_ARG, "null server");
+    return s->sched->suspend(id);
+}
+
+int scl_server_resume(scl_server* s, long long id) {
+    if (!s) return report(SCL_ERR_ARG, "null server");
+    return s->sched->resume(id);
 }
 
 int scl_server_result(scl_server* s, long long id, int which, int comp, double* out, size_t n) {

@@ -111,6 +111,8 @@ struct Job {
     double mass0[2] = {0, 0}, mass[2] = {0, 0};
     bool mass0_set = false;
     bool cancel = false;
+    bool suspend = false;             // leave the device at the next round boundary
+    std::vector<char> saved;          // exact state of a suspended job (backend format), else empty
     std::string error;
     std::vector<double> rho[2], f[2]; // deliverables (reference layout)
     std::vector<uint64_t> bits;       // packed mask (pack_mask), built before admission
@@ -135,9 +137,19 @@ class Backend {
     }
     // Memory a job needs inside the arena.
     virtual size_t footprint(const JobSpec& s) const = 0;
-    // Build the job's lattice at arena offset j.off.  On failure return an
+    // Build the job's lattice at arena offset j.off: from j.saved when it is
+    // not empty (a resumed job), else from the spec.  On failure return an
     // error code with the message in *err (the job fails, others continue).
     virtual int setup(Job& j, std::string* err) = 0;
+    // Copy a resident job's exact state to *out (restored by setup), for
+    // pre-emption; backends that cannot, leave the default.
+    virtual bool can_save() const { return false; }
+    virtual int save(Job& j, std::vector<char>* out, std::string* err) {
+        (void)j;
+        (void)out;
+        *err = "this backend cannot pre-empt jobs";
+        return SCL_ERR_UNSUPPORTED;
+    }
     // Advance jobs[k] by steps[k] >= 1 steps (all steps[k] <= rsteps); on
     // return jobs[k]->mass holds the FP64 mass per component afterwards.
     virtual int run_round(const std::vector<Job*>& jobs, const std::vector<int>& steps, int rsteps,
@@ -166,6 +178,8 @@ class Scheduler {
     int status(long long id, scl_job_status* st);
     int wait(long long id, double timeout_ms, scl_job_status* st);
     int cancel(long long id);
+    int suspend(long long id);
+    int resume(long long id);
     int result(long long id, int which, int comp, double* out, size_t n);
     int release(long long id);
     int drain(double timeout_ms);
@@ -179,7 +193,8 @@ class Scheduler {
     void admit_locked(double now, std::vector<Job*>* admitted);
     void retire(Job* j, int state, const std::string& err);
     void fill_status(const Job& j, scl_job_status* st) const;
-    bool finished(int state) const { return state >= SCL_JOB_DONE; }
+    bool finished(int state) const { return state >= SCL_JOB_DONE && state <= SCL_JOB_TIMED_OUT; }
+    void enqueue_locked(Job* j);
 
     std::unique_ptr<Backend> be_;
     scl_server_config cfg_;

@@ -32,7 +32,8 @@ EXPORTS = [
     "scl_field_checkpoint_size", "scl_field_checkpoint", "scl_field_restore",
     # serving layer (bound in serve.py)
     "scl_server_default_config", "scl_server_create", "scl_server_submit", "scl_server_status",
-    "scl_server_wait", "scl_server_cancel", "scl_server_result", "scl_server_release", "scl_server_drain",
+    "scl_server_wait", "scl_server_cancel", "scl_server_suspend", "scl_server_resume", "scl_server_result",
+    "scl_server_release", "scl_server_drain",
     "scl_server_get_stats", "scl_server_destroy", "scl_job_mask",
 ]
 

@@ -19,7 +19,7 @@ from . import native as nat
 IC_UNIFORM, IC_DROPLET, IC_STRATIFIED = 0, 1, 2
 DELIVER_RHO, DELIVER_F = 1, 2
 QUEUED, RUNNING, DONE, FAILED, CANCELLED, TIMED_OUT = range(6)
-STATE_NAMES = ["queued", "running", "done", "failed", "cancelled", "timed_out"]
+STATE_NAMES = ["queued", "running", "done", "failed", "cancelled", "timed_out", "suspended"]
 RHO, F = nat.RHO, nat.F
 SCL_ERR_BUSY, SCL_ERR_TIMEOUT = 5, 6
 
@@ -93,6 +93,8 @@ def bind(lib, create="scl_server_create"):
         "scl_server_status": [vp, ctypes.c_longlong, ctypes.POINTER(JobStatus)],
         "scl_server_wait": [vp, ctypes.c_longlong, ctypes.c_double, ctypes.POINTER(JobStatus)],
         "scl_server_cancel": [vp, ctypes.c_longlong],
+        "scl_server_suspend": [vp, ctypes.c_longlong],
+        "scl_server_resume": [vp, ctypes.c_longlong],
         "scl_server_result": [vp, ctypes.c_longlong, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_size_t],
         "scl_server_release": [vp, ctypes.c_longlong],
         "scl_server_drain": [vp, ctypes.c_double],
@@ -210,6 +212,14 @@ class Server:
 
     def cancel(self, jid):
         _check(self.lib, self.lib.scl_server_cancel(self.h, jid))
+
+    def suspend(self, jid):
+        """Pre-empt a job: its exact state leaves the device at the next round
+        boundary (at once if queued) until resume()."""
+        _check(self.lib, self.lib.scl_server_suspend(self.h, jid))
+
+    def resume(self, jid):
+        _check(self.lib, self.lib.scl_server_resume(self.h, jid))
 
     def result(self, jid, which, comp):
         """A deliverable of a finished job: RHO (ny x nx) or F (9 x ny x nx)."""

@@ -78,7 +78,39 @@ class CpuBackend : public scl::serve::Backend {
             oracle_set(o, c, ORACLE_F, f.data(), f.size());
         }
         oracle_solid_surface_force(o);
+        if (!j.saved.empty()) {   // a resumed job: populations and equilibrium velocities (save)
+            if (j.saved.size() != saved_bytes(n)) {
+                oracle_destroy(o);
+                *err = "saved state does not match the lattice";
+                return SCL_ERR_ARG;
+            }
+            const double* in = (const double*)j.saved.data();
+            for (int c = 0; c < 2; ++c, in += 9 * n) oracle_set(o, c, ORACLE_F, in, 9 * n);
+            for (int c = 0; c < 2; ++c)
+                for (int fld : {ORACLE_UEQX, ORACLE_UEQY}) {
+                    oracle_set(o, c, fld, in, n);
+                    in += n;
+                }
+            j.saved.clear();
+        }
         j.dev = o;
+        return SCL_OK;
+    }
+
+    static size_t saved_bytes(size_t n) { return (2 * 9 * n + 4 * n) * sizeof(double); }
+    bool can_save() const override { return true; }
+    int save(Job& j, std::vector<char>* out, std::string* err) override {
+        (void)err;
+        const oracle_sys* o = (const oracle_sys*)j.dev;
+        const size_t n = (size_t)j.spec.nx * j.spec.ny;
+        out->resize(saved_bytes(n));
+        double* p = (double*)out->data();
+        for (int c = 0; c < 2; ++c, p += 9 * n) oracle_get(o, c, ORACLE_F, p, 9 * n);
+        for (int c = 0; c < 2; ++c)
+            for (int fld : {ORACLE_UEQX, ORACLE_UEQY}) {
+                oracle_get(o, c, fld, p, n);
+                p += n;
+            }
         return SCL_OK;
     }
 

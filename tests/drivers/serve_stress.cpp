@@ -1,7 +1,8 @@
 // serve_stress.cpp -- TEST INFRASTRUCTURE.  Concurrent clients against the
 // serving scheduler (CPU backend, tests/drivers/serve_cpu.cpp): submissions
-// with back-pressure retries, release times, cancellations, waits, result
-// reads, releases and stats polling from several threads at once.  Built with
+// with back-pressure retries, release times, cancellations, pre-emptions
+// (suspend and resume), waits, result reads, releases and stats polling from
+// several threads at once.  Built with
 // ThreadSanitizer on the host compiler; exits non-zero if any job ends in an
 // unexpected state or a delivered result is not finite.
 #include <atomic>
@@ -61,6 +62,12 @@ int main() {
                 continue;
             }
             if (k % 5 == 3) scl_server_cancel(srv, id);
+            if (k % 5 == 1) {
+                // pre-empt and resume; a job that already finished refuses both
+                scl_server_suspend(srv, id);
+                std::this_thread::sleep_for(std::chrono::microseconds(400));
+                scl_server_resume(srv, id);
+            }
             scl_job_status st;
             if (scl_server_wait(srv, id, 30000.0, &st) != SCL_OK) {
                 ++bad;

@@ -279,3 +279,31 @@ def test_reference_density_of_the_deviation_forms():
         nat.step2(g, fl[0], fl[1], case["force"][0], case["force"][1], job["steps"])
         for c in range(2):
             assert np.array_equal(fl[c].get_f().reshape(-1), want[c].reshape(-1))
+
+
+@pytest.mark.parametrize("form", ["fp64", "fp32dev", "fp16dev"])
+def test_suspended_job_resumes_bit_for_bit(form):
+    """Pre-emption on the device: a wide porous job (two-step tiles in the
+    deviation forms) suspended after some rounds leaves the device with its
+    exact state, another job runs in its memory, and after resume it
+    finishes with the same bits as the job run uninterrupted."""
+    import time
+    job = dict(POROUS, nx=300, ny=120, steps=161, geom_seed=4, ic_seed=6, deliver=serve.DELIVER_F)
+    alone = run_alone(job, form, round_steps=8)
+    with gpu_server(form, round_steps=8, max_jobs=1) as srv:
+        jid = srv.submit(**job)
+        t_end = time.time() + 30
+        while srv.status(jid)["steps_done"] < 24 and time.time() < t_end:
+            time.sleep(0.001)
+        srv.suspend(jid)
+        while srv.status(jid)["state"] != "suspended" and time.time() < t_end:
+            time.sleep(0.001)
+        st = srv.status(jid)
+        assert st["state"] == "suspended" and 24 <= st["steps_done"] < job["steps"], st
+        other = srv.submit(nx=257, ny=64, steps=40, ic_seed=3, radius=12, **DROP)
+        assert srv.wait(other, 60000)["state"] == "done"
+        srv.resume(jid)
+        assert srv.wait(jid, 60000)["state"] == "done"
+        got = [srv.result(jid, serve.F, c) for c in range(2)]
+    for c in range(2):
+        assert np.array_equal(got[c], alone[c])

@@ -326,3 +326,64 @@ def test_fluid_updates_count_fluid_sites_only(cpulib):
     assert st["lattice_updates"] == 48 * 40 * 20 + 40 * 24 * 12
     assert st["fluid_updates"] == fluid_gen * 20 + fluid_caller * 12
     assert 0 < st["fluid_updates"] < st["lattice_updates"]
+
+
+def _wait_state(srv, jid, state, timeout_s=10.0):
+    t_end = time.time() + timeout_s
+    while time.time() < t_end:
+        st = srv.status(jid)
+        if st["state"] == state:
+            return st
+        time.sleep(0.005)
+    raise AssertionError("job %d did not reach %s: %s" % (jid, state, srv.status(jid)))
+
+
+def test_suspend_and_resume_continue_bit_for_bit(cpulib):
+    """Pre-emption: a resident job suspended mid-run leaves the device (its
+    memory freed, another job admitted in its place), waits suspended, and
+    after resume finishes with the populations of the uninterrupted oracle
+    run.  A queued job is set aside at once; suspending twice is harmless;
+    finished jobs refuse; a suspended job can be cancelled; resume of a job
+    whose suspension is still pending just keeps it running."""
+    job = dict(nx=48, ny=40, steps=60, porosity=0.85, rmin=3, rmax=6, geom_seed=1, walls=True, radius=9,
+               ic_seed=10, g_solid=(0.1, -0.1), force=(wl.body_force(1e-5), None), **DROP)
+    ref = oracle_run(wl.served_case(**job), job["steps"])
+    cpulib.sct_set_round_delay(15)
+    try:
+        with cpu_server(cpulib, round_steps=4, max_jobs=1) as srv:
+            a = srv.submit(deliver=serve.DELIVER_F, **job)
+            b = srv.submit(nx=16, ny=16, steps=8)
+            q = srv.submit(nx=16, ny=16, steps=8)
+            srv.suspend(q)                                   # queued: set aside at once
+            assert srv.status(q)["state"] == "suspended"
+            _wait_state(srv, a, "running")
+            while srv.status(a)["steps_done"] < 8:
+                time.sleep(0.005)
+            srv.suspend(a)
+            srv.suspend(a)
+            st = _wait_state(srv, a, "suspended")
+            assert 8 <= st["steps_done"] < job["steps"]
+            assert srv.wait(b, 10000)["state"] == "done"      # admitted in a's place
+            assert srv.status(a)["state"] == "suspended" and srv.stats()["arena_used"] == 0
+            srv.resume(a)
+            srv.resume(q)
+            assert srv.wait(a, 20000)["state"] == "done"
+            assert srv.wait(q, 20000)["state"] == "done"
+            for c in range(2):
+                assert np.array_equal(srv.result(a, serve.F, c).reshape(-1), ref[c])
+            with pytest.raises(serve.ServerError):
+                srv.suspend(a)                               # finished
+            with pytest.raises(serve.ServerError):
+                srv.resume(b)                                # not suspended
+            c_ = srv.submit(nx=16, ny=16, steps=100000)
+            _wait_state(srv, c_, "running")
+            srv.suspend(c_)
+            srv.resume(c_)                                   # still pending: keeps running
+            time.sleep(0.1)
+            assert srv.status(c_)["state"] == "running"
+            srv.suspend(c_)
+            _wait_state(srv, c_, "suspended")
+            srv.cancel(c_)
+            assert srv.status(c_)["state"] == "cancelled"
+    finally:
+        cpulib.sct_set_round_delay(0)
