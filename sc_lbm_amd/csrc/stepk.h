@@ -16,10 +16,10 @@
 // columns 2s-1 .. T-2s of the strip, so a strip writes OUT columns from
 // column H = 2K-1 on (OUT: a multiple of 32 with FP32 storage, so no
 // 128-byte line is written by two strips) and bands recompute 2(K-1) rows
-// of the first stage on each side.  Per row: (a) file the densities of
-// S_n(r + 1), loaded one row earlier, and issue the loads of row r + 2;
-// then for each stage its gather and its collision, 2K - 1 barriers in all.
-// The barriers wait only for LDS, so the loads stay in flight.
+// of the first stage on each side.  Per row: file the densities of
+// S_n(r + 1), loaded one row earlier, and issue the loads of row r + 2; then
+// each stage's gather and collision, in one of two schedules (two_phase
+// below).  The barriers wait only for LDS, so the loads stay in flight.
 //
 // LDS per stage: densities in rings of 3 rows; post-collision populations
 // by direction class (the next stage's gather at row g reads direction i
@@ -96,6 +96,18 @@ struct XkArgs {
     void* dst[2];
     UeqRing u;
 };
+
+// The row schedule of a tile.  Two phases (stage s lags stage 1 by 3(s - 1)
+// rows, two barriers per row) where measured faster: FP32 storage, no
+// solids, three steps (76.7k against 72.8k MLUPS at 4096^2).  Elsewhere the
+// stages follow each other within the row (2K - 1 barriers): there the
+// two-phase schedule's longer register lifetimes cost occupancy (FP16 two
+// steps: 72.4k against 82.9k; FP32 three steps with solids: 26.0k against
+// 41.7k).  Both give the same bits.
+template <class ST, bool OPEN, int K>
+__host__ __device__ constexpr bool two_phase() {
+    return sizeof(typename ST::S) == 4 && OPEN && K == 3;
+}
 
 template <class ST, bool OPEN, int K, int T, bool FORCE>
 __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0, const int y1,
@@ -185,150 +197,324 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
         }
     };
 
-    const int r_first = y0 - 2 * K + 2, r_last = y1 + 2 * K - 3;   // rows stage 1 collides
-    const int load_last = y1 + 2 * K - 2;                          // last row loaded
-    // S_n populations rotate by name through three arrays (loop unrolled by 3)
-    float pX[2][Q], pY[2][Q], pZ[2][Q];
-    float sc[K - 1][2][Q], sp[K - 1][2][Q];   // stage s >= 2: rows gathered now / collided now
-    float rest_now[K - 1][2], rest_prev[K - 1][2];
-    // neighbourhood window: nbw[j] = bits of row r + 2 - j
-    unsigned nbw[2 * K + 1];
+    if constexpr (two_phase<ST, OPEN, K>()) {
+        // Stage s (1-based) collides row r - 3(s - 1) in iteration r and, for
+        // s >= 2, gathers row r - 3(s - 1) + 1 from stage s - 1's ring.  The
+        // three-row lag makes every gather read rows that earlier iterations
+        // collided, so an iteration has two phases and two barriers: (A) stage
+        // 1's densities of row r + 1 and the loads of row r + 2, every stage's
+        // gather and densities; (B) every stage's collision.  The rings keep
+        // their sizes: a ring row's last reader (phase A) and its next writer
+        // (phase B) meet in the same iteration, on either side of its barrier.
+        const int r_first = y0 - 2 * K + 2;                // first row stage 1 collides
+        const int r1_last = y1 + 2 * K - 3;                // last row stage 1 collides
+        const int r_last = y1 - 1 + 3 * (K - 1);           // the iteration of stage K's last row
+        const int load_last = y1 + 2 * K - 2;              // last row loaded
+        // S_n populations rotate by name through three arrays (loop unrolled by 3)
+        float pX[2][Q], pY[2][Q], pZ[2][Q];
+        float sc[K - 1][2][Q], sp[K - 1][2][Q];   // stage s >= 2: rows gathered now / collided now
+        // rest populations of stage s - 1 by the iteration that collided them:
+        // [0] this one, [1] one earlier, [2] two earlier (the row gathered now)
+        float rest[K - 1][2][3];
+        // neighbourhood window: nbw[j] = bits of row r + 2 - j
+        constexpr int NB = 3 * K;
+        unsigned nbw[NB];
 #pragma unroll
-    for (int j = 0; j < 2 * K + 1; ++j) nbw[j] = 0u;
+        for (int j = 0; j < NB; ++j) nbw[j] = 0u;
 #pragma unroll
-    for (int q = 0; q < K - 1; ++q)
+        for (int q = 0; q < K - 1; ++q)
 #pragma unroll
-        for (int c = 0; c < 2; ++c) {
-            rest_now[q][c] = rest_prev[q][c] = 0.f;
+            for (int c = 0; c < 2; ++c) {
+                rest[q][c][0] = rest[q][c][1] = rest[q][c][2] = 0.f;
 #pragma unroll
-            for (int i = 0; i < Q; ++i) sc[q][c][i] = sp[q][c][i] = 0.f;
+                for (int i = 0; i < Q; ++i) sc[q][c][i] = sp[q][c][i] = 0.f;
+            }
+        // prologue: densities of rows r_first - 1, r_first; populations of row
+        // r_first (pX); loads of row r_first + 1 (pY) in flight
+        {
+            const unsigned nbq = nbits(r_first - 1);
+            gather1(r_first - 1, nbq, pZ);
+#pragma unroll
+            for (int c = 0; c < 2; ++c) L.d[0][r3(r_first - 1)][c][t] = dens(pZ[c]);
+            nbw[2] = nbits(r_first);
+            gather1(r_first, nbw[2], pX);
+#pragma unroll
+            for (int c = 0; c < 2; ++c) L.d[0][r3(r_first)][c][t] = dens(pX[c]);
+            nbw[1] = nbits(r_first + 1);
+            gather1(r_first + 1, nbw[1], pY);
+            mask_words(r_first + 3);
         }
-    // prologue: densities of rows r_first - 1, r_first; populations of row
-    // r_first (pX); loads of row r_first + 1 (pY) in flight
-    {
-        const unsigned nbq = nbits(r_first - 1);
-        gather1(r_first - 1, nbq, pZ);
-#pragma unroll
-        for (int c = 0; c < 2; ++c) L.d[0][r3(r_first - 1)][c][t] = dens(pZ[c]);
-        nbw[2] = nbits(r_first);
-        gather1(r_first, nbw[2], pX);
-#pragma unroll
-        for (int c = 0; c < 2; ++c) L.d[0][r3(r_first)][c][t] = dens(pX[c]);
-        nbw[1] = nbits(r_first + 1);
-        gather1(r_first + 1, nbw[1], pY);
-        mask_words(r_first + 3);
-    }
-    unsigned t_r1 = tri(wr(r_first + 1), Xm, X, Xp), t_r2 = tri(wr(r_first + 2), Xm, X, Xp);
+        unsigned t_r1 = tri(wr(r_first + 1), Xm, X, Xp), t_r2 = tri(wr(r_first + 2), Xm, X, Xp);
 
-    auto row = [&](const int r, float (&pa)[2][Q], float (&pb)[2][Q], float (&pn)[2][Q]) {
-        // (a) stage 1: loads of row r + 2 (the last row reloads load_last,
-        // cached, rather than branching round the loads); densities of r + 1
-        const unsigned t_r3 = tri_of_words();
-        mask_words(r + 4);
-        nbw[0] = neigh(t_r1, t_r2, t_r3);
-        t_r1 = t_r2;
-        t_r2 = t_r3;
-        gather1(min(r + 2, load_last), nbw[0], pn);
+        auto row = [&](const int r, float (&pa)[2][Q], float (&pb)[2][Q], float (&pn)[2][Q]) {
+            // (A) stage 1: loads of row r + 2 (past the band the last row
+            // reloads load_last, cached, rather than branching round the loads);
+            // densities of r + 1
+            const unsigned t_r3 = tri_of_words();
+            mask_words(r + 4);
+            nbw[0] = neigh(t_r1, t_r2, t_r3);
+            t_r1 = t_r2;
+            t_r2 = t_r3;
+            gather1(min(r + 2, load_last), nbw[0], pn);
 #pragma unroll
-        for (int c = 0; c < 2; ++c) L.d[0][r3(r + 1)][c][t] = dens(pb[c]);
-        __syncthreads();
-        // stage 1: collide row r into ring 0
-        if (t >= 1 && t < T - 1 && (OPEN || (nbw[2] & 1u))) {
-            float nr[2][Q], rc[2], oa[Q], ob[Q];
-            nbr_dens(0, r, nr, rc);
-            collide2_core<float, UeqStage, FORCE>(a, UeqStage{A.u, 0}, wr(r) * a.pitch + X, pa, rc, nr, OPEN ? 0x1FFu : nbw[2], oa, ob);
-            if constexpr (K > 1) {
+            for (int c = 0; c < 2; ++c) L.d[0][r3(r + 1)][c][t] = dens(pb[c]);
+            // (A) stage s >= 2: gather row g from ring q (stage s - 1), collided
+            // by stage s - 1 in iterations up to r - 1
+#pragma unroll
+            for (int s = 2; s <= K; ++s) {
+                const int q = s - 2;
+                const int g = r - 3 * (s - 1) + 1;
+                if (g >= y0 - 2 * (K - s) - 1 && g <= y1 + 2 * (K - s) && t >= 2 * s - 2 && t < T - 2 * s + 2) {
+                    const unsigned nbg = nbw[3 * s - 2];
+                    if (OPEN || (nbg & 1u)) {
+#pragma unroll
+                        for (int c = 0; c < 2; ++c) {
+                            sc[q][c][0] = rest[q][c][2];
+#pragma unroll
+                            for (int i = 1; i < Q; ++i) {
+                                const bool src_fluid = OPEN || ((nbg >> opp(i)) & 1u);
+                                const S v = src_fluid ? pr(q, g - cy(i), c, i, t - cx(i)) : pr(q, g, c, opp(i), t);
+                                sc[q][c][i] = ST::dec(v);
+                            }
+                        }
+                    } else {
+#pragma unroll
+                        for (int c = 0; c < 2; ++c)
+#pragma unroll
+                            for (int i = 0; i < Q; ++i) sc[q][c][i] = 0.f;
+                    }
+#pragma unroll
+                    for (int c = 0; c < 2; ++c) L.d[s - 1][r3(g)][c][t] = dens(sc[q][c]);
+                }
+            }
+            __syncthreads();
+            // (B) stage 1: collide row r into ring 0
+            if (r <= r1_last && t >= 1 && t < T - 1 && (OPEN || (nbw[2] & 1u))) {
+                float nr[2][Q], rc[2], oa[Q], ob[Q];
+                nbr_dens(0, r, nr, rc);
+                collide2_core<float, UeqStage, FORCE>(a, UeqStage{A.u, 0}, wr(r) * a.pitch + X, pa, rc, nr,
+                                                      OPEN ? 0x1FFu : nbw[2], oa, ob);
 #pragma unroll
                 for (int i = 1; i < Q; ++i) {
                     pr(0, r, 0, i, t) = ST::enc(oa[i]);
                     pr(0, r, 1, i, t) = ST::enc(ob[i]);
                 }
-                rest_now[0][0] = ST::dec(ST::enc(oa[0]));
-                rest_now[0][1] = ST::dec(ST::enc(ob[0]));
+                rest[0][0][0] = ST::dec(ST::enc(oa[0]));
+                rest[0][1][0] = ST::dec(ST::enc(ob[0]));
             }
-        }
-        __syncthreads();
+            // (B) stage s >= 2: collide row h = r - 3(s - 1) from the populations
+            // gathered one iteration earlier
 #pragma unroll
-        for (int s = 2; s <= K; ++s) {
-            const int q = s - 2;   // register / ring index of this stage's inputs
-            // gather of stage s at row g from ring q (stage s-1)
-            const int g = r - 2 * s + 3;
-            if (g >= y0 - 2 * (K - s) - 1 && t >= 2 * s - 2 && t < T - 2 * s + 2) {
-                const unsigned nbg = nbw[2 * s - 1];
-                if (OPEN || (nbg & 1u)) {
-#pragma unroll
-                    for (int c = 0; c < 2; ++c) {
-                        sc[q][c][0] = rest_prev[q][c];
+            for (int s = 2; s <= K; ++s) {
+                const int q = s - 2;
+                const int h = r - 3 * (s - 1);
+                const unsigned nbh = nbw[3 * s - 1];
+                if (s < K) {
+                    if (h >= y0 - 2 * (K - s) && h <= y1 + 2 * (K - s) - 1 && t >= 2 * s - 1 && t < T - 2 * s + 1 &&
+                        (OPEN || (nbh & 1u))) {
+                        float nr[2][Q], rc[2], oa[Q], ob[Q];
+                        nbr_dens(s - 1, h, nr, rc);
+                        collide2_core<float, UeqStage, FORCE>(a, UeqStage{A.u, s - 1}, wr(h) * a.pitch + X, sp[q], rc,
+                                                              nr, OPEN ? 0x1FFu : nbh, oa, ob);
 #pragma unroll
                         for (int i = 1; i < Q; ++i) {
-                            const bool src_fluid = OPEN || ((nbg >> opp(i)) & 1u);
-                            const S v = src_fluid ? pr(q, g - cy(i), c, i, t - cx(i)) : pr(q, g, c, opp(i), t);
-                            sc[q][c][i] = ST::dec(v);
+                            pr(s - 1, h, 0, i, t) = ST::enc(oa[i]);
+                            pr(s - 1, h, 1, i, t) = ST::enc(ob[i]);
                         }
+                        rest[s - 1][0][0] = ST::dec(ST::enc(oa[0]));
+                        rest[s - 1][1][0] = ST::dec(ST::enc(ob[0]));
                     }
-                } else {
-#pragma unroll
-                    for (int c = 0; c < 2; ++c)
-#pragma unroll
-                        for (int i = 0; i < Q; ++i) sc[q][c][i] = 0.f;
-                }
-#pragma unroll
-                for (int c = 0; c < 2; ++c) L.d[s - 1][r3(g)][c][t] = dens(sc[q][c]);
-            }
-            __syncthreads();
-            // collision of stage s at row h = r - 2(s - 1)
-            const int h = r - 2 * (s - 1);
-            const unsigned nbh = nbw[2 * s];
-            if (s < K) {
-                if (h >= y0 - 2 * (K - s) && t >= 2 * s - 1 && t < T - 2 * s + 1 && (OPEN || (nbh & 1u))) {
+                } else if (h >= y0 && h < y1 && out_lane && (OPEN || (nbh & 1u))) {
                     float nr[2][Q], rc[2], oa[Q], ob[Q];
                     nbr_dens(s - 1, h, nr, rc);
-                    collide2_core<float, UeqStage, FORCE>(a, UeqStage{A.u, s - 1}, wr(h) * a.pitch + X, sp[q], rc, nr, OPEN ? 0x1FFu : nbh, oa, ob);
+                    collide2_core<float, UeqStage, FORCE>(a, UeqStage{A.u, s - 1}, h * a.pitch + X, sp[q], rc, nr,
+                                                          OPEN ? 0x1FFu : nbh, oa, ob);
+                    const OffT home = (OffT)h * pitch_b + (OffT)X * ES;
 #pragma unroll
-                    for (int i = 1; i < Q; ++i) {
-                        pr(s - 1, h, 0, i, t) = ST::enc(oa[i]);
-                        pr(s - 1, h, 1, i, t) = ST::enc(ob[i]);
+                    for (int i = 0; i < Q; ++i) {
+                        ST::stc((S*)((char*)A.dst[0] + (size_t)i * plane_b + home), oa[i]);
+                        ST::stc((S*)((char*)A.dst[1] + (size_t)i * plane_b + home), ob[i]);
                     }
-                    rest_now[s - 1][0] = ST::dec(ST::enc(oa[0]));
-                    rest_now[s - 1][1] = ST::dec(ST::enc(ob[0]));
-                }
-                __syncthreads();
-            } else if (h >= y0 && h < y1 && out_lane && (OPEN || (nbh & 1u))) {
-                float nr[2][Q], rc[2], oa[Q], ob[Q];
-                nbr_dens(s - 1, h, nr, rc);
-                collide2_core<float, UeqStage, FORCE>(a, UeqStage{A.u, s - 1}, h * a.pitch + X, sp[q], rc, nr, OPEN ? 0x1FFu : nbh, oa, ob);
-                const OffT home = (OffT)h * pitch_b + (OffT)X * ES;
-#pragma unroll
-                for (int i = 0; i < Q; ++i) {
-                    ST::stc((S*)((char*)A.dst[0] + (size_t)i * plane_b + home), oa[i]);
-                    ST::stc((S*)((char*)A.dst[1] + (size_t)i * plane_b + home), ob[i]);
                 }
             }
+            __syncthreads();
+            // rotate (the S_n populations rotate by name)
+#pragma unroll
+            for (int q = 0; q < K - 1; ++q)
+#pragma unroll
+                for (int c = 0; c < 2; ++c) {
+#pragma unroll
+                    for (int i = 0; i < Q; ++i) sp[q][c][i] = sc[q][c][i];
+                    rest[q][c][2] = rest[q][c][1];
+                    rest[q][c][1] = rest[q][c][0];
+                }
+#pragma unroll
+            for (int j = NB - 1; j >= 1; --j) nbw[j] = nbw[j - 1];
+        };
+        // iterations r_first .. r_last; the bounds are uniform over the
+        // workgroup, so every thread reaches the same barriers.  The row index is
+        // read from lane 0 to tell the compiler so: otherwise it keeps the loop
+        // counter and every row computation in vector registers (a divergent
+        // loop).
+        for (int r0 = r_first;; r0 += 3) {
+            const int r = __builtin_amdgcn_readfirstlane(r0);
+            row(r, pX, pY, pZ);
+            if (r + 1 > r_last) break;
+            row(r + 1, pY, pZ, pX);
+            if (r + 2 > r_last) break;
+            row(r + 2, pZ, pX, pY);
+            if (r + 3 > r_last) break;
         }
-        // rotate (the S_n populations rotate by name)
+    } else {
+        // Stage s collides row r - 2(s - 1) in iteration r and, for s >= 2,
+        // first gathers row r - 2s + 3: each gather reads the row stage s - 1
+        // collided in the same iteration, so the stages follow each other
+        // with a barrier between every gather and collision (2K - 1 per row).
+        const int r_first = y0 - 2 * K + 2, r_last = y1 + 2 * K - 3;   // rows stage 1 collides
+        const int load_last = y1 + 2 * K - 2;                          // last row loaded
+        // S_n populations rotate by name through three arrays (loop unrolled by 3)
+        float pX[2][Q], pY[2][Q], pZ[2][Q];
+        float sc[K - 1][2][Q], sp[K - 1][2][Q];   // stage s >= 2: rows gathered now / collided now
+        float rest_now[K - 1][2], rest_prev[K - 1][2];
+        // neighbourhood window: nbw[j] = bits of row r + 2 - j
+        unsigned nbw[2 * K + 1];
+#pragma unroll
+        for (int j = 0; j < 2 * K + 1; ++j) nbw[j] = 0u;
 #pragma unroll
         for (int q = 0; q < K - 1; ++q)
 #pragma unroll
             for (int c = 0; c < 2; ++c) {
+                rest_now[q][c] = rest_prev[q][c] = 0.f;
 #pragma unroll
-                for (int i = 0; i < Q; ++i) sp[q][c][i] = sc[q][c][i];
-                rest_prev[q][c] = rest_now[q][c];
+                for (int i = 0; i < Q; ++i) sc[q][c][i] = sp[q][c][i] = 0.f;
             }
+        // prologue: densities of rows r_first - 1, r_first; populations of row
+        // r_first (pX); loads of row r_first + 1 (pY) in flight
+        {
+            const unsigned nbq = nbits(r_first - 1);
+            gather1(r_first - 1, nbq, pZ);
 #pragma unroll
-        for (int j = 2 * K; j >= 1; --j) nbw[j] = nbw[j - 1];
-    };
-    // rows r_first .. r_last; the bounds are uniform over the workgroup, so
-    // every thread reaches the same barriers.  The row index is read from
-    // lane 0 to tell the compiler so: otherwise it keeps the loop counter
-    // and every row computation in vector registers (a divergent loop).
-    for (int r0 = r_first;; r0 += 3) {
-        const int r = __builtin_amdgcn_readfirstlane(r0);
-        row(r, pX, pY, pZ);
-        if (r + 1 > r_last) break;
-        row(r + 1, pY, pZ, pX);
-        if (r + 2 > r_last) break;
-        row(r + 2, pZ, pX, pY);
-        if (r + 3 > r_last) break;
+            for (int c = 0; c < 2; ++c) L.d[0][r3(r_first - 1)][c][t] = dens(pZ[c]);
+            nbw[2] = nbits(r_first);
+            gather1(r_first, nbw[2], pX);
+#pragma unroll
+            for (int c = 0; c < 2; ++c) L.d[0][r3(r_first)][c][t] = dens(pX[c]);
+            nbw[1] = nbits(r_first + 1);
+            gather1(r_first + 1, nbw[1], pY);
+            mask_words(r_first + 3);
+        }
+        unsigned t_r1 = tri(wr(r_first + 1), Xm, X, Xp), t_r2 = tri(wr(r_first + 2), Xm, X, Xp);
+
+        auto row = [&](const int r, float (&pa)[2][Q], float (&pb)[2][Q], float (&pn)[2][Q]) {
+            // (a) stage 1: loads of row r + 2 (the last row reloads load_last,
+            // cached, rather than branching round the loads); densities of r + 1
+            const unsigned t_r3 = tri_of_words();
+            mask_words(r + 4);
+            nbw[0] = neigh(t_r1, t_r2, t_r3);
+            t_r1 = t_r2;
+            t_r2 = t_r3;
+            gather1(min(r + 2, load_last), nbw[0], pn);
+#pragma unroll
+            for (int c = 0; c < 2; ++c) L.d[0][r3(r + 1)][c][t] = dens(pb[c]);
+            __syncthreads();
+            // stage 1: collide row r into ring 0
+            if (t >= 1 && t < T - 1 && (OPEN || (nbw[2] & 1u))) {
+                float nr[2][Q], rc[2], oa[Q], ob[Q];
+                nbr_dens(0, r, nr, rc);
+                collide2_core<float, UeqStage, FORCE>(a, UeqStage{A.u, 0}, wr(r) * a.pitch + X, pa, rc, nr, OPEN ? 0x1FFu : nbw[2], oa, ob);
+                if constexpr (K > 1) {
+#pragma unroll
+                    for (int i = 1; i < Q; ++i) {
+                        pr(0, r, 0, i, t) = ST::enc(oa[i]);
+                        pr(0, r, 1, i, t) = ST::enc(ob[i]);
+                    }
+                    rest_now[0][0] = ST::dec(ST::enc(oa[0]));
+                    rest_now[0][1] = ST::dec(ST::enc(ob[0]));
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int s = 2; s <= K; ++s) {
+                const int q = s - 2;   // register / ring index of this stage's inputs
+                // gather of stage s at row g from ring q (stage s-1)
+                const int g = r - 2 * s + 3;
+                if (g >= y0 - 2 * (K - s) - 1 && t >= 2 * s - 2 && t < T - 2 * s + 2) {
+                    const unsigned nbg = nbw[2 * s - 1];
+                    if (OPEN || (nbg & 1u)) {
+#pragma unroll
+                        for (int c = 0; c < 2; ++c) {
+                            sc[q][c][0] = rest_prev[q][c];
+#pragma unroll
+                            for (int i = 1; i < Q; ++i) {
+                                const bool src_fluid = OPEN || ((nbg >> opp(i)) & 1u);
+                                const S v = src_fluid ? pr(q, g - cy(i), c, i, t - cx(i)) : pr(q, g, c, opp(i), t);
+                                sc[q][c][i] = ST::dec(v);
+                            }
+                        }
+                    } else {
+#pragma unroll
+                        for (int c = 0; c < 2; ++c)
+#pragma unroll
+                            for (int i = 0; i < Q; ++i) sc[q][c][i] = 0.f;
+                    }
+#pragma unroll
+                    for (int c = 0; c < 2; ++c) L.d[s - 1][r3(g)][c][t] = dens(sc[q][c]);
+                }
+                __syncthreads();
+                // collision of stage s at row h = r - 2(s - 1)
+                const int h = r - 2 * (s - 1);
+                const unsigned nbh = nbw[2 * s];
+                if (s < K) {
+                    if (h >= y0 - 2 * (K - s) && t >= 2 * s - 1 && t < T - 2 * s + 1 && (OPEN || (nbh & 1u))) {
+                        float nr[2][Q], rc[2], oa[Q], ob[Q];
+                        nbr_dens(s - 1, h, nr, rc);
+                        collide2_core<float, UeqStage, FORCE>(a, UeqStage{A.u, s - 1}, wr(h) * a.pitch + X, sp[q], rc, nr, OPEN ? 0x1FFu : nbh, oa, ob);
+#pragma unroll
+                        for (int i = 1; i < Q; ++i) {
+                            pr(s - 1, h, 0, i, t) = ST::enc(oa[i]);
+                            pr(s - 1, h, 1, i, t) = ST::enc(ob[i]);
+                        }
+                        rest_now[s - 1][0] = ST::dec(ST::enc(oa[0]));
+                        rest_now[s - 1][1] = ST::dec(ST::enc(ob[0]));
+                    }
+                    __syncthreads();
+                } else if (h >= y0 && h < y1 && out_lane && (OPEN || (nbh & 1u))) {
+                    float nr[2][Q], rc[2], oa[Q], ob[Q];
+                    nbr_dens(s - 1, h, nr, rc);
+                    collide2_core<float, UeqStage, FORCE>(a, UeqStage{A.u, s - 1}, h * a.pitch + X, sp[q], rc, nr, OPEN ? 0x1FFu : nbh, oa, ob);
+                    const OffT home = (OffT)h * pitch_b + (OffT)X * ES;
+#pragma unroll
+                    for (int i = 0; i < Q; ++i) {
+                        ST::stc((S*)((char*)A.dst[0] + (size_t)i * plane_b + home), oa[i]);
+                        ST::stc((S*)((char*)A.dst[1] + (size_t)i * plane_b + home), ob[i]);
+                    }
+                }
+            }
+            // rotate (the S_n populations rotate by name)
+#pragma unroll
+            for (int q = 0; q < K - 1; ++q)
+#pragma unroll
+                for (int c = 0; c < 2; ++c) {
+#pragma unroll
+                    for (int i = 0; i < Q; ++i) sp[q][c][i] = sc[q][c][i];
+                    rest_prev[q][c] = rest_now[q][c];
+                }
+#pragma unroll
+            for (int j = 2 * K; j >= 1; --j) nbw[j] = nbw[j - 1];
+        };
+        // rows r_first .. r_last; the bounds are uniform over the workgroup, so
+        // every thread reaches the same barriers.  The row index is read from
+        // lane 0 to tell the compiler so: otherwise it keeps the loop counter
+        // and every row computation in vector registers (a divergent loop).
+        for (int r0 = r_first;; r0 += 3) {
+            const int r = __builtin_amdgcn_readfirstlane(r0);
+            row(r, pX, pY, pZ);
+            if (r + 1 > r_last) break;
+            row(r + 1, pY, pZ, pX);
+            if (r + 2 > r_last) break;
+            row(r + 2, pZ, pX, pY);
+            if (r + 3 > r_last) break;
+        }
     }
 }
 
