@@ -210,9 +210,10 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
         const int r1_last = y1 + 2 * K - 3;                // last row stage 1 collides
         const int r_last = y1 - 1 + 3 * (K - 1);           // the iteration of stage K's last row
         const int load_last = y1 + 2 * K - 2;              // last row loaded
-        // S_n populations rotate by name through three arrays (loop unrolled by 3)
+        // S_n populations rotate by name through three arrays, the rows that
+        // stages s >= 2 gather and collide through two (loop unrolled by 6)
         float pX[2][Q], pY[2][Q], pZ[2][Q];
-        float sc[K - 1][2][Q], sp[K - 1][2][Q];   // stage s >= 2: rows gathered now / collided now
+        float sA[K - 1][2][Q], sB[K - 1][2][Q];
         // rest populations of stage s - 1 by the iteration that collided them:
         // [0] this one, [1] one earlier, [2] two earlier (the row gathered now)
         float rest[K - 1][2][3];
@@ -227,7 +228,7 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
             for (int c = 0; c < 2; ++c) {
                 rest[q][c][0] = rest[q][c][1] = rest[q][c][2] = 0.f;
 #pragma unroll
-                for (int i = 0; i < Q; ++i) sc[q][c][i] = sp[q][c][i] = 0.f;
+                for (int i = 0; i < Q; ++i) sA[q][c][i] = sB[q][c][i] = 0.f;
             }
         // prologue: densities of rows r_first - 1, r_first; populations of row
         // r_first (pX); loads of row r_first + 1 (pY) in flight
@@ -246,7 +247,10 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
         }
         unsigned t_r1 = tri(wr(r_first + 1), Xm, X, Xp), t_r2 = tri(wr(r_first + 2), Xm, X, Xp);
 
-        auto row = [&](const int r, float (&pa)[2][Q], float (&pb)[2][Q], float (&pn)[2][Q]) {
+        // sg: stage s >= 2 gathers into it (the row it collides next
+        // iteration); sx: the row it collides now, gathered one iteration earlier
+        auto row = [&](const int r, float (&pa)[2][Q], float (&pb)[2][Q], float (&pn)[2][Q], float (&sg)[K - 1][2][Q],
+                       float (&sx)[K - 1][2][Q]) {
             // (A) stage 1: loads of row r + 2 (past the band the last row
             // reloads load_last, cached, rather than branching round the loads);
             // densities of r + 1
@@ -269,22 +273,22 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
                     if (OPEN || (nbg & 1u)) {
 #pragma unroll
                         for (int c = 0; c < 2; ++c) {
-                            sc[q][c][0] = rest[q][c][2];
+                            sg[q][c][0] = rest[q][c][2];
 #pragma unroll
                             for (int i = 1; i < Q; ++i) {
                                 const bool src_fluid = OPEN || ((nbg >> opp(i)) & 1u);
                                 const S v = src_fluid ? pr(q, g - cy(i), c, i, t - cx(i)) : pr(q, g, c, opp(i), t);
-                                sc[q][c][i] = ST::dec(v);
+                                sg[q][c][i] = ST::dec(v);
                             }
                         }
                     } else {
 #pragma unroll
                         for (int c = 0; c < 2; ++c)
 #pragma unroll
-                            for (int i = 0; i < Q; ++i) sc[q][c][i] = 0.f;
+                            for (int i = 0; i < Q; ++i) sg[q][c][i] = 0.f;
                     }
 #pragma unroll
-                    for (int c = 0; c < 2; ++c) L.d[s - 1][r3(g)][c][t] = dens(sc[q][c]);
+                    for (int c = 0; c < 2; ++c) L.d[s - 1][r3(g)][c][t] = dens(sg[q][c]);
                 }
             }
             __syncthreads();
@@ -314,7 +318,7 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
                         (OPEN || (nbh & 1u))) {
                         float nr[2][Q], rc[2], oa[Q], ob[Q];
                         nbr_dens(s - 1, h, nr, rc);
-                        collide2_core<float, UeqStage, FORCE>(a, UeqStage{A.u, s - 1}, wr(h) * a.pitch + X, sp[q], rc,
+                        collide2_core<float, UeqStage, FORCE>(a, UeqStage{A.u, s - 1}, wr(h) * a.pitch + X, sx[q], rc,
                                                               nr, OPEN ? 0x1FFu : nbh, oa, ob);
 #pragma unroll
                         for (int i = 1; i < Q; ++i) {
@@ -327,7 +331,7 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
                 } else if (h >= y0 && h < y1 && out_lane && (OPEN || (nbh & 1u))) {
                     float nr[2][Q], rc[2], oa[Q], ob[Q];
                     nbr_dens(s - 1, h, nr, rc);
-                    collide2_core<float, UeqStage, FORCE>(a, UeqStage{A.u, s - 1}, h * a.pitch + X, sp[q], rc, nr,
+                    collide2_core<float, UeqStage, FORCE>(a, UeqStage{A.u, s - 1}, h * a.pitch + X, sx[q], rc, nr,
                                                           OPEN ? 0x1FFu : nbh, oa, ob);
                     const OffT home = (OffT)h * pitch_b + (OffT)X * ES;
 #pragma unroll
@@ -338,13 +342,11 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
                 }
             }
             __syncthreads();
-            // rotate (the S_n populations rotate by name)
+            // rotate (the populations rotate by name)
 #pragma unroll
             for (int q = 0; q < K - 1; ++q)
 #pragma unroll
                 for (int c = 0; c < 2; ++c) {
-#pragma unroll
-                    for (int i = 0; i < Q; ++i) sp[q][c][i] = sc[q][c][i];
                     rest[q][c][2] = rest[q][c][1];
                     rest[q][c][1] = rest[q][c][0];
                 }
@@ -356,14 +358,20 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
         // read from lane 0 to tell the compiler so: otherwise it keeps the loop
         // counter and every row computation in vector registers (a divergent
         // loop).
-        for (int r0 = r_first;; r0 += 3) {
+        for (int r0 = r_first;; r0 += 6) {
             const int r = __builtin_amdgcn_readfirstlane(r0);
-            row(r, pX, pY, pZ);
+            row(r, pX, pY, pZ, sA, sB);
             if (r + 1 > r_last) break;
-            row(r + 1, pY, pZ, pX);
+            row(r + 1, pY, pZ, pX, sB, sA);
             if (r + 2 > r_last) break;
-            row(r + 2, pZ, pX, pY);
+            row(r + 2, pZ, pX, pY, sA, sB);
             if (r + 3 > r_last) break;
+            row(r + 3, pX, pY, pZ, sB, sA);
+            if (r + 4 > r_last) break;
+            row(r + 4, pY, pZ, pX, sA, sB);
+            if (r + 5 > r_last) break;
+            row(r + 5, pZ, pX, pY, sB, sA);
+            if (r + 6 > r_last) break;
         }
     } else {
         // Stage s collides row r - 2(s - 1) in iteration r and, for s >= 2,
