@@ -109,6 +109,56 @@ __host__ __device__ constexpr bool two_phase() {
     return sizeof(typename ST::S) == 4 && OPEN && K == 3;
 }
 
+// Whether the row loop is unrolled by 6, so that the two register arrays of
+// stages s >= 2 (the row gathered for the next iteration, the row collided
+// now) swap roles by name, or by 3 with a copy per row.  Renaming saves 6% of
+// the vector instructions; it is kept where it measured faster, the
+// solid-free three-step tiles (FP32 4096^2: 76.8k -> 78.0k MLUPS, FP16:
+// 74.3k -> 75.1k).  With two steps it gained nothing (FP32) or cost
+// occupancy (FP16: 164 -> 177 VGPRs, three waves per SIMD to two); the
+// variants with solids spill scalar registers.
+template <class ST, bool OPEN, int K>
+__host__ __device__ constexpr bool rename_rows() {
+    return OPEN && K == 3;
+}
+
+// Calls row(r, ...) for r = r_first .. r_last, the populations of S_n
+// rotating by name through pX, pY, pZ.  The bounds are uniform over the
+// workgroup, so every thread reaches the same barriers.  The row index is
+// read from lane 0 to tell the compiler so: otherwise it keeps the loop
+// counter and every row computation in vector registers (a divergent loop).
+template <bool RENAME, class Row, class P, class G>
+__device__ __attribute__((always_inline)) inline void walk_rows(const int r_first, const int r_last, Row& row, P& pX,
+                                                                P& pY, P& pZ, G& sA, G& sB) {
+    if constexpr (RENAME) {
+        for (int r0 = r_first;; r0 += 6) {
+            const int r = __builtin_amdgcn_readfirstlane(r0);
+            row(r, pX, pY, pZ, sA, sB);
+            if (r + 1 > r_last) break;
+            row(r + 1, pY, pZ, pX, sB, sA);
+            if (r + 2 > r_last) break;
+            row(r + 2, pZ, pX, pY, sA, sB);
+            if (r + 3 > r_last) break;
+            row(r + 3, pX, pY, pZ, sB, sA);
+            if (r + 4 > r_last) break;
+            row(r + 4, pY, pZ, pX, sA, sB);
+            if (r + 5 > r_last) break;
+            row(r + 5, pZ, pX, pY, sB, sA);
+            if (r + 6 > r_last) break;
+        }
+    } else {   // row() copies sA to sB at its end
+        for (int r0 = r_first;; r0 += 3) {
+            const int r = __builtin_amdgcn_readfirstlane(r0);
+            row(r, pX, pY, pZ, sA, sB);
+            if (r + 1 > r_last) break;
+            row(r + 1, pY, pZ, pX, sA, sB);
+            if (r + 2 > r_last) break;
+            row(r + 2, pZ, pX, pY, sA, sB);
+            if (r + 3 > r_last) break;
+        }
+    }
+}
+
 template <class ST, bool OPEN, int K, int T, bool FORCE>
 __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0, const int y1,
                                   XkLds<ST, OPEN, K, T>& L) {
@@ -210,8 +260,8 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
         const int r1_last = y1 + 2 * K - 3;                // last row stage 1 collides
         const int r_last = y1 - 1 + 3 * (K - 1);           // the iteration of stage K's last row
         const int load_last = y1 + 2 * K - 2;              // last row loaded
-        // S_n populations rotate by name through three arrays, the rows that
-        // stages s >= 2 gather and collide through two (loop unrolled by 6)
+        // S_n populations rotate through three arrays, the rows that stages
+        // s >= 2 gather and collide through two (walk_rows)
         float pX[2][Q], pY[2][Q], pZ[2][Q];
         float sA[K - 1][2][Q], sB[K - 1][2][Q];
         // rest populations of stage s - 1 by the iteration that collided them:
@@ -347,32 +397,17 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
             for (int q = 0; q < K - 1; ++q)
 #pragma unroll
                 for (int c = 0; c < 2; ++c) {
+                    if constexpr (!rename_rows<ST, OPEN, K>())
+#pragma unroll
+                        for (int i = 0; i < Q; ++i) sx[q][c][i] = sg[q][c][i];
                     rest[q][c][2] = rest[q][c][1];
                     rest[q][c][1] = rest[q][c][0];
                 }
 #pragma unroll
             for (int j = NB - 1; j >= 1; --j) nbw[j] = nbw[j - 1];
         };
-        // iterations r_first .. r_last; the bounds are uniform over the
-        // workgroup, so every thread reaches the same barriers.  The row index is
-        // read from lane 0 to tell the compiler so: otherwise it keeps the loop
-        // counter and every row computation in vector registers (a divergent
-        // loop).
-        for (int r0 = r_first;; r0 += 6) {
-            const int r = __builtin_amdgcn_readfirstlane(r0);
-            row(r, pX, pY, pZ, sA, sB);
-            if (r + 1 > r_last) break;
-            row(r + 1, pY, pZ, pX, sB, sA);
-            if (r + 2 > r_last) break;
-            row(r + 2, pZ, pX, pY, sA, sB);
-            if (r + 3 > r_last) break;
-            row(r + 3, pX, pY, pZ, sB, sA);
-            if (r + 4 > r_last) break;
-            row(r + 4, pY, pZ, pX, sA, sB);
-            if (r + 5 > r_last) break;
-            row(r + 5, pZ, pX, pY, sB, sA);
-            if (r + 6 > r_last) break;
-        }
+        // iterations r_first .. r_last
+        walk_rows<rename_rows<ST, OPEN, K>()>(r_first, r_last, row, pX, pY, pZ, sA, sB);
     } else {
         // Stage s collides row r - 2(s - 1) in iteration r and, for s >= 2,
         // first gathers row r - 2s + 3: each gather reads the row stage s - 1
@@ -380,9 +415,10 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
         // with a barrier between every gather and collision (2K - 1 per row).
         const int r_first = y0 - 2 * K + 2, r_last = y1 + 2 * K - 3;   // rows stage 1 collides
         const int load_last = y1 + 2 * K - 2;                          // last row loaded
-        // S_n populations rotate by name through three arrays (loop unrolled by 3)
+        // S_n populations rotate through three arrays, the rows that stages
+        // s >= 2 gather and collide through two (walk_rows)
         float pX[2][Q], pY[2][Q], pZ[2][Q];
-        float sc[K - 1][2][Q], sp[K - 1][2][Q];   // stage s >= 2: rows gathered now / collided now
+        float sA[K - 1][2][Q], sB[K - 1][2][Q];
         float rest_now[K - 1][2], rest_prev[K - 1][2];
         // neighbourhood window: nbw[j] = bits of row r + 2 - j
         unsigned nbw[2 * K + 1];
@@ -394,7 +430,7 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
             for (int c = 0; c < 2; ++c) {
                 rest_now[q][c] = rest_prev[q][c] = 0.f;
 #pragma unroll
-                for (int i = 0; i < Q; ++i) sc[q][c][i] = sp[q][c][i] = 0.f;
+                for (int i = 0; i < Q; ++i) sA[q][c][i] = sB[q][c][i] = 0.f;
             }
         // prologue: densities of rows r_first - 1, r_first; populations of row
         // r_first (pX); loads of row r_first + 1 (pY) in flight
@@ -413,7 +449,10 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
         }
         unsigned t_r1 = tri(wr(r_first + 1), Xm, X, Xp), t_r2 = tri(wr(r_first + 2), Xm, X, Xp);
 
-        auto row = [&](const int r, float (&pa)[2][Q], float (&pb)[2][Q], float (&pn)[2][Q]) {
+        // sg: stage s >= 2 gathers into it (the row it collides next
+        // iteration); sx: the row it collides now, gathered one iteration earlier
+        auto row = [&](const int r, float (&pa)[2][Q], float (&pb)[2][Q], float (&pn)[2][Q], float (&sg)[K - 1][2][Q],
+                       float (&sx)[K - 1][2][Q]) {
             // (a) stage 1: loads of row r + 2 (the last row reloads load_last,
             // cached, rather than branching round the loads); densities of r + 1
             const unsigned t_r3 = tri_of_words();
@@ -451,22 +490,22 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
                     if (OPEN || (nbg & 1u)) {
 #pragma unroll
                         for (int c = 0; c < 2; ++c) {
-                            sc[q][c][0] = rest_prev[q][c];
+                            sg[q][c][0] = rest_prev[q][c];
 #pragma unroll
                             for (int i = 1; i < Q; ++i) {
                                 const bool src_fluid = OPEN || ((nbg >> opp(i)) & 1u);
                                 const S v = src_fluid ? pr(q, g - cy(i), c, i, t - cx(i)) : pr(q, g, c, opp(i), t);
-                                sc[q][c][i] = ST::dec(v);
+                                sg[q][c][i] = ST::dec(v);
                             }
                         }
                     } else {
 #pragma unroll
                         for (int c = 0; c < 2; ++c)
 #pragma unroll
-                            for (int i = 0; i < Q; ++i) sc[q][c][i] = 0.f;
+                            for (int i = 0; i < Q; ++i) sg[q][c][i] = 0.f;
                     }
 #pragma unroll
-                    for (int c = 0; c < 2; ++c) L.d[s - 1][r3(g)][c][t] = dens(sc[q][c]);
+                    for (int c = 0; c < 2; ++c) L.d[s - 1][r3(g)][c][t] = dens(sg[q][c]);
                 }
                 __syncthreads();
                 // collision of stage s at row h = r - 2(s - 1)
@@ -476,7 +515,7 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
                     if (h >= y0 - 2 * (K - s) && t >= 2 * s - 1 && t < T - 2 * s + 1 && (OPEN || (nbh & 1u))) {
                         float nr[2][Q], rc[2], oa[Q], ob[Q];
                         nbr_dens(s - 1, h, nr, rc);
-                        collide2_core<float, UeqStage, FORCE>(a, UeqStage{A.u, s - 1}, wr(h) * a.pitch + X, sp[q], rc, nr, OPEN ? 0x1FFu : nbh, oa, ob);
+                        collide2_core<float, UeqStage, FORCE>(a, UeqStage{A.u, s - 1}, wr(h) * a.pitch + X, sx[q], rc, nr, OPEN ? 0x1FFu : nbh, oa, ob);
 #pragma unroll
                         for (int i = 1; i < Q; ++i) {
                             pr(s - 1, h, 0, i, t) = ST::enc(oa[i]);
@@ -489,7 +528,7 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
                 } else if (h >= y0 && h < y1 && out_lane && (OPEN || (nbh & 1u))) {
                     float nr[2][Q], rc[2], oa[Q], ob[Q];
                     nbr_dens(s - 1, h, nr, rc);
-                    collide2_core<float, UeqStage, FORCE>(a, UeqStage{A.u, s - 1}, h * a.pitch + X, sp[q], rc, nr, OPEN ? 0x1FFu : nbh, oa, ob);
+                    collide2_core<float, UeqStage, FORCE>(a, UeqStage{A.u, s - 1}, h * a.pitch + X, sx[q], rc, nr, OPEN ? 0x1FFu : nbh, oa, ob);
                     const OffT home = (OffT)h * pitch_b + (OffT)X * ES;
 #pragma unroll
                     for (int i = 0; i < Q; ++i) {
@@ -498,31 +537,20 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
                     }
                 }
             }
-            // rotate (the S_n populations rotate by name)
+            // rotate (the populations rotate by name)
 #pragma unroll
             for (int q = 0; q < K - 1; ++q)
 #pragma unroll
                 for (int c = 0; c < 2; ++c) {
+                    if constexpr (!rename_rows<ST, OPEN, K>())
 #pragma unroll
-                    for (int i = 0; i < Q; ++i) sp[q][c][i] = sc[q][c][i];
+                        for (int i = 0; i < Q; ++i) sx[q][c][i] = sg[q][c][i];
                     rest_prev[q][c] = rest_now[q][c];
                 }
 #pragma unroll
             for (int j = 2 * K; j >= 1; --j) nbw[j] = nbw[j - 1];
         };
-        // rows r_first .. r_last; the bounds are uniform over the workgroup, so
-        // every thread reaches the same barriers.  The row index is read from
-        // lane 0 to tell the compiler so: otherwise it keeps the loop counter
-        // and every row computation in vector registers (a divergent loop).
-        for (int r0 = r_first;; r0 += 3) {
-            const int r = __builtin_amdgcn_readfirstlane(r0);
-            row(r, pX, pY, pZ);
-            if (r + 1 > r_last) break;
-            row(r + 1, pY, pZ, pX);
-            if (r + 2 > r_last) break;
-            row(r + 2, pZ, pX, pY);
-            if (r + 3 > r_last) break;
-        }
+        walk_rows<rename_rows<ST, OPEN, K>()>(r_first, r_last, row, pX, pY, pZ, sA, sB);
     }
 }
 
