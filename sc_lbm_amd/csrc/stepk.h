@@ -122,7 +122,25 @@ __host__ __device__ constexpr bool rename_rows() {
     return OPEN && K == 3;
 }
 
-// Calls row(r, ...) for r = r_first .. r_last, the populations of S_n
+// Whether the three-row ring slots are compile-time constants (the LDS
+// addresses then need no per-row scalar slot arithmetic: 20% fewer scalar
+// and 3.5% fewer vector instructions in the three-step tiles).  Elsewhere
+// the hoisted addresses cost occupancy: FP16 two steps 164 -> 178 VGPRs
+// (three waves per SIMD to two), FP32 three steps with solids past 256.
+template <class ST, bool OPEN, int K>
+__host__ __device__ constexpr bool const_slots() {
+    return OPEN && K == 3;
+}
+
+// A row index and its ring phase: (r - the tile's first row) modulo 3 (6
+// with the loop unrolled by 6), a constant in every unrolled call.
+struct RowAnchor {
+    int r, ph;
+};
+template <int P>
+using Ph = std::integral_constant<int, P>;
+
+// Calls row(phase, r, ...) for r = r_first .. r_last, the populations of S_n
 // rotating by name through pX, pY, pZ.  The bounds are uniform over the
 // workgroup, so every thread reaches the same barriers.  The row index is
 // read from lane 0 to tell the compiler so: otherwise it keeps the loop
@@ -133,27 +151,27 @@ __device__ __attribute__((always_inline)) inline void walk_rows(const int r_firs
     if constexpr (RENAME) {
         for (int r0 = r_first;; r0 += 6) {
             const int r = __builtin_amdgcn_readfirstlane(r0);
-            row(r, pX, pY, pZ, sA, sB);
+            row(Ph<0>{}, r, pX, pY, pZ, sA, sB);
             if (r + 1 > r_last) break;
-            row(r + 1, pY, pZ, pX, sB, sA);
+            row(Ph<1>{}, r + 1, pY, pZ, pX, sB, sA);
             if (r + 2 > r_last) break;
-            row(r + 2, pZ, pX, pY, sA, sB);
+            row(Ph<2>{}, r + 2, pZ, pX, pY, sA, sB);
             if (r + 3 > r_last) break;
-            row(r + 3, pX, pY, pZ, sB, sA);
+            row(Ph<3>{}, r + 3, pX, pY, pZ, sB, sA);
             if (r + 4 > r_last) break;
-            row(r + 4, pY, pZ, pX, sA, sB);
+            row(Ph<4>{}, r + 4, pY, pZ, pX, sA, sB);
             if (r + 5 > r_last) break;
-            row(r + 5, pZ, pX, pY, sB, sA);
+            row(Ph<5>{}, r + 5, pZ, pX, pY, sB, sA);
             if (r + 6 > r_last) break;
         }
     } else {   // row() copies sA to sB at its end
         for (int r0 = r_first;; r0 += 3) {
             const int r = __builtin_amdgcn_readfirstlane(r0);
-            row(r, pX, pY, pZ, sA, sB);
+            row(Ph<0>{}, r, pX, pY, pZ, sA, sB);
             if (r + 1 > r_last) break;
-            row(r + 1, pY, pZ, pX, sA, sB);
+            row(Ph<1>{}, r + 1, pY, pZ, pX, sA, sB);
             if (r + 2 > r_last) break;
-            row(r + 2, pZ, pX, pY, sA, sB);
+            row(Ph<2>{}, r + 2, pZ, pX, pY, sA, sB);
             if (r + 3 > r_last) break;
         }
     }
@@ -191,13 +209,24 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
         r %= ny;
         return r < 0 ? r + ny : r;
     };
-    auto r3 = [&](int r) { return (r - y0 + 3 * (K + 2)) % 3; };   // r >= y0 - 3(K+2)
-    auto r2 = [&](int r) { return (r - y0 + 2 * (K + 2)) & 1; };
+    // Ring slots.  Row `row` sits in slot (row - ra) mod 3 of a three-row
+    // ring and (row - ra) mod 2 of a two-row ring, ra the first row stage 1
+    // collides (row >= ra - 1).  Every row index is an anchor row plus a
+    // constant, and the anchor's phase, (an.r - ra) mod 3, is a compile-time
+    // constant: with const_slots the three-row slots fold into the LDS
+    // offsets.  The two-row slots stay computed (folding them too raised the
+    // register count past two waves per SIMD).
+    const int ra = y0 - 2 * K + 2;
+    auto r3 = [&](const RowAnchor& an, int row) {
+        if constexpr (const_slots<ST, OPEN, K>()) return ((an.ph + (row - an.r)) % 3 + 3) % 3;
+        else return (row - ra + 3) % 3;
+    };
+    auto r2 = [&](const RowAnchor&, int row) { return (row - ra) & 1; };
     // LDS slot of direction i (1..8) of row `row`, column `col`, ring of stage q (0-based)
-    auto pr = [&](int q, int row, int c, int i, int col) -> S& {
-        if (x2_class(i) == 0) return L.pu[q][r3(row)][c][x2_index(i)][col];
-        if (x2_class(i) == 1) return L.pz[q][r2(row)][c][x2_index(i)][col];
-        return L.pd[q][OPEN ? 0 : r2(row)][c][x2_index(i)][col];
+    auto pr = [&](const RowAnchor& an, int q, int row, int c, int i, int col) -> S& {
+        if (x2_class(i) == 0) return L.pu[q][r3(an, row)][c][x2_index(i)][col];
+        if (x2_class(i) == 1) return L.pz[q][r2(an, row)][c][x2_index(i)][col];
+        return L.pd[q][OPEN ? 0 : r2(an, row)][c][x2_index(i)][col];
     };
     auto nbits = [&](int r) -> unsigned {
         return neigh(tri(wr(r - 1), Xm, X, Xp), tri(wr(r), Xm, X, Xp), tri(wr(r + 1), Xm, X, Xp));
@@ -232,8 +261,8 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
             return (unsigned)((mw[0] >> (Xm & 63)) & 1ull) | ((unsigned)((mw[1] >> (X & 63)) & 1ull) << 1) |
                    ((unsigned)((mw[2] >> (Xp & 63)) & 1ull) << 2);
     };
-    auto nbr_dens = [&](int q, int r, float (&nr)[2][Q], float (&rc)[2]) {
-        const int sS = r3(r - 1), sC = r3(r), sN = r3(r + 1);
+    auto nbr_dens = [&](const RowAnchor& an, int q, int r, float (&nr)[2][Q], float (&rc)[2]) {
+        const int sS = r3(an, r - 1), sC = r3(an, r), sN = r3(an, r + 1);
 #pragma unroll
         for (int i = 1; i < Q; ++i) {
             const int sl = cy(i) == 1 ? sN : (cy(i) == -1 ? sS : sC);
@@ -283,14 +312,15 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
         // prologue: densities of rows r_first - 1, r_first; populations of row
         // r_first (pX); loads of row r_first + 1 (pY) in flight
         {
+            const RowAnchor an0{r_first, 0};
             const unsigned nbq = nbits(r_first - 1);
             gather1(r_first - 1, nbq, pZ);
 #pragma unroll
-            for (int c = 0; c < 2; ++c) L.d[0][r3(r_first - 1)][c][t] = dens(pZ[c]);
+            for (int c = 0; c < 2; ++c) L.d[0][r3(an0, r_first - 1)][c][t] = dens(pZ[c]);
             nbw[2] = nbits(r_first);
             gather1(r_first, nbw[2], pX);
 #pragma unroll
-            for (int c = 0; c < 2; ++c) L.d[0][r3(r_first)][c][t] = dens(pX[c]);
+            for (int c = 0; c < 2; ++c) L.d[0][r3(an0, r_first)][c][t] = dens(pX[c]);
             nbw[1] = nbits(r_first + 1);
             gather1(r_first + 1, nbw[1], pY);
             mask_words(r_first + 3);
@@ -299,8 +329,9 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
 
         // sg: stage s >= 2 gathers into it (the row it collides next
         // iteration); sx: the row it collides now, gathered one iteration earlier
-        auto row = [&](const int r, float (&pa)[2][Q], float (&pb)[2][Q], float (&pn)[2][Q], float (&sg)[K - 1][2][Q],
-                       float (&sx)[K - 1][2][Q]) {
+        auto row = [&](auto ph, const int r, float (&pa)[2][Q], float (&pb)[2][Q], float (&pn)[2][Q],
+                       float (&sg)[K - 1][2][Q], float (&sx)[K - 1][2][Q]) {
+            const RowAnchor an{r, decltype(ph)::value};
             // (A) stage 1: loads of row r + 2 (past the band the last row
             // reloads load_last, cached, rather than branching round the loads);
             // densities of r + 1
@@ -311,7 +342,7 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
             t_r2 = t_r3;
             gather1(min(r + 2, load_last), nbw[0], pn);
 #pragma unroll
-            for (int c = 0; c < 2; ++c) L.d[0][r3(r + 1)][c][t] = dens(pb[c]);
+            for (int c = 0; c < 2; ++c) L.d[0][r3(an, r + 1)][c][t] = dens(pb[c]);
             // (A) stage s >= 2: gather row g from ring q (stage s - 1), collided
             // by stage s - 1 in iterations up to r - 1
 #pragma unroll
@@ -327,7 +358,7 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
 #pragma unroll
                             for (int i = 1; i < Q; ++i) {
                                 const bool src_fluid = OPEN || ((nbg >> opp(i)) & 1u);
-                                const S v = src_fluid ? pr(q, g - cy(i), c, i, t - cx(i)) : pr(q, g, c, opp(i), t);
+                                const S v = src_fluid ? pr(an, q, g - cy(i), c, i, t - cx(i)) : pr(an, q, g, c, opp(i), t);
                                 sg[q][c][i] = ST::dec(v);
                             }
                         }
@@ -338,20 +369,20 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
                             for (int i = 0; i < Q; ++i) sg[q][c][i] = 0.f;
                     }
 #pragma unroll
-                    for (int c = 0; c < 2; ++c) L.d[s - 1][r3(g)][c][t] = dens(sg[q][c]);
+                    for (int c = 0; c < 2; ++c) L.d[s - 1][r3(an, g)][c][t] = dens(sg[q][c]);
                 }
             }
             __syncthreads();
             // (B) stage 1: collide row r into ring 0
             if (r <= r1_last && t >= 1 && t < T - 1 && (OPEN || (nbw[2] & 1u))) {
                 float nr[2][Q], rc[2], oa[Q], ob[Q];
-                nbr_dens(0, r, nr, rc);
+                nbr_dens(an, 0, r, nr, rc);
                 collide2_core<float, UeqStage, FORCE>(a, UeqStage{A.u, 0}, wr(r) * a.pitch + X, pa, rc, nr,
                                                       OPEN ? 0x1FFu : nbw[2], oa, ob);
 #pragma unroll
                 for (int i = 1; i < Q; ++i) {
-                    pr(0, r, 0, i, t) = ST::enc(oa[i]);
-                    pr(0, r, 1, i, t) = ST::enc(ob[i]);
+                    pr(an, 0, r, 0, i, t) = ST::enc(oa[i]);
+                    pr(an, 0, r, 1, i, t) = ST::enc(ob[i]);
                 }
                 rest[0][0][0] = ST::dec(ST::enc(oa[0]));
                 rest[0][1][0] = ST::dec(ST::enc(ob[0]));
@@ -367,20 +398,20 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
                     if (h >= y0 - 2 * (K - s) && h <= y1 + 2 * (K - s) - 1 && t >= 2 * s - 1 && t < T - 2 * s + 1 &&
                         (OPEN || (nbh & 1u))) {
                         float nr[2][Q], rc[2], oa[Q], ob[Q];
-                        nbr_dens(s - 1, h, nr, rc);
+                        nbr_dens(an, s - 1, h, nr, rc);
                         collide2_core<float, UeqStage, FORCE>(a, UeqStage{A.u, s - 1}, wr(h) * a.pitch + X, sx[q], rc,
                                                               nr, OPEN ? 0x1FFu : nbh, oa, ob);
 #pragma unroll
                         for (int i = 1; i < Q; ++i) {
-                            pr(s - 1, h, 0, i, t) = ST::enc(oa[i]);
-                            pr(s - 1, h, 1, i, t) = ST::enc(ob[i]);
+                            pr(an, s - 1, h, 0, i, t) = ST::enc(oa[i]);
+                            pr(an, s - 1, h, 1, i, t) = ST::enc(ob[i]);
                         }
                         rest[s - 1][0][0] = ST::dec(ST::enc(oa[0]));
                         rest[s - 1][1][0] = ST::dec(ST::enc(ob[0]));
                     }
                 } else if (h >= y0 && h < y1 && out_lane && (OPEN || (nbh & 1u))) {
                     float nr[2][Q], rc[2], oa[Q], ob[Q];
-                    nbr_dens(s - 1, h, nr, rc);
+                    nbr_dens(an, s - 1, h, nr, rc);
                     collide2_core<float, UeqStage, FORCE>(a, UeqStage{A.u, s - 1}, h * a.pitch + X, sx[q], rc, nr,
                                                           OPEN ? 0x1FFu : nbh, oa, ob);
                     const OffT home = (OffT)h * pitch_b + (OffT)X * ES;
@@ -435,14 +466,15 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
         // prologue: densities of rows r_first - 1, r_first; populations of row
         // r_first (pX); loads of row r_first + 1 (pY) in flight
         {
+            const RowAnchor an0{r_first, 0};
             const unsigned nbq = nbits(r_first - 1);
             gather1(r_first - 1, nbq, pZ);
 #pragma unroll
-            for (int c = 0; c < 2; ++c) L.d[0][r3(r_first - 1)][c][t] = dens(pZ[c]);
+            for (int c = 0; c < 2; ++c) L.d[0][r3(an0, r_first - 1)][c][t] = dens(pZ[c]);
             nbw[2] = nbits(r_first);
             gather1(r_first, nbw[2], pX);
 #pragma unroll
-            for (int c = 0; c < 2; ++c) L.d[0][r3(r_first)][c][t] = dens(pX[c]);
+            for (int c = 0; c < 2; ++c) L.d[0][r3(an0, r_first)][c][t] = dens(pX[c]);
             nbw[1] = nbits(r_first + 1);
             gather1(r_first + 1, nbw[1], pY);
             mask_words(r_first + 3);
@@ -451,8 +483,9 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
 
         // sg: stage s >= 2 gathers into it (the row it collides next
         // iteration); sx: the row it collides now, gathered one iteration earlier
-        auto row = [&](const int r, float (&pa)[2][Q], float (&pb)[2][Q], float (&pn)[2][Q], float (&sg)[K - 1][2][Q],
-                       float (&sx)[K - 1][2][Q]) {
+        auto row = [&](auto ph, const int r, float (&pa)[2][Q], float (&pb)[2][Q], float (&pn)[2][Q],
+                       float (&sg)[K - 1][2][Q], float (&sx)[K - 1][2][Q]) {
+            const RowAnchor an{r, decltype(ph)::value};
             // (a) stage 1: loads of row r + 2 (the last row reloads load_last,
             // cached, rather than branching round the loads); densities of r + 1
             const unsigned t_r3 = tri_of_words();
@@ -462,18 +495,18 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
             t_r2 = t_r3;
             gather1(min(r + 2, load_last), nbw[0], pn);
 #pragma unroll
-            for (int c = 0; c < 2; ++c) L.d[0][r3(r + 1)][c][t] = dens(pb[c]);
+            for (int c = 0; c < 2; ++c) L.d[0][r3(an, r + 1)][c][t] = dens(pb[c]);
             __syncthreads();
             // stage 1: collide row r into ring 0
             if (t >= 1 && t < T - 1 && (OPEN || (nbw[2] & 1u))) {
                 float nr[2][Q], rc[2], oa[Q], ob[Q];
-                nbr_dens(0, r, nr, rc);
+                nbr_dens(an, 0, r, nr, rc);
                 collide2_core<float, UeqStage, FORCE>(a, UeqStage{A.u, 0}, wr(r) * a.pitch + X, pa, rc, nr, OPEN ? 0x1FFu : nbw[2], oa, ob);
                 if constexpr (K > 1) {
 #pragma unroll
                     for (int i = 1; i < Q; ++i) {
-                        pr(0, r, 0, i, t) = ST::enc(oa[i]);
-                        pr(0, r, 1, i, t) = ST::enc(ob[i]);
+                        pr(an, 0, r, 0, i, t) = ST::enc(oa[i]);
+                        pr(an, 0, r, 1, i, t) = ST::enc(ob[i]);
                     }
                     rest_now[0][0] = ST::dec(ST::enc(oa[0]));
                     rest_now[0][1] = ST::dec(ST::enc(ob[0]));
@@ -494,7 +527,7 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
 #pragma unroll
                             for (int i = 1; i < Q; ++i) {
                                 const bool src_fluid = OPEN || ((nbg >> opp(i)) & 1u);
-                                const S v = src_fluid ? pr(q, g - cy(i), c, i, t - cx(i)) : pr(q, g, c, opp(i), t);
+                                const S v = src_fluid ? pr(an, q, g - cy(i), c, i, t - cx(i)) : pr(an, q, g, c, opp(i), t);
                                 sg[q][c][i] = ST::dec(v);
                             }
                         }
@@ -505,7 +538,7 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
                             for (int i = 0; i < Q; ++i) sg[q][c][i] = 0.f;
                     }
 #pragma unroll
-                    for (int c = 0; c < 2; ++c) L.d[s - 1][r3(g)][c][t] = dens(sg[q][c]);
+                    for (int c = 0; c < 2; ++c) L.d[s - 1][r3(an, g)][c][t] = dens(sg[q][c]);
                 }
                 __syncthreads();
                 // collision of stage s at row h = r - 2(s - 1)
@@ -514,12 +547,12 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
                 if (s < K) {
                     if (h >= y0 - 2 * (K - s) && t >= 2 * s - 1 && t < T - 2 * s + 1 && (OPEN || (nbh & 1u))) {
                         float nr[2][Q], rc[2], oa[Q], ob[Q];
-                        nbr_dens(s - 1, h, nr, rc);
+                        nbr_dens(an, s - 1, h, nr, rc);
                         collide2_core<float, UeqStage, FORCE>(a, UeqStage{A.u, s - 1}, wr(h) * a.pitch + X, sx[q], rc, nr, OPEN ? 0x1FFu : nbh, oa, ob);
 #pragma unroll
                         for (int i = 1; i < Q; ++i) {
-                            pr(s - 1, h, 0, i, t) = ST::enc(oa[i]);
-                            pr(s - 1, h, 1, i, t) = ST::enc(ob[i]);
+                            pr(an, s - 1, h, 0, i, t) = ST::enc(oa[i]);
+                            pr(an, s - 1, h, 1, i, t) = ST::enc(ob[i]);
                         }
                         rest_now[s - 1][0] = ST::dec(ST::enc(oa[0]));
                         rest_now[s - 1][1] = ST::dec(ST::enc(ob[0]));
@@ -527,7 +560,7 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
                     __syncthreads();
                 } else if (h >= y0 && h < y1 && out_lane && (OPEN || (nbh & 1u))) {
                     float nr[2][Q], rc[2], oa[Q], ob[Q];
-                    nbr_dens(s - 1, h, nr, rc);
+                    nbr_dens(an, s - 1, h, nr, rc);
                     collide2_core<float, UeqStage, FORCE>(a, UeqStage{A.u, s - 1}, h * a.pitch + X, sx[q], rc, nr, OPEN ? 0x1FFu : nbh, oa, ob);
                     const OffT home = (OffT)h * pitch_b + (OffT)X * ES;
 #pragma unroll
