@@ -416,9 +416,10 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
                                                           OPEN ? 0x1FFu : nbh, oa, ob);
                     const OffT home = (OffT)h * pitch_b + (OffT)X * ES;
 #pragma unroll
-                    for (int i = 0; i < Q; ++i) {
-                        ST::stc((S*)((char*)A.dst[0] + (size_t)i * plane_b + home), oa[i]);
-                        ST::stc((S*)((char*)A.dst[1] + (size_t)i * plane_b + home), ob[i]);
+                    for (int i = 0; i < Q; ++i) {   // 32-bit offsets: a component's buffer is < 4 GiB
+                        const OffT o = home + (OffT)i * plane_b;
+                        ST::stc((S*)((char*)A.dst[0] + o), oa[i]);
+                        ST::stc((S*)((char*)A.dst[1] + o), ob[i]);
                     }
                 }
             }
@@ -564,9 +565,10 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
                     collide2_core<float, UeqStage, FORCE>(a, UeqStage{A.u, s - 1}, h * a.pitch + X, sx[q], rc, nr, OPEN ? 0x1FFu : nbh, oa, ob);
                     const OffT home = (OffT)h * pitch_b + (OffT)X * ES;
 #pragma unroll
-                    for (int i = 0; i < Q; ++i) {
-                        ST::stc((S*)((char*)A.dst[0] + (size_t)i * plane_b + home), oa[i]);
-                        ST::stc((S*)((char*)A.dst[1] + (size_t)i * plane_b + home), ob[i]);
+                    for (int i = 0; i < Q; ++i) {   // 32-bit offsets: a component's buffer is < 4 GiB
+                        const OffT o = home + (OffT)i * plane_b;
+                        ST::stc((S*)((char*)A.dst[0] + o), oa[i]);
+                        ST::stc((S*)((char*)A.dst[1] + o), ob[i]);
                     }
                 }
             }
