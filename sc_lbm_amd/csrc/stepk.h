@@ -124,12 +124,22 @@ __host__ __device__ constexpr bool rename_rows() {
 
 // Whether the three-row ring slots are compile-time constants (the LDS
 // addresses then need no per-row scalar slot arithmetic: 20% fewer scalar
-// and 3.5% fewer vector instructions in the three-step tiles).  Elsewhere
-// the hoisted addresses cost occupancy: FP16 two steps 164 -> 178 VGPRs
-// (three waves per SIMD to two), FP32 three steps with solids past 256.
+// and 3.5% fewer vector instructions in the three-step tiles).  With solids
+// the registers are short (FP32 three steps: past 256 VGPRs).
 template <class ST, bool OPEN, int K>
 __host__ __device__ constexpr bool const_slots() {
-    return OPEN && K == 3;
+    return OPEN && (K == 3 || sizeof(typename ST::S) == 2);
+}
+
+// Whether each row takes the lane index afresh (an empty asm that the
+// compiler must assume changes it), so that the LDS addresses derived from
+// it are formed inside the row rather than hoisted out of the loop, each in
+// a register of its own.  FP16 two steps: 164 -> 124 VGPRs, four waves per
+// SIMD instead of three.  The FP32 tiles are held to their occupancy by LDS,
+// where the hoisted addresses save instructions.
+template <class ST, bool OPEN, int K>
+__host__ __device__ constexpr bool fresh_lane() {
+    return OPEN && K == 2 && sizeof(typename ST::S) == 2;
 }
 
 // A row index and its ring phase: (r - the tile's first row) modulo 3 (6
@@ -198,7 +208,7 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
         else return neigh_bits(s, c, n);
     };
 
-    const int t = threadIdx.x;
+    int t = threadIdx.x;   // (fresh_lane)
     const int x0 = bx * OUT;
     int X = x0 - H + t;   // one wrap at most (nx >= T)
     X = X < 0 ? X + nx : (X >= nx ? X - nx : X);
@@ -332,6 +342,7 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
         auto row = [&](auto ph, const int r, float (&pa)[2][Q], float (&pb)[2][Q], float (&pn)[2][Q],
                        float (&sg)[K - 1][2][Q], float (&sx)[K - 1][2][Q]) {
             const RowAnchor an{r, decltype(ph)::value};
+            if constexpr (fresh_lane<ST, OPEN, K>()) asm volatile("" : "+v"(t));
             // (A) stage 1: loads of row r + 2 (past the band the last row
             // reloads load_last, cached, rather than branching round the loads);
             // densities of r + 1
@@ -487,6 +498,7 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
         auto row = [&](auto ph, const int r, float (&pa)[2][Q], float (&pb)[2][Q], float (&pn)[2][Q],
                        float (&sg)[K - 1][2][Q], float (&sx)[K - 1][2][Q]) {
             const RowAnchor an{r, decltype(ph)::value};
+            if constexpr (fresh_lane<ST, OPEN, K>()) asm volatile("" : "+v"(t));
             // (a) stage 1: loads of row r + 2 (the last row reloads load_last,
             // cached, rather than branching round the loads); densities of r + 1
             const unsigned t_r3 = tri_of_words();
