@@ -417,15 +417,17 @@ bool x2_wanted(const scl_geometry* g, int precision) {
     return g->fusion >= 2 || g->x2_tuned[precision] >= 0;
 }
 
-// Workgroups of the two-step kernel resident per CU (registers and LDS).
+// Workgroups of the two-step kernel resident per CU (registers and LDS), of
+// its force-free variant: the one the autotuner times (FP16 solid-free: four,
+// the variant with a force three).
 int x2_occupancy(int precision, bool open) {
     int& o = g_lib.x2_occ[precision][open ? 1 : 0];
     if (!o) {
         const void* k = precision == SCL_FP32_DEV
-                            ? (open ? reinterpret_cast<const void*>(&k_step2x<StoreF32, true, true>)
-                                    : reinterpret_cast<const void*>(&k_step2x<StoreF32, false, true>))
-                            : (open ? reinterpret_cast<const void*>(&k_step2x<StoreF16, true, true>)
-                                    : reinterpret_cast<const void*>(&k_step2x<StoreF16, false, true>));
+                            ? (open ? reinterpret_cast<const void*>(&k_step2x<StoreF32, true, false>)
+                                    : reinterpret_cast<const void*>(&k_step2x<StoreF32, false, false>))
+                            : (open ? reinterpret_cast<const void*>(&k_step2x<StoreF16, true, false>)
+                                    : reinterpret_cast<const void*>(&k_step2x<StoreF16, false, false>));
         int n = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, kX2Threads, 0) != hipSuccess || n < 1) n = 2;
         o = n;
@@ -444,7 +446,7 @@ int x2_wave_rows(const scl_geometry* g, int precision, int waves) {
 }
 
 int x2_rows_for(const scl_geometry* g, int precision) {
-    // default: two whole waves (52 rows at 4096^2 with FP32 storage, 46 with FP16)
+    // default: two whole waves (52 rows at 4096^2 with FP32 storage, 35 with FP16)
     const int r = g->x2_rows ? g->x2_rows
                              : (g->x2_tuned[precision] > 0 ? g->x2_tuned[precision] : x2_wave_rows(g, precision, 2));
     return std::min(r, g->ny);
@@ -522,7 +524,7 @@ int x3_occupancy() {
     if (!o) {
         int n = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &n, reinterpret_cast<const void*>(&k_stepk<ST, OPEN, 3, x3_threads<ST, OPEN>(), true>),
+                &n, reinterpret_cast<const void*>(&k_stepk<ST, OPEN, 3, x3_threads<ST, OPEN>(), false>),
                 x3_threads<ST, OPEN>(), 0) != hipSuccess ||
             n < 1)
             n = 1;
