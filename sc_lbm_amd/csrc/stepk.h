@@ -10,9 +10,10 @@
 // the result is bit for bit that of K single steps.
 //
 // Work decomposition.  A workgroup of T threads owns one column per thread
-// and walks a band of rows from south to north.  Stage s (1..K) collides row
-// r - 2(s-1) while stage 1 collides row r; stage s >= 2 first gathers row
-// r - 2s + 3 from the post-collision ring of stage s-1.  Stage s is valid on
+// and walks a band of rows from south to north.  While stage 1 collides row
+// r, stage s (1..K) collides row r - 2(s-1) (per-stage schedule) or
+// r - 3(s-1) (two-phase schedule), having gathered it from the
+// post-collision ring of stage s-1 in the iteration before.  Stage s is valid on
 // columns 2s-1 .. T-2s of the strip, so a strip writes OUT columns from
 // column H = 2K-1 on (OUT: a multiple of 32 with FP32 storage, so no
 // 128-byte line is written by two strips) and bands recompute 2(K-1) rows
