@@ -338,6 +338,38 @@ def test_two_steps_per_launch_equal_single_steps(prec, name):
         assert np.array_equal(out[0][c], out[2][c])
 
 
+@pytest.mark.parametrize("prec", [nat.FP32_DEV, nat.FP16_DEV])
+def test_multi_step_kernels_at_every_band_height_remainder(prec):
+    """The multi-step tiles walk their rows in a loop unrolled by 3 or 6,
+    with ring slots fixed per unrolled row: band heights 20..25 end the
+    walk at every remainder (a band runs its height plus 8 or 10 rows), and
+    the last band of the 300-row lattice is partial.  Two and three steps
+    per launch equal single steps bit for bit at each."""
+    case = wl.spinodal(300, G=1.2, seed=9)
+
+    def run(fusion, band):
+        g = nat.Geometry(case["mask"])
+        g.set_step_fusion(fusion, band)
+        fl = [nat.Field(g, case["tau"][c], prec, 1.0, g_solid=case["g_solid"][c], g_fluid=case["g_fluid"][c])
+              for c in range(2)]
+        for c in range(2):
+            fl[c].set_f(case["f"][c])
+        nat.solid_surface_force(g, fl[0], fl[1])
+        nat.step2(g, fl[0], fl[1], None, None, 13)   # bootstrap step, then 12
+        out = [f.get_f() for f in fl]
+        for f in fl:
+            f.close()
+        g.close()
+        return out
+
+    want = run(1, 0)
+    for band in range(20, 26):
+        for fusion in (2, 3):
+            got = run(fusion, band)
+            for c in range(2):
+                assert np.array_equal(got[c], want[c]), (fusion, band, c)
+
+
 @pytest.mark.parametrize("prec", [nat.FP64, nat.FP32_DEV, nat.FP16_DEV])
 @pytest.mark.parametrize("name", ["porous", "dry"])
 def test_checkpoint_restore_continues_bit_for_bit(prec, name):
