@@ -606,12 +606,13 @@ class GpuServe final : public serve::Backend {
 
     // Tile shape by measurement: two resident 2048 x 2048 porous jobs
     // (porosity 0.8, disks of radius 8..20, droplet initial state) in
-    // scratch memory; every (width 128/256, lookahead 1/2, rows 16/32)
-    // runs 2 + 8 steps and the fastest mean step wins.  Taller tiles can
-    // win on two large jobs but leave the small lattices of a served mix
-    // short of workgroups: on a mix of sides 128..2048 the capacity was
-    // 34.4k / 33.4k / 30.2k MLUPS with 16 / 32 / 64 rows (256 columns,
-    // lookahead 2), and the probe had chosen 64.
+    // scratch memory; every (width 128/256, lookahead 1/2) of 16-row tiles
+    // runs 2 + 8 steps and the fastest mean step wins.  The rows are not
+    // tuned on the probe: taller tiles win on two large jobs but leave the
+    // small lattices of a served mix short of workgroups.  On a mix of
+    // sides 128..2048 the capacity was 34.4k-35.1k / 33.0k-33.4k / 30.2k
+    // MLUPS with 16 / 32 / 64 rows (256 columns, lookahead 2), and the
+    // probe had chosen 64, then 32 once 64 was excluded.
     int autotune(std::string* err) {
         JobSpec s;
         s.nx = s.ny = 2048;
@@ -648,7 +649,7 @@ class GpuServe final : public serve::Backend {
         }
         float best = 1e30f;
         int bnw = 4, brows = 32, bla = 1;
-        const int nws[] = {2, 4}, las[] = {1, 2}, rowss[] = {16, 32};
+        const int nws[] = {2, 4}, las[] = {1, 2}, rowss[] = {16};
         for (int nw : nws)
             for (int la : las)
                 for (int rows : rowss) {
