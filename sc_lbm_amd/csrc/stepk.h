@@ -124,23 +124,25 @@ __host__ __device__ constexpr bool rename_rows() {
 }
 
 // Whether the three-row ring slots are compile-time constants (the LDS
-// addresses then need no per-row scalar slot arithmetic: 20% fewer scalar
-// and 3.5% fewer vector instructions in the three-step tiles).  With solids
-// the registers are short (FP32 three steps: past 256 VGPRs).
+// addresses then need no per-row scalar slot arithmetic: 20-37% fewer
+// scalar and 2.5-3.7% fewer vector instructions).  With solids only
+// together with fresh_lane (alone, the FP32 three-step tile went past 256
+// VGPRs).
 template <class ST, bool OPEN, int K>
 __host__ __device__ constexpr bool const_slots() {
-    return OPEN && (K == 3 || sizeof(typename ST::S) == 2);
+    return OPEN ? (K == 3 || sizeof(typename ST::S) == 2) : true;
 }
 
 // Whether each row takes the lane index afresh (an empty asm that the
 // compiler must assume changes it), so that the LDS addresses derived from
 // it are formed inside the row rather than hoisted out of the loop, each in
 // a register of its own.  FP16 two steps: 164 -> 124 VGPRs, four waves per
-// SIMD instead of three.  The FP32 tiles are held to their occupancy by LDS,
-// where the hoisted addresses save instructions.
+// SIMD instead of three; with solids it keeps the compile-time slots within
+// the registers of the same occupancy.  The solid-free FP32 tiles are held
+// to their occupancy by LDS, where the hoisted addresses save instructions.
 template <class ST, bool OPEN, int K>
 __host__ __device__ constexpr bool fresh_lane() {
-    return OPEN && K == 2 && sizeof(typename ST::S) == 2;
+    return OPEN ? (K == 2 && sizeof(typename ST::S) == 2) : true;
 }
 
 // A row index and its ring phase: (r - the tile's first row) modulo 3 (6
