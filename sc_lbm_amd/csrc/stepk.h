@@ -112,15 +112,15 @@ __host__ __device__ constexpr bool two_phase() {
 
 // Whether the row loop is unrolled by 6, so that the two register arrays of
 // stages s >= 2 (the row gathered for the next iteration, the row collided
-// now) swap roles by name, or by 3 with a copy per row.  Renaming saves 6% of
-// the vector instructions; it is kept where it measured faster, the
-// solid-free three-step tiles (FP32 4096^2: 76.8k -> 78.0k MLUPS, FP16:
-// 74.3k -> 75.1k).  With two steps it gained nothing (FP32) or cost
-// occupancy (FP16: 164 -> 177 VGPRs, three waves per SIMD to two); the
-// variants with solids spill scalar registers.
+// now) swap roles by name, or by 3 with a copy per row.  Renaming saves 6-8%
+// of the vector instructions; it is kept where it measured faster or equal,
+// the three-step tiles (solid-free FP32 4096^2: 76.8k -> 78.0k MLUPS, FP16:
+// 74.3k -> 75.1k; porous FP32 equal, FP16 45.1k -> 46.6k).  With two steps
+// it gained nothing (FP32) or cost occupancy (FP16: 164 -> 177 VGPRs, three
+// waves per SIMD to two).
 template <class ST, bool OPEN, int K>
 __host__ __device__ constexpr bool rename_rows() {
-    return OPEN && K == 3;
+    return K == 3;
 }
 
 // Whether the three-row ring slots are compile-time constants (the LDS
