@@ -323,7 +323,8 @@ typedef struct scl_server_config {
     int max_jobs;               /* resident job slots                                  */
     int max_pending;            /* queued requests before submit returns SCL_ERR_BUSY */
     int round_steps;            /* steps per round (graph length), 1..1024            */
-    int tile_cols, tile_rows, lookahead;    /* persistent-kernel tile; 0 = autotuned  */
+    int tile_cols, tile_rows, lookahead;    /* persistent-kernel tile; tile_cols 0 =
+                                   width and lookahead timed at start-up, 16 rows */
     double mass_tol;            /* watchdog: |mass - mass0| > mass_tol*|mass0| fails a job;
                                    0 = by precision (1e-6 FP64, 1e-3 FP32, 0.1 FP16) */
     int max_bypass;             /* later jobs admitted past a waiting one before it blocks admission */
@@ -351,7 +352,8 @@ typedef struct scl_server_stats {
 } scl_server_stats;
 
 /* Defaults: FP32 deviations, 8 GiB arena, 64 slots, 4096 pending, rounds of
- * 32 steps, autotuned tile, mass_tol by precision, max_bypass 16. */
+ * 32 steps, tile timed at start-up (16 rows), mass_tol by precision,
+ * max_bypass 16. */
 int scl_server_default_config(scl_server_config* cfg);
 /* Creates the server on the library's device and starts its request loop. */
 int scl_server_create(const scl_server_config* cfg, scl_server** out);
