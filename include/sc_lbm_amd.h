@@ -324,7 +324,7 @@ typedef struct scl_server_config {
     int max_pending;            /* queued requests before submit returns SCL_ERR_BUSY */
     int round_steps;            /* steps per round (graph length), 1..1024            */
     int tile_cols, tile_rows, lookahead;    /* persistent-kernel tile; tile_cols 0 =
-                                   width and lookahead timed at start-up, 16 rows */
+                                   width and lookahead timed at start-up; rows 0 = 16 */
     double mass_tol;            /* watchdog: |mass - mass0| > mass_tol*|mass0| fails a job;
                                    0 = by precision (1e-6 FP64, 1e-3 FP32, 0.1 FP16) */
     int max_bypass;             /* later jobs admitted past a waiting one before it blocks admission */

@@ -157,7 +157,7 @@ class GpuServe final : public serve::Backend {
         mass_grid_ = 4 * cus_;
         if (cfg_.tile_cols) {
             nw_ = cfg_.tile_cols / 64;
-            rows_ = cfg_.tile_rows ? cfg_.tile_rows : 32;
+            rows_ = cfg_.tile_rows ? cfg_.tile_rows : 16;
             la_ = cfg_.lookahead ? cfg_.lookahead : 1;
             if ((nw_ != 2 && nw_ != 4) || nw_ * 64 != cfg_.tile_cols) {
                 *err = "server tiles are 128 or 256 columns wide";
@@ -736,7 +736,7 @@ class GpuServe final : public serve::Backend {
     double* partial_d_ = nullptr;
     double* mass_d_ = nullptr;
     double* mass_h_ = nullptr;
-    int nw_ = 4, rows_ = 32, la_ = 1, grid_ = 0, mass_grid_ = 1024;
+    int nw_ = 4, rows_ = 16, la_ = 1, grid_ = 0, mass_grid_ = 1024;
     bool x2_ = false;                  // two steps per launch for wide jobs (deviation forms)
     int x2rows_ = 32, grid2_ = 0;      // band rows of the two-step tiles; its persistent grid
     uint64_t* items2_d_ = nullptr;
