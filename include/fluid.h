@@ -13,13 +13,19 @@
 #ifndef SCLBM_FLUID_H
 #define SCLBM_FLUID_H
 
+#include <algorithm>
 #include <cstddef>
+#include <iostream>
+#include <random>
 #include <string>
 #include <vector>
 
+#include "common.h"
 #include "geometry.h"
+#include "io_operations/lbm_io.h"
 #include "rng.h"
 #include "sc_lbm_amd.h"
+#include "utils.h"
 
 class LBM;
 namespace sclbm_detail {
@@ -152,11 +158,16 @@ private:
     const std::vector<double>& chost(int a) const;
     void sync_host(int a) const;
     scl_field* device_field(const Geometry& geom);
+    void bind_geometry(const Geometry& geom);
     void push_device();
     void device_wrote() const;
     void release_device();
     void write_var(const std::vector<double>& variable, const std::string& fname) const;
     void write_var(const std::vector<double>& variable, const std::string& fname, const bool is_3D) const;
 };
+
+// the reference's fluid.h includes lbm.h (include/fluid.h:9): drivers that
+// include only fluid.h use LBM too
+#include "lbm.h"
 
 #endif

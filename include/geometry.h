@@ -5,7 +5,6 @@
  * text I/O.  The mask is mirrored to the GPU (one bit per site) the first
  * time a device operation needs it and again after every change.
  *
- * Not provided: add_array (object arrays, unused by the step).
  */
 #ifndef SCLBM_GEOMETRY_H
 #define SCLBM_GEOMETRY_H
@@ -15,7 +14,9 @@
 #include <string>
 #include <vector>
 
+#include "common.h"
 #include "sc_lbm_amd.h"
+#include "utils.h"
 
 class Geometry {
 public:
@@ -36,6 +37,19 @@ public:
     void add_square(const size_t Ls, const size_t xc, const size_t yc);
     void add_ellipse(const size_t Lx, const size_t Ly, const size_t xc, const size_t yc);
     void add_circle(const size_t D, const size_t xc, const size_t yc);
+
+    /// Regular arrays of a built-in object (src/geometry.cpp:182-240,
+    /// src/arrays/regular_array.cpp:11-100).  object_properties are the
+    /// arguments of the matching add_ call ({Lx, Ly, xc, yc} for "rectangle"
+    /// and "ellipse", {L, xc, yc} for "square" and "circle", case-insensitive);
+    /// array_bounds = {{x lower, x upper}, {y lower, y upper}}.  The first form
+    /// takes the gaps between neighbouring objects, the second the numbers of
+    /// objects in x and y.  alpha (staggering) is unused, as in the reference.
+    void add_array(const std::vector<size_t>& object_properties, const std::vector<std::vector<size_t>>& array_bounds,
+                   const size_t dx, const size_t dy, const std::string& object_type, const size_t alpha = 0);
+    void add_array(const std::vector<size_t>& object_properties, const std::vector<std::vector<size_t>>& array_bounds,
+                   const std::vector<size_t>& object_numbers, const std::string& object_type,
+                   const size_t alpha = 0);
 
     /// Bounds checks (src/geometry.cpp:150-175); throw std::runtime_error
     void check_object_bounds(const size_t Lx, const size_t Ly, const size_t xc, const size_t yc,

@@ -19,12 +19,17 @@
 #ifndef SCLBM_LBM_H
 #define SCLBM_LBM_H
 
+#include <algorithm>
+#include <iostream>
 #include <vector>
 
+#include "common.h"
 #include "fluid.h"
 #include "geometry.h"
+#include "io_operations/lbm_io.h"
 #include "rng.h"
 #include "sc_lbm_amd.h"
+#include "utils.h"
 
 class LBM {
 public:

@@ -112,7 +112,9 @@ int scl_field_download(scl_field* f, int which, double* dst, size_t n);
  * field restored from it continues bit for bit as the original would.  The
  * fluid-solid force is not part of it: call scl_solid_surface_force on the
  * restored fields as after creation.  scl_field_restore needs a field of the
- * same size, precision and rho0 (SCL_ERR_ARG otherwise); bytes must equal
+ * same size, precision and rho0 on a geometry with the same mask (the
+ * checkpoint records a hash of it: solid sites hold bounce-back slots), and
+ * returns SCL_ERR_ARG otherwise; bytes must equal
  * scl_field_checkpoint_size. */
 int scl_field_checkpoint_size(const scl_field* f, size_t* bytes);
 int scl_field_checkpoint(scl_field* f, void* dst, size_t bytes);

@@ -90,6 +90,9 @@ def ref_lib():
         lib.ref_geometry.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_size_t, ctypes.c_int,
                                      ctypes.c_int, _szp, _szp, _szp, _ip]
         lib.ref_add_rectangle.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_size_t] * 4 + [_ip]
+        lib.ref_write_f.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p]
+        lib.ref_geometry_object.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int,
+                                            _szp, _szp, ctypes.c_size_t, ctypes.c_size_t, _ip]
         lib._extra = True
     return lib
 
@@ -198,6 +201,20 @@ def add_rectangle(mask, lx, ly, xc, yc):
     if rc:
         raise RuntimeError("rectangle does not fit the domain")
     return m
+
+
+def ref_geometry_object(nx, ny, kind, props, obj_type="", bounds=(0, 0, 0, 0), u=0, v=0):
+    """Mask built by the compiled reference's Geometry from one object call:
+    kind 0 add_ellipse, 1 add_square, 2 add_array by gaps (u, v), 3 add_array
+    by numbers of objects (u, v).  Returns (status, mask); status 0 ok,
+    1 runtime_error, 2 invalid_argument, 3 out_of_range, 4 other."""
+    lib = ref_lib()
+    p = np.array(list(props) or [0], dtype=np.uint64)
+    b = np.array(bounds, dtype=np.uint64)
+    out = np.zeros(nx * ny, dtype=np.int32)
+    rc = lib.ref_geometry_object(nx, ny, kind, obj_type.encode(), len(props), p.ctypes.data_as(_szp),
+                                 b.ctypes.data_as(_szp), u, v, out.ctypes.data_as(_ip))
+    return rc, out.reshape(ny, nx)
 
 
 def ref_geometry(nx, ny, dh=0, where="x", circles=()):

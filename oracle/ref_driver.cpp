@@ -153,6 +153,47 @@ int ref_add_rectangle(int nx, int ny, size_t lx, size_t ly, size_t xc, size_t yc
     return 0;
 }
 
+/* Geometry(nx, ny) + one object call: kind 0 add_ellipse(p0, p1, p2, p3),
+ * 1 add_square(p0, p1, p2), 2 add_array by gaps, 3 add_array by numbers
+ * (object type `type`, nprops properties, bounds {x0, x1, y0, y1}, then the
+ * two gaps or counts u, v).  Returns 0, or 1 runtime_error, 2 invalid_argument,
+ * 3 out_of_range, 4 another exception; the mask is written only on success. */
+int ref_geometry_object(int nx, int ny, int kind, const char* type, int nprops, const size_t* p,
+                        const size_t* bounds, size_t u, size_t v, int* out_mask) {
+    try {
+        Geometry g((size_t)nx, (size_t)ny);
+        std::vector<size_t> props(p, p + nprops);
+        std::vector<std::vector<size_t>> b = {{bounds[0], bounds[1]}, {bounds[2], bounds[3]}};
+        if (kind == 0) g.add_ellipse(p[0], p[1], p[2], p[3]);
+        else if (kind == 1) g.add_square(p[0], p[1], p[2]);
+        else if (kind == 2) g.add_array(props, b, u, v, type);
+        else g.add_array(props, b, std::vector<size_t>{u, v}, type);
+        const std::vector<int>& m = g.get_geom();
+        std::memcpy(out_mask, m.data(), m.size() * sizeof(int));
+    } catch (const std::invalid_argument&) {
+        return 2;
+    } catch (const std::out_of_range&) {
+        return 3;
+    } catch (const std::runtime_error&) {
+        return 1;
+    } catch (...) {
+        return 4;
+    }
+    return 0;
+}
+
+/* Fluid::write_f(prefix) of component c: the reference's 6-digit text files
+ * prefix_0.txt .. prefix_8.txt (src/fluid.cpp:346-353). */
+int ref_write_f(void* h, int c, const char* prefix) {
+    RefSys* s = static_cast<RefSys*>(h);
+    try {
+        s->fl[c]->write_f(prefix);
+    } catch (...) {
+        return 1;
+    }
+    return 0;
+}
+
 int ref_init_droplet(void* h, double rb, double rd, double rbd, double rdb,
                      double xc, double yc, double r) {
     RefSys* s = static_cast<RefSys*>(h);

@@ -35,20 +35,23 @@ CXX_FLAGS = ["-std=c++14", "-O2", "-fPIC", "-Wall", "-pthread", "-I" + INCLUDE, 
 
 def build_library():
     """libsclbm.so = the HIP kernels and C-ABI (hipcc) + the C++ drop-in
-    classes Geometry / Fluid / LBM (g++), linked into one shared library."""
+    classes Geometry / Fluid / LBM (g++), linked into one shared library.
+    The translation units compile in parallel."""
+    from concurrent.futures import ThreadPoolExecutor
     bdir = os.path.join(HERE, "_build")
     os.makedirs(bdir, exist_ok=True)
-    objs = []
+    jobs = []
     for src in sorted(os.listdir(CSRC)):
         if src.endswith(".hip"):
             o = os.path.join(bdir, src[:-4] + ".o")
-            _run([HIPCC] + HIP_FLAGS + ["-c", "-o", o, os.path.join(CSRC, src)])
-            objs.append(o)
+            jobs.append((o, [HIPCC] + HIP_FLAGS + ["-c", "-o", o, os.path.join(CSRC, src)]))
     for src in sorted(os.listdir(HOST)):
         if src.endswith(".cpp"):
             o = os.path.join(bdir, src[:-4] + ".o")
-            _run(["g++"] + CXX_FLAGS + ["-c", "-o", o, os.path.join(HOST, src)])
-            objs.append(o)
+            jobs.append((o, ["g++"] + CXX_FLAGS + ["-c", "-o", o, os.path.join(HOST, src)]))
+    with ThreadPoolExecutor(max_workers=min(8, len(jobs))) as pool:
+        list(pool.map(lambda j: _run(j[1]), jobs))
+    objs = [o for o, _ in jobs]
     out = os.path.join(HERE, "libsclbm.so")
     tmp = out + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread"])
@@ -61,6 +64,13 @@ def build_oracle():
     _run(["make", "-C", od, "liboracle.so"])
     if os.path.isdir("/root/reference/src"):
         _run(["make", "-C", od, "ref"])
+        # the reference's drivers and unit tests compiled against the drop-in
+        sys.path.insert(0, ROOT)
+        from oracle import dropin_refs
+        res = dropin_refs.build()
+        bad = sorted(k for k, v in res.items() if v[0] == "fail")
+        if bad:
+            raise RuntimeError("reference programs failed to compile against the drop-in: %s" % bad)
 
 
 if __name__ == "__main__":

@@ -116,6 +116,7 @@ struct scl_geometry {
     long long plane = 0;
     uint64_t* mask = nullptr;
     std::vector<int> host_mask;
+    uint64_t mask_hash = 0;             // FNV-1a of the packed mask (checkpoints record it)
     // Launch shape of the fused kernel: the caller's choice (scl_set_tile /
     // scl_set_lookahead; 0 = not set), else the autotuner's per component
     // count and precision (scl_autotune), else the built-in default.
@@ -814,6 +815,19 @@ int scl_geometry_create(int nx, int ny, const int* mask, scl_geometry** out) {
     for (int y = 0; y < ny; ++y)
         for (int x = 0; x < nx; ++x)
             if (mask[(size_t)y * nx + x] == 1) bits[(size_t)y * g->wpr + (x >> 6)] |= 1ull << (x & 63);
+    // FNV-1a over the size and the packed words: a checkpoint's populations
+    // (bounce-back slots included) belong to exactly this mask
+    uint64_t hsh = 1469598103934665603ull;
+    auto mix = [&hsh](uint64_t w) {
+        for (int b = 0; b < 8; ++b) {
+            hsh ^= (w >> (8 * b)) & 0xff;
+            hsh *= 1099511628211ull;
+        }
+    };
+    mix((uint64_t)nx);
+    mix((uint64_t)ny);
+    for (uint64_t w : bits) mix(w);
+    g->mask_hash = hsh;
     CHK(dalloc(&g->mask, bits.size()));
     HIPCHK(hipMemcpy(g->mask, bits.data(), bits.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
     CHK(dalloc(&g->part, (size_t)3 * ny + 3));
@@ -1124,8 +1138,9 @@ struct CkptHeader {
     int32_t precision, nx, ny, phase;
     double rho0;
     uint64_t pop_bytes, ueq_bytes;
+    uint64_t mask_hash;                 // the geometry the populations belong to
 };
-constexpr char kCkptMagic[8] = {'S', 'C', 'L', 'C', 'K', 'P', 'T', '1'};
+constexpr char kCkptMagic[8] = {'S', 'C', 'L', 'C', 'K', 'P', 'T', '2'};
 
 size_t ckpt_pop_bytes(const scl_field* f) { return n_pop(f->g) * f->esize(); }
 size_t ckpt_ueq_bytes(const scl_field* f) { return 2 * n_sites(f->g) * sizeof(double); }
@@ -1151,6 +1166,7 @@ int scl_field_checkpoint(scl_field* f, void* dst, size_t bytes) {
     h.ny = f->g->ny;
     h.phase = f->phase;
     h.rho0 = f->rho0;
+    h.mask_hash = f->g->mask_hash;
     h.pop_bytes = ckpt_pop_bytes(f);
     h.ueq_bytes = ckpt_ueq_bytes(f);
     char* out = (char*)dst;
@@ -1180,6 +1196,9 @@ int scl_field_restore(scl_field* f, const void* src, size_t bytes) {
                     f->rho0);
     if (h.pop_bytes != ckpt_pop_bytes(f) || h.ueq_bytes != ckpt_ueq_bytes(f) || (h.phase != PH_S && h.phase != PH_P))
         return fail(SCL_ERR_ARG, "corrupt checkpoint header");
+    if (h.mask_hash != f->g->mask_hash)
+        return fail(SCL_ERR_ARG, "checkpoint of a field on another geometry (mask hash %016llx, field's %016llx)",
+                    (unsigned long long)h.mask_hash, (unsigned long long)f->g->mask_hash);
     CHK(materialize(f));
     const char* in = (const char*)src + sizeof h;
     HIPCHK(hipMemcpyAsync(f->f[f->cur], in, h.pop_bytes, hipMemcpyHostToDevice, g_lib.stream));

@@ -5,6 +5,8 @@
 // device field.  Initialisers write the populations on the host exactly as
 // the reference does (f_j = rho/9 on fluid sites, the same loop order and
 // RNG calls), and the next device operation uploads them.
+#include <cerrno>
+#include <cstring>
 #include <fstream>
 #include <iostream>
 #include <iterator>
@@ -121,6 +123,13 @@ scl_field* Fluid::device_field(const Geometry& geom) {
     return dev;
 }
 
+// The initialisers only bind the geometry: the device field is created by the
+// first device operation, so building and initialising a Fluid (and reading
+// its host arrays) needs no GPU.
+void Fluid::bind_geometry(const Geometry& geom) {
+    if (!dev) dev_geom = &geom;
+}
+
 // Upload every array whose device copy is stale (host copies that were
 // written, or handed out, since the last device operation).
 void Fluid::push_device() {
@@ -157,7 +166,7 @@ void Fluid::set_precision(int prec, double r0) {
     precision = prec;
     rho0 = r0;
     for (int a = 0; a < A_COUNT; ++a) dev_ok[a] = false;
-    if (g) device_field(*g);
+    dev_geom = g;   // the device field is created again by the next device operation
 }
 
 // ---- initialisers (src/fluid.cpp:43-151) ----------------------------------
@@ -176,7 +185,7 @@ void Fluid::zero_density_ini(const Geometry& geom) {
         arr[a].resize(Ntot * Ndir, 0.0);
         dev_ok[a] = false;
     }
-    device_field(geom);
+    bind_geometry(geom);
 }
 
 void Fluid::simple_ini(const Geometry& geom, const double rho_0, const bool mcmp) {
@@ -206,7 +215,7 @@ void Fluid::simple_ini(const Geometry& geom, const double rho_0, const bool mcmp
     sync_host(A_FEQ);
     arr[A_FEQ].resize(Ntot * Ndir, 0.0);
     dev_ok[A_FEQ] = false;
-    device_field(geom);
+    bind_geometry(geom);
 }
 
 void Fluid::initialize_randomly_perturbed_density(const Geometry& geom, const double rho_0, RNG& gen,
@@ -240,7 +249,7 @@ void Fluid::initialize_randomly_perturbed_density(const Geometry& geom, const do
     sync_host(A_FEQ);
     arr[A_FEQ].resize(Ntot * Ndir, 0.0);
     dev_ok[A_FEQ] = false;
-    device_field(geom);
+    bind_geometry(geom);
 }
 
 void Fluid::set_multicomponent_interactions(const double Gs) {
@@ -301,18 +310,24 @@ void Fluid::restart_from_file(const std::string& fname) {
     for (size_t id = 0; id < Ndir; ++id) {
         const std::string fn = fname + "_" + std::to_string(id) + ".txt";
         std::ifstream in(fn);
-        if (!in) throw std::ios_base::failure("Error opening file: " + fn);
+        if (!in) {
+            std::cerr << "Error opening file " << fn << std::endl;
+            throw std::ios_base::failure(std::strerror(errno));
+        }
+        // every line of the file is one row y (values separated by spaces);
+        // a missing row is std::out_of_range, as the reference's .at()
+        std::vector<std::vector<double>> rows;
         std::string line;
-        size_t yj = 0;
-        while (std::getline(in, line) && yj < Ny) {
+        while (std::getline(in, line)) {
             std::istringstream ss(line);
+            std::vector<double> r;
             double v;
-            size_t xi = 0;
-            while (ss >> v) {
-                f.at(id * Ntot + yj * Nx + xi) = v;
-                ++xi;
-            }
-            ++yj;
+            while (ss >> v) r.push_back(v);
+            rows.push_back(std::move(r));
+        }
+        for (size_t yj = 0; yj < Ny; ++yj) {
+            const std::vector<double>& r = rows.at(yj);
+            for (size_t xi = 0; xi < r.size(); ++xi) f.at(id * Ntot + yj * Nx + xi) = r[xi];
         }
     }
     dev_ok[A_F] = false;
@@ -320,7 +335,10 @@ void Fluid::restart_from_file(const std::string& fname) {
 
 void Fluid::write_var(const std::vector<double>& variable, const std::string& fname) const {
     std::ofstream out(fname, std::ios_base::out | std::ios_base::trunc);
-    if (!out) throw std::ios_base::failure("Error opening file: " + fname);
+    if (!out) {
+        std::cerr << "Error opening file " << fname << std::endl;
+        throw std::ios_base::failure(std::strerror(errno));
+    }
     for (size_t y = 0; y < Ny; ++y) {
         std::copy(variable.begin() + y * Nx, variable.begin() + (y + 1) * Nx,
                   std::ostream_iterator<double>(out, " "));

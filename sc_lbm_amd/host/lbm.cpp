@@ -30,8 +30,9 @@ struct Engine {
 
     static void prepare(Fluid* f, const Geometry* g) {
         if (!f->dev) {
-            if (!g) throw std::logic_error("Fluid is not initialised (no *_ini call)");
-            f->device_field(*g);
+            const Geometry* geom = g ? g : f->dev_geom;
+            if (!geom) throw std::logic_error("Fluid " + f->name + " is not initialised (no *_ini call)");
+            f->device_field(*geom);
         }
         f->push_device();
     }

@@ -73,8 +73,40 @@ def build():
            os.path.join(HERE, "sc_driver.cpp"), "-o", BIN, "-L" + lib, "-lsclbm",
            "-Wl,-rpath,$ORIGIN/../../../sc_lbm_amd"]
     subprocess.check_call(cmd)
+    build_dropin_host()
     build_serve_cpu()
     return BIN
+
+
+DROPIN_HOST = os.path.join(HERE, "bin", "dropin_host")
+
+
+def build_dropin_host():
+    """The builder / initialiser / text-I/O driver of the drop-in classes."""
+    os.makedirs(os.path.dirname(DROPIN_HOST), exist_ok=True)
+    lib = os.path.join(ROOT, "sc_lbm_amd")
+    cmd = ["g++", "-std=c++11", "-O2", "-Wall", "-I" + os.path.join(ROOT, "include"),
+           os.path.join(HERE, "dropin_host.cpp"), "-o", DROPIN_HOST, "-L" + lib, "-lsclbm",
+           "-Wl,-rpath,$ORIGIN/../../../sc_lbm_amd"]
+    subprocess.check_call(cmd)
+    return DROPIN_HOST
+
+
+def run_dropin_host(workdir, *cmds):
+    """Runs tests/drivers/bin/dropin_host with the given command tokens;
+    returns (status, mask, [f_A, f_B] or [], stdout)."""
+    out = os.path.join(workdir, "dropin_host.bin")
+    args = [DROPIN_HOST, out] + [repr(float(c)) if isinstance(c, float) else str(c) for c in cmds]
+    p = subprocess.run(args, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=600,
+                       cwd=workdir)
+    if p.returncode != 0:
+        raise RuntimeError("dropin_host failed (%d): %s" % (p.returncode, p.stdout[-2000:]))
+    with open(out, "rb") as fh:
+        status, nx, ny = struct.unpack("<3i", fh.read(12))
+        mask = np.frombuffer(fh.read(4 * nx * ny), dtype=np.int32).reshape(ny, nx).copy()
+        (nf,) = struct.unpack("<i", fh.read(4))
+        fs = [np.frombuffer(fh.read(8 * 9 * nx * ny), dtype=np.float64).copy() for _ in range(nf)]
+    return status, mask, fs, p.stdout
 
 
 SERVE_CPU = os.path.join(HERE, "bin", "libserve_cpu.so")

@@ -141,5 +141,49 @@ def main():
     print("wrote geometry_cases")
 
 
+# Object and array builders (kind 0 add_ellipse, 1 add_square, 2 add_array by
+# gaps, 3 add_array by numbers): (nx, ny, kind, type, props, bounds, u, v).
+# The array cases include the four array drivers' own calls
+# (benchmarks/single_phase_one_component_flows/x_flow_array_ellipses.cpp,
+# x_flow_array_staggered_circles.cpp, y_flow_array_squares.cpp,
+# y_flow_staggered_array_rectangles.cpp).
+OBJECT_CASES = [
+    (30, 20, 0, "", (7, 5, 10, 8), (0, 0, 0, 0), 0, 0),
+    (30, 20, 0, "", (8, 6, 12, 9), (0, 0, 0, 0), 0, 0),
+    (30, 20, 0, "", (11, 3, 15, 10), (0, 0, 0, 0), 0, 0),
+    (30, 20, 0, "", (9, 9, 3, 10), (0, 0, 0, 0), 0, 0),       # lower bound violated
+    (30, 20, 1, "", (5, 10, 8), (0, 0, 0, 0), 0, 0),
+    (30, 20, 1, "", (6, 20, 10), (0, 0, 0, 0), 0, 0),
+    (30, 20, 1, "", (7, 27, 10), (0, 0, 0, 0), 0, 0),         # upper bound violated
+    (200, 100, 2, "circle", (5, 10, 15), (5, 190, 3, 75), 10, 25),
+    (200, 100, 2, "circle", (7, 17, 29), (10, 182, 10, 100), 15, 25),
+    (100, 200, 3, "square", (5, 15, 10), (3, 70, 5, 190), 5, 20),
+    (200, 100, 3, "ellipse", (9, 5, 10, 10), (3, 190, 3, 95), 9, 6),
+    (100, 200, 2, "rectangle", (4, 7, 10, 10), (5, 90, 5, 190), 6, 9),
+    (100, 200, 3, "Rectangle", (5, 3, 12, 20), (6, 95, 10, 150), 6, 4),
+    (60, 40, 2, "ELLIPSE", (6, 4, 5, 5), (1, 55, 1, 35), 2, 3),
+    (60, 40, 2, "hexagon", (5, 5, 5), (1, 55, 1, 35), 2, 2),   # invalid type
+    (60, 40, 2, "circle", (5, 5, 5), (1, 80, 1, 35), 2, 2),    # past the lattice in x: wraps rows
+    (60, 40, 2, "circle", (5, 5, 5), (1, 55, 1, 60), 2, 2),    # past the lattice in y: out_of_range
+]
+
+
+def geometry_objects():
+    geo = {}
+    for k, (nx, ny, kind, typ, props, bounds, u, v) in enumerate(OBJECT_CASES):
+        rc, mm = po.ref_geometry_object(nx, ny, kind, props, typ, bounds, u, v)
+        geo["rc_%d" % k] = np.int64(rc)
+        geo["mask_%d" % k] = mm.astype(np.int8)
+        geo["spec_%d" % k] = np.array([nx, ny, kind, u, v] + list(bounds) + list(props), dtype=np.int64)
+        geo["type_%d" % k] = np.array(typ)
+    np.savez_compressed(os.path.join(OUT, "geometry_objects.npz"), **geo)
+    print("wrote geometry_objects", {k: int(geo["rc_%d" % k]) for k in range(len(OBJECT_CASES))})
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["objects"]:
+        assert po.ref_available(), "build the reference first: make -C oracle ref"
+        geometry_objects()
+    else:
+        main()
+        geometry_objects()

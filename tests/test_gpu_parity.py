@@ -408,10 +408,20 @@ def test_checkpoint_restore_continues_bit_for_bit(prec, name):
     other = nat.Field(g2, 1.0, nat.FP32_DEV if prec != nat.FP32_DEV else nat.FP64, 1.0)
     with pytest.raises(nat.SclError):
         other.restore(ck[0])
-    for f in fl + fl2 + [other]:
+    # ... and a field of the same size and precision on another mask (its
+    # solid sites would hold the wrong bounce-back slots)
+    mask3 = case["mask"].copy()
+    fy, fx = np.argwhere(mask3 == 1)[len(np.argwhere(mask3 == 1)) // 2]
+    mask3[fy, fx] = 0
+    g3 = nat.Geometry(mask3)
+    on_other_mask = nat.Field(g3, case["tau"][0], prec, 1.0)
+    with pytest.raises(nat.SclError, match="another geometry"):
+        on_other_mask.restore(ck[0])
+    for f in fl + fl2 + [other, on_other_mask]:
         f.close()
     g.close()
     g2.close()
+    g3.close()
 
 
 @pytest.mark.parametrize("prec", [nat.FP32_DEV, nat.FP16_DEV])
