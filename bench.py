@@ -42,7 +42,7 @@ def parse():
     p.add_argument("--warmup", type=int, default=50)
     p.add_argument("--n", type=int, default=4096, help="lattice side")
     p.add_argument("--forms", default="fp32dev,fp64,fp16dev")
-    p.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU baseline sample")
+    p.add_argument("--cpu-steps", type=int, default=3, help="timed steps of the CPU baseline (about 2.7 s each)")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--tile", default="", help="cols,rows[,lookahead] override (default: autotuned)")
     p.add_argument("--fusion", default="", help="steps per launch[,band rows] override (default: autotuned)")
@@ -63,22 +63,28 @@ def log(msg):
 def time_form(nat, wl, case, form, steps, warmup, tile, fusion=""):
     g = nat.Geometry(case["mask"])
     prec = nat.PRECISION[form]
-    if tile:
-        v = [int(t) for t in tile.split(",")]
-        g.set_tile(v[0], v[1])
-        if len(v) > 2:
-            g.set_lookahead(v[2])
-    else:
-        g.autotune(2, prec)          # launch shape chosen by measurement, before the timed region
-    if fusion:
-        v = [int(t) for t in fusion.split(",")]
-        g.set_step_fusion(v[0], v[1] if len(v) > 1 else 0)
     fl = []
     for c in range(2):
         f = nat.Field(g, case["tau"][c], prec, 1.0, g_solid=0.0, g_fluid=case["g_fluid"][c])
         f.set_f(case["f"][c])
         fl.append(f)
     nat.solid_surface_force(g, fl[0], fl[1])
+    # The launch shape is chosen by measurement (on scratch fields) after the
+    # uploads, right before the warmup: the ~0.3 s of GPU work it takes keeps
+    # the clocks up.  With the autotuner first, the GPU idled for the ~0.7 s
+    # the host needs to convert and upload the 4096^2 populations, and the
+    # first ~100 ms of launches after that ran at lower clocks (k_stepk 0.75
+    # ms against 0.585 ms per launch, whatever the data).
+    if tile:
+        v = [int(t) for t in tile.split(",")]
+        g.set_tile(v[0], v[1])
+        if len(v) > 2:
+            g.set_lookahead(v[2])
+    else:
+        g.autotune(2, prec)
+    if fusion:
+        v = [int(t) for t in fusion.split(",")]
+        g.set_step_fusion(v[0], v[1] if len(v) > 1 else 0)
     nat.step2(g, fl[0], fl[1], None, None, warmup)
     nat.synchronize()
     nat.timing(1)                       # record every launch of the timed region
@@ -97,15 +103,16 @@ def time_form(nat, wl, case, form, steps, warmup, tile, fusion=""):
     n_sites = case["mask"].size
     mlups = n_sites * steps / dt / 1e6
     launch_ms = avg_ms * steps / cnt if cnt else 0.0
-    # achieved: the algorithmic bytes of the lattice updates the kernel
-    # performs (BYTES_PER_LU per site and step) over its time, i.e. kernel
-    # MLUPS x bytes per update; launch_pass_gbs: one read and one write of
-    # every population, what a launch moves whatever the steps it fuses (one:
-    # k_step, two: k_step2x, three: k_stepk), over the launch time
+    # launch_pass_gbs: the algorithmic bytes of one launch, one read and one
+    # write of every population, whatever the steps it fuses (one: k_step,
+    # two: k_step2x, three: k_stepk), over the launch time; effective_gbs:
+    # the lattice updates the kernel performs x BYTES_PER_LU over its time
+    # (kernel MLUPS x bytes per update), what one step per launch would move
     return dict(form=form, seconds=dt, ms_per_step=dt * 1e3 / steps, mlups=mlups,
+                bytes_per_launch=n_sites * BYTES_PER_LU[form],
                 kernel={1: "k_step", 2: "k_step2x", 3: "k_stepk"}[spl], steps_per_launch=spl, band_rows=band,
                 kernel_avg_ms=launch_ms, kernel_step_ms=avg_ms, kernel_launches=cnt, tile=list(tile_used),
-                achieved_gbs=n_sites * BYTES_PER_LU[form] / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else None,
+                effective_gbs=n_sites * BYTES_PER_LU[form] / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else None,
                 launch_pass_gbs=n_sites * BYTES_PER_LU[form] / (launch_ms * 1e-3) / 1e9 if launch_ms > 0 else None,
                 effective_pct_hbm=100.0 * mlups * 1e6 * BYTES_PER_LU[form] / (HBM_PEAK_GBS * 1e9), mass=mass)
 
@@ -176,7 +183,11 @@ def served_stream(nat, a):
                         porosity=round(jobs[k]["porosity"], 4), error=sts[k].get("error", ""),
                         steps_done=sts[k].get("steps_done"))
                    for k, st in enumerate(sts) if st["state"] == "failed"]
-    return dict(jobs=len(jobs), done=len(done), failed=sum(s["state"] == "failed" for s in sts), failed_jobs=failed_jobs,
+    scope = ("the served stream as specified (steps U[2000, 50000])" if a.serve_step_scale == 1.0 else
+             "a shortened served stream: the specified steps U[2000, 50000] x %g; not the specified "
+             "configuration, whose full run is recorded in profiles/served_stream_full.md" % a.serve_step_scale)
+    return dict(scope=scope, jobs=len(jobs), done=len(done), failed=sum(s["state"] == "failed" for s in sts),
+                failed_jobs=failed_jobs,
                 step_scale=a.serve_step_scale, steps_range=[int(round(2000 * a.serve_step_scale)),
                                                             int(round(50000 * a.serve_step_scale))],
                 offered_load=a.serve_load, capacity_mlups=cap, arrival_rate_per_s=rate,
@@ -200,39 +211,20 @@ def served_stream(nat, a):
                 sample_worst=worst[:5])
 
 
-def cpu_baseline(seconds):
-    """Serial CPU time of the same two-component step on a bounded sample:
-    the compiled reference when it was built, else the oracle restatement."""
-    from oracle import pyoracle as po
-    from sc_lbm_amd import workloads as wl
-    kind = "reference" if po.ref_available() else "port"
-    n = 256
-    case = wl.spinodal(n, G=1.2, seed=5)
-    s = po.System(case["mask"], case["tau"], case["g_fluid"], case["g_solid"],
-                  kind="ref" if kind == "reference" else "oracle")
-    for c in range(2):
-        s.set_f(c, case["f"][c])
-    s.solid_surface_force()
-    s.run(2)
-    steps, t = 0, 0.0
-    chunk = 4
-    while t < seconds:
-        t0 = time.perf_counter()
-        s.run(chunk)
-        t += time.perf_counter() - t0
-        steps += chunk
-    mlups = n * n * steps / t / 1e6
-    cpu = "unknown"
-    try:
-        for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                cpu = line.split(":", 1)[1].strip()
-                break
-    except OSError:
-        pass
-    return dict(value=mlups, unit="MLUPS", cores=1, kind=kind,
-                sample="two-component Shan-Chen step, %dx%d periodic, %d steps in %.1f s, 1 thread of %d (%s)"
-                       % (n, n, steps, t, os.cpu_count() or 0, cpu))
+def cpu_baseline(n, steps):
+    """Serial CPU time of the same two-component step on the same 4096^2
+    lattice: the oracle restatement (oracle/sc_oracle.cpp, the reference's
+    operation order), one thread pinned to one core in a child process, one
+    untimed step and `steps` timed ones.  The compiled reference's own step
+    runs at 0.44x the oracle's on one core at 4096^2 (measured in the build
+    container: 2.79 against 6.34 MLUPS)."""
+    from oracle import cpu_baseline as cb
+    r = cb.run_child(n=n, steps=steps, kind="oracle")
+    return dict(value=r["mlups"], unit="MLUPS", cores=1, kind="port",
+                sample="two-component Shan-Chen step, %dx%d periodic spinodal (the GPU workload), %d timed steps "
+                       "in %.1f s (%s s each) after one untimed step; 1 thread pinned to core %s of %d (%s)"
+                       % (n, n, steps, r["seconds"], ", ".join("%.2f" % t for t in r["per_step_s"]),
+                          r["core"][0], os.cpu_count() or 0, cb.cpu_model()))
 
 
 def main():
@@ -249,17 +241,25 @@ def main():
     for form in a.forms.split(","):
         log("timing %s: %d warmup + %d steps" % (form, a.warmup, a.steps))
         res[form] = time_form(nat, wl, case, form, a.steps, a.warmup, a.tile, a.fusion)
-        log("  %s: %.1f MLUPS, %s %.4f ms per launch (%d steps), %.0f GB/s algorithmic (%.0f GB/s per launch pass)"
+        log("  %s: %.1f MLUPS, %s %.4f ms per launch (%d steps), %.0f GB/s per launch pass (%.0f GB/s per step)"
             % (form, res[form]["mlups"], res[form]["kernel"], res[form]["kernel_avg_ms"],
-               res[form]["steps_per_launch"], res[form]["achieved_gbs"] or 0.0, res[form]["launch_pass_gbs"] or 0.0))
+               res[form]["steps_per_launch"], res[form]["launch_pass_gbs"] or 0.0, res[form]["effective_gbs"] or 0.0))
     head = res.get("fp32dev") or next(iter(res.values()))
-    traffic = valu_busy = None
+    traffic = valu_busy = tsrc = wg_per_cu = None
     tpath = os.path.join(ROOT, "profiles", "traffic_%s_%s_%d.json" % (head["kernel"], head["form"], a.n))
     if os.path.exists(tpath):
         tj = json.load(open(tpath))
         traffic = tj.get("bytes_per_launch")
-        valu_busy = tj.get("valu_busy_pct")   # the multi-step kernels are issue-bound, not HBM-bound
-    achieved = head["achieved_gbs"]
+        valu_busy = tj.get("valu_busy_pct")
+        wg_per_cu = tj.get("workgroups_per_cu")
+        tsrc = os.path.relpath(tpath, ROOT)
+    achieved = head["launch_pass_gbs"]
+    if valu_busy is not None and achieved and achieved < 0.7 * HBM_PEAK_GBS:
+        limiter = ("instruction issue and latency, not HBM bandwidth: VALU busy %.1f%%%s, HBM at %.0f%% of peak"
+                   % (valu_busy, ", %d workgroups per CU" % wg_per_cu if wg_per_cu else "",
+                      100.0 * achieved / HBM_PEAK_GBS))
+    else:
+        limiter = "HBM bandwidth"
     out = {
         "metric": "MLUPS + %HBM peak (FP32-dev vs FP64), 2-comp SC; p50/p99 served-job latency",
         "value": head["mlups"],
@@ -273,24 +273,26 @@ def main():
         "vs_baseline": None,
         "dtype": "f32" if head["form"] == "fp32dev" else ("f64" if head["form"] == "fp64" else "f16-store/f32"),
         "data": "synthetic",
-        "config": {"workload": "two-component Shan-Chen %dx%d periodic, tau 1, G 1.2, %s fused step"
-                               % (a.n, a.n, head["form"]),
+        "config": {"workload": "two-component Shan-Chen %dx%d periodic spinodal (rho 1 +- 1%%, tau 1, G 1.2), "
+                               "%s fused step" % (a.n, a.n, head["form"]),
                    "lattice": [a.n, a.n], "components": 2, "form": head["form"],
                    "tile": head["tile"], "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
-                     "bytes_per_update": BYTES_PER_LU[head["form"]],
-                     "launch_pass_gbs": head["launch_pass_gbs"],
-                     "hbm_moved_gbs": traffic / (head["kernel_avg_ms"] * 1e-3) / 1e9 if traffic else None,
-                     "note": "achieved = the kernel's lattice updates x %d B (read and write every population "
-                             "once per step) over its time; a launch fusing %d steps performs %d updates per "
-                             "site but moves the populations once (launch_pass_gbs, hbm_moved_gbs = measured "
-                             "traffic per launch over the launch time)"
-                             % (BYTES_PER_LU[head["form"]], head["steps_per_launch"], head["steps_per_launch"]),
+                     "bytes_per_launch": head["bytes_per_launch"],
                      "kernel": head["kernel"], "steps_per_launch": head["steps_per_launch"],
-                     "valu_busy_pct": valu_busy},
+                     "limiter": limiter, "valu_busy_pct": valu_busy, "traffic_source": tsrc,
+                     "per_step_effective_gbs": head["effective_gbs"],
+                     "per_step_effective_frac": head["effective_gbs"] / HBM_PEAK_GBS if head["effective_gbs"] else None,
+                     "note": "achieved: the algorithmic bytes of one launch of the kernel (every population of both "
+                             "components read once and written once, %d B per site, whatever the %d steps it "
+                             "fuses) over its HIP-event time; traffic: FETCH_SIZE + WRITE_SIZE per launch from the "
+                             "profile in traffic_source, calibrated on a copy kernel; per_step_effective: the "
+                             "kernel's lattice updates x %d B over its time, the bandwidth a one-step-per-launch "
+                             "kernel would need for the same MLUPS"
+                             % (BYTES_PER_LU[head["form"]], head["steps_per_launch"], BYTES_PER_LU[head["form"]])},
         "forms": {k: {kk: v[kk] for kk in ("mlups", "effective_pct_hbm", "kernel", "steps_per_launch",
-                                           "band_rows", "kernel_avg_ms", "achieved_gbs", "launch_pass_gbs",
+                                           "band_rows", "kernel_avg_ms", "launch_pass_gbs", "effective_gbs",
                                            "tile")}
                   for k, v in res.items()},
     }
@@ -303,8 +305,8 @@ def main():
         log("  p50 %.1f ms, p99 %.1f ms, aggregate %.0f MLUPS, %d failed, sample max |drho| %s"
             % (sv["p50_ms"], sv["p99_ms"], sv["aggregate_mlups"], sv["failed"], sv["sample_max_abs_rho_err"]))
     if not a.no_cpu:
-        log("timing the CPU baseline (about %.0f s)" % a.cpu_seconds)
-        out["cpu_baseline"] = cpu_baseline(a.cpu_seconds)
+        log("timing the CPU baseline (%d steps at %d^2, one core)" % (a.cpu_steps, a.n))
+        out["cpu_baseline"] = cpu_baseline(a.n, a.cpu_steps)
     print(json.dumps(out))
 
 

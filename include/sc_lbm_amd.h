@@ -41,6 +41,12 @@ extern "C" {
 #define SCL_ERR_UNSUPPORTED 4/* operation not available for this precision       */
 #define SCL_ERR_BUSY 5       /* server queue full: back-pressure, retry later     */
 #define SCL_ERR_TIMEOUT 6    /* a wait ended before the job or server finished    */
+#define SCL_ERR_INEXACT 7    /* SCL_FP64: a fused step reused an equilibrium velocity
+                                older than the previous step's at a zero-density site (the
+                                reference's rule, src/lbm.cpp:262-269, needs the previous
+                                one): reported once, by the next download, moments or
+                                checkpoint of the field; the data are still delivered.  A
+                                served FP64 job fails with this reason. */
 
 /* Population storage forms. */
 #define SCL_FP64 0           /* plain FP64, bit-exact with the reference          */
@@ -325,8 +331,10 @@ typedef struct scl_server_config {
     int max_jobs;               /* resident job slots                                  */
     int max_pending;            /* queued requests before submit returns SCL_ERR_BUSY */
     int round_steps;            /* steps per round (graph length), 1..1024            */
-    int tile_cols, tile_rows, lookahead;    /* persistent-kernel tile; tile_cols 0 =
-                                   width and lookahead timed at start-up; rows 0 = 16 */
+    int tile_cols, tile_rows, lookahead;    /* persistent-kernel tile: columns 128 or 256,
+                                   rows 1..4096 (0 = 16), lookahead 1 or 2; tile_cols 0 = the
+                                   width is timed at start-up, and the lookahead too when 0
+                                   (nonzero rows and lookahead are kept) */
     double mass_tol;            /* watchdog: |mass - mass0| > mass_tol*|mass0| fails a job;
                                    0 = by precision (1e-6 FP64, 1e-3 FP32, 0.1 FP16) */
     int max_bypass;             /* later jobs admitted past a waiting one before it blocks admission */

@@ -32,6 +32,12 @@ __device__ inline unsigned nbits_at(const Grid2& g, int x, int y) {
 // Stream with link bounce-back as a pull (src/lbm.cpp:343-435 seen from the
 // receiving site); solid sites become 0.  Works for any storage form (the
 // deviation forms store 0 at solid sites, which are never read).
+// One 32-bit word, written from the device (ordered on the stream and
+// graph-safe, unlike a host memset).
+static __global__ void k_set_u32(unsigned* p, unsigned v) {
+    if (threadIdx.x == 0) *G(p) = v;
+}
+
 template <class ST>
 __global__ void k_gather(Grid2 g, const typename ST::S* __restrict__ src, typename ST::S* __restrict__ dst) {
     int x, y, s;

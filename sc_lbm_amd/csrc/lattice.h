@@ -34,9 +34,17 @@ static_assert(opp(5) == 7 && opp(6) == 8 && opp(7) == 5 && opp(8) == 6, "opp dia
 // Density below which the reference keeps the previous equilibrium velocity
 // (include/utils.h:145-150 with tol 1e-16, src/lbm.cpp:262-269).
 constexpr double kZeroTol = 1e-16;
-// Fused kernels record u_eq in HBM only where a component's density is below
-// this value (the only sites where the next step can fall back to it).
-constexpr double kUeqWatch = 1e-8;
+// Fused kernels hold u_eq in HBM only where a component's density is below
+// a threshold (the only sites where the next step can fall back to it):
+// 1e-8 in FP64.  The deviation forms hold it below a threshold above the
+// noise their storage leaves in the density of an empty component
+// (StoreF32 / StoreF16 ::kUeqWatch: 1e-5 and 2e-3, against about 1e-7 and
+// 5e-4 at rho0 = 1), where the density can fall to 0 and back in one step.
+// Held sites cost a store and an atomic per step, so the thresholds stay
+// below the densities of the dilute phases of the configurations (a 5e-2
+// threshold halved the FP16 throughput of the 4096^2 spinodal lattice).
+constexpr double kUeqWatch64 = 1e-8;
+constexpr double kUeqWatch = kUeqWatch64;
 
 // FP16 deviation scale: stored value = (f_i - w_i*rho0) * 2^8.
 constexpr float kHalfScale = 256.0f;
@@ -96,33 +104,39 @@ struct StoreF64 {
     using S = double;
     using T = double;
     static constexpr bool kDev = false;
+    static constexpr T kUeqWatch = kUeqWatch64;
     __device__ static inline T ld(const S* p) { return *G(p); }
     __device__ static inline void st(S* p, T v) { stream_store<S>(p, v); }
     __device__ static inline void stc(S* p, T v) { *G(p) = v; }
     __device__ static inline S enc(T v) { return v; }   // value as stored
     __device__ static inline T dec(S s) { return s; }   // stored value as loaded
+    __device__ static inline T rnd(T v) { return v; }   // v rounded to what the store keeps
 };
 
 struct StoreF32 {
     using S = float;
     using T = float;
     static constexpr bool kDev = true;
+    static constexpr T kUeqWatch = 1e-5f;
     __device__ static inline T ld(const S* p) { return *G(p); }
     __device__ static inline void st(S* p, T v) { stream_store<S>(p, v); }
     __device__ static inline void stc(S* p, T v) { *G(p) = v; }
     __device__ static inline S enc(T v) { return v; }
     __device__ static inline T dec(S s) { return s; }
+    __device__ static inline T rnd(T v) { return v; }
 };
 
 struct StoreF16 {
     using S = _Float16;
     using T = float;
     static constexpr bool kDev = true;
+    static constexpr T kUeqWatch = 2e-3f;
     __device__ static inline T ld(const S* p) { return (float)(*G(p)) * kHalfInv; }
     __device__ static inline void st(S* p, T v) { stream_store<S>(p, (_Float16)(v * kHalfScale)); }
     __device__ static inline void stc(S* p, T v) { *G(p) = (_Float16)(v * kHalfScale); }
     __device__ static inline S enc(T v) { return (_Float16)(v * kHalfScale); }
     __device__ static inline T dec(S s) { return (float)s * kHalfInv; }
+    __device__ static inline T rnd(T v) { return dec(enc(v)); }
 };
 
 // Wrap a column / row index that is at most one step outside [0, n).

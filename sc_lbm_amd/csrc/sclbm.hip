@@ -165,6 +165,12 @@ struct scl_field {
     double* ueq_block = nullptr;
     double* ueqs[kUeqRing][2] = {};
     int ucur = 0;
+    // FP64: zero-density rule records (CompArgs::stamp), n_sites + 2 entries,
+    // the last two the error flag and the step as of which every held
+    // equilibrium velocity is valid; nstep = time steps completed (fused or
+    // streamed per operator).
+    unsigned* stamp = nullptr;
+    unsigned nstep = 0;
     // reference per-site arrays (FP64, pitched)
     double *rho = nullptr, *ux = nullptr, *uy = nullptr, *ueqx = nullptr, *ueqy = nullptr;
     double *frx = nullptr, *fry = nullptr, *fsx = nullptr, *fsy = nullptr, *feq = nullptr;
@@ -195,6 +201,7 @@ void set_pgeom(scl_field* f, scl_geometry* g) {
 
 // The deviation forms' equilibrium-velocity buffers after `steps` steps.
 void advance_ueq(scl_field* f, int steps) {
+    f->nstep += (unsigned)steps;
     if (f->precision == SCL_FP64) return;
     f->ucur = (f->ucur + steps) % kUeqRing;
     f->ueqx = f->ueqs[f->ucur][0];
@@ -203,6 +210,33 @@ void advance_ueq(scl_field* f, int steps) {
 
 size_t n_sites(const scl_geometry* g) { return (size_t)g->pitch * g->ny; }
 size_t n_pop(const scl_geometry* g) { return (size_t)g->plane * Q; }
+
+// Declares every held equilibrium velocity of f valid as of step v (after an
+// upload, a restore or the per-operator velocities, which set them all).
+int set_ueq_valid(scl_field* f, unsigned v) {
+    if (!f->stamp) return SCL_OK;   // FP64 fields keep the records
+    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, g_lib.stream, f->stamp + n_sites(f->g) + 1, v);
+    HIPCHK(hipGetLastError());
+    return SCL_OK;
+}
+
+// The zero-density rule's flag (ueq_reuse): SCL_ERR_INEXACT, once, if a
+// fused step reused an equilibrium velocity older than the previous step's.
+int check_ueq_flag(scl_field* f) {
+    if (!f->stamp) return SCL_OK;
+    unsigned flag = 0;
+    HIPCHK(hipMemcpyAsync(&flag, f->stamp + n_sites(f->g), sizeof flag, hipMemcpyDeviceToHost, g_lib.stream));
+    HIPCHK(hipStreamSynchronize(g_lib.stream));
+    if (!flag) return SCL_OK;
+    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, g_lib.stream, f->stamp + n_sites(f->g), 0u);
+    HIPCHK(hipGetLastError());
+    const double watch = kUeqWatch;
+    return fail(SCL_ERR_INEXACT,
+                "zero-density rule: a site's density fell from >= %g to <= 1e-16 within one step, and the fused "
+                "kernel had not kept its previous equilibrium velocity (results may differ from the reference's "
+                "at that site; the per-operator entry points keep every site's)",
+                watch);
+}
 
 int ensure_aux(scl_field* f) {
     if (f->rho) return SCL_OK;
@@ -315,8 +349,9 @@ void fill_comp(CompArgs<T>& c, const scl_field* f, int src_buf, const double* fo
     c.gneg = (T)(-1.0 * f->g_fluid);
     c.gsneg = (T)f->gsneg_captured;
     c.rho0 = (T)f->rho0;
-    c.has_force = force ? 1 : 0;
-    for (int i = 0; i < Q; ++i) c.force[i] = force ? (T)force[i] : T(0);
+    set_force(c, force, f->omega);
+    c.step = f->nstep + 1;
+    c.stamp = f->stamp;
     if (f->precision == SCL_FP64) {
         c.ueqx = f->ueqx;
         c.ueqy = f->ueqy;
@@ -960,6 +995,10 @@ int scl_field_create(const scl_geometry* g, int precision, double tau, double rh
         f->ueqx = f->ueqs[0][0];
         f->ueqy = f->ueqs[0][1];
     }
+    if (precision == SCL_FP64) {
+        CHK(dalloc(&f->stamp, n + 2));
+        HIPCHK(hipMemsetAsync(f->stamp, 0, (n + 2) * sizeof(unsigned), g_lib.stream));
+    }
     f->g->refs++;
     *out = f.release();
     return SCL_OK;
@@ -986,6 +1025,7 @@ int scl_field_destroy(scl_field* f) {
         dfree(f->ueq_block);
     }
     for (double* p : {f->rho, f->ux, f->uy, f->frx, f->fry, f->fsx, f->fsy, f->feq, f->stage}) dfree(p);
+    dfree(f->stamp);
     scl_geometry* g = f->g;
     scl_geometry* pg = f->pg;
     delete f;
@@ -1052,6 +1092,7 @@ int scl_field_upload(scl_field* f, int which, const double* src, size_t n) {
             HIPCHK(hipGetLastError());
         }
         f->phase = PH_S;
+        CHK(set_ueq_valid(f, f->nstep));
         HIPCHK(hipStreamSynchronize(g_lib.stream));
         return SCL_OK;
     }
@@ -1064,6 +1105,7 @@ int scl_field_upload(scl_field* f, int which, const double* src, size_t n) {
     }
     CHK(materialize(f));
     CHK(copy_planes(g, aux_ptr(f, which), src, which == SCL_FEQ ? 9 : 1, true));
+    if (which == SCL_UEQX || which == SCL_UEQY) CHK(set_ueq_valid(f, f->nstep));
     if (which == SCL_FSX || which == SCL_FSY) {
         // the reference's first F_solid entries are in use from now on
         f->has_solid_force = true;
@@ -1112,7 +1154,7 @@ int scl_field_download(scl_field* f, int which, double* dst, size_t n) {
             CHK(copy_planes(g, f->stage, dst, 9, false));
         }
         HIPCHK(hipStreamSynchronize(g_lib.stream));
-        return SCL_OK;
+        return check_ueq_flag(f);
     }
     if (which < SCL_FEQ || which > SCL_FSY) return fail(SCL_ERR_ARG, "bad field selector %d", which);
     const size_t want = which == SCL_FEQ ? 9 * ns : ns;
@@ -1124,7 +1166,7 @@ int scl_field_download(scl_field* f, int which, double* dst, size_t n) {
     CHK(materialize(f));
     CHK(copy_planes(g, aux_ptr(f, which), dst, which == SCL_FEQ ? 9 : 1, false));
     HIPCHK(hipStreamSynchronize(g_lib.stream));
-    return SCL_OK;
+    return check_ueq_flag(f);
 }
 
 // ---- exact checkpoint -------------------------------------------------------
@@ -1178,7 +1220,7 @@ int scl_field_checkpoint(scl_field* f, void* dst, size_t bytes) {
     HIPCHK(hipMemcpyAsync(out, f->ueqx, half, hipMemcpyDeviceToHost, g_lib.stream));
     HIPCHK(hipMemcpyAsync(out + half, f->ueqy, half, hipMemcpyDeviceToHost, g_lib.stream));
     HIPCHK(hipStreamSynchronize(g_lib.stream));
-    return SCL_OK;
+    return check_ueq_flag(f);
 }
 
 int scl_field_restore(scl_field* f, const void* src, size_t bytes) {
@@ -1208,6 +1250,7 @@ int scl_field_restore(scl_field* f, const void* src, size_t bytes) {
     HIPCHK(hipMemcpyAsync(f->ueqy, in + half, half, hipMemcpyHostToDevice, g_lib.stream));
     f->phase = h.phase;
     if (f->phase == PH_P) set_pgeom(f, f->g);
+    CHK(set_ueq_valid(f, f->nstep));
     HIPCHK(hipStreamSynchronize(g_lib.stream));
     return SCL_OK;
 }
@@ -1288,7 +1331,9 @@ int scl_equilibrium_velocities(const scl_geometry* g, scl_field* a, scl_field* b
                 b->ux, b->uy, b->ueqx, b->ueqy};
     hipLaunchKernelGGL(k_eq_velocities, site_grid(g), dim3(256), 0, g_lib.stream, g->grid(), A, B);
     HIPCHK(hipGetLastError());
-    return SCL_OK;
+    // every site's value is this step's now (dry sites keep theirs, as the reference)
+    CHK(set_ueq_valid(a, a->nstep + 1));
+    return set_ueq_valid(b, b->nstep + 1);
 }
 
 int scl_collide1(const scl_geometry* g, scl_field* f) {
@@ -1330,6 +1375,7 @@ static int stream_one(const scl_geometry* g, scl_field* f) {
                        (const double*)f->f[f->cur], (double*)f->f[f->cur ^ 1]);
     HIPCHK(hipGetLastError());
     f->cur ^= 1;
+    f->nstep++;
     return SCL_OK;
 }
 
@@ -1427,7 +1473,7 @@ int scl_moments(const scl_geometry* g, scl_field* f, double out[3]) {
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(out, g->part + 3 * (size_t)g->ny, 3 * sizeof(double), hipMemcpyDeviceToHost, g_lib.stream));
     HIPCHK(hipStreamSynchronize(g_lib.stream));
-    return SCL_OK;
+    return check_ueq_flag(f);
 }
 
 // Laplace law, as laplace_law/compute_surface_tension.py:63-70 evaluates it

@@ -61,6 +61,7 @@ struct ServeHdr {
     int n_step;                 // k_serve takes items [0, n_step): the jobs it has steps of
     const uint64_t* items2;     // two-step tiles (strips of x2_out columns)
     int n_items2;
+    unsigned* flags;            // per job: the zero-density rule's flags of both components
 };
 
 // Tile encoding, 16 bits each from the low end: job, strip, y0, y1.
@@ -102,6 +103,8 @@ k_serve(const ServeHdr* __restrict__ hdr, int s, unsigned* __restrict__ ctr) {
         if (s >= J.steps) continue;
         if (J.x2 && s != J.steps - 1) continue;   // the pairs are k_serve2x's
         StepArgs<T> a = J.a;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) a.c[c].step = J.a.c[c].step + (unsigned)s;
         if constexpr (ST::kDev) {
             const int ki = (J.ucur + s) % 3, ko = (J.ucur + s + 1) % 3;
 #pragma unroll
@@ -256,6 +259,8 @@ __global__ __launch_bounds__(256) void k_serve_mass_final(const ServeHdr* __rest
     }
     block_sum2(m0, m1, red);
     if (threadIdx.x == 0) {
+        const long long pl = J.a.plane;   // FP64 jobs keep the zero-density rule's records
+        hdr->flags[j] = J.a.c[0].stamp ? G(J.a.c[0].stamp)[pl] | G(J.a.c[1].stamp)[pl] : 0u;
         hdr->mass[2 * j] = m0;
         hdr->mass[2 * j + 1] = m1;
     }

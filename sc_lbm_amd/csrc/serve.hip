@@ -58,7 +58,7 @@ size_t align256(size_t n) { return (n + 255) / 256 * 256; }
 struct Layout {
     int pitch = 0, wpr = 0;
     long long plane = 0;
-    size_t mask = 0, f[2][2] = {}, ueq[3][2][2] = {}, rho[2] = {}, stage = 0, total = 0;
+    size_t mask = 0, f[2][2] = {}, ueq[3][2][2] = {}, rho[2] = {}, stamp[2] = {}, stage = 0, total = 0;
     int nueq = 1;   // equilibrium-velocity buffer sets: 1 (FP64) or 3 (deviation forms, rotating)
     bool has_stage = false;
 };
@@ -82,6 +82,10 @@ Layout layout_of(const JobSpec& s, size_t es) {
         for (int c = 0; c < 2; ++c)
             for (int d = 0; d < 2; ++d) L.ueq[k][c][d] = take((size_t)L.plane * 8);
     for (int c = 0; c < 2; ++c) L.rho[c] = take((size_t)L.plane * 8);
+    // FP64: the zero-density rule's records (CompArgs::stamp), a word per
+    // site, the flag and the step as of which every held value is valid
+    if (es == 8)
+        for (int c = 0; c < 2; ++c) L.stamp[c] = take((size_t)(L.plane + 2) * 4);
     L.has_stage = !s.f[0].empty() || (s.deliver & SCL_DELIVER_F);
     if (L.has_stage) L.stage = take((size_t)Q * L.plane * 8);
     L.total = off;
@@ -100,6 +104,8 @@ struct GJob {
     double* ueq(int k, int c, int d) const { return (double*)(base + L.ueq[k][c][d]); }
     int ucur = 0;   // deviation forms: buffer set holding the latest equilibrium velocities
     double* rho(int c) const { return (double*)(base + L.rho[c]); }
+    unsigned* stamp(int c) const { return (unsigned*)(base + L.stamp[c]); }
+    unsigned nstep = 0;   // time steps completed (numbers the next launch's steps)
     double* stage() const { return (double*)(base + L.stage); }
     int nx = 0, ny = 0;
 };
@@ -118,9 +124,9 @@ class GpuServe final : public serve::Backend {
         if (stream_) (void)hipStreamSynchronize(stream_);
         for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
         for (void* p : {(void*)arena_, (void*)hdr_d_, (void*)ctr_d_, table_d_, (void*)items_d_, (void*)partial_d_,
-                        (void*)mass_d_, (void*)items2_d_})
+                        (void*)mass_d_, (void*)items2_d_, (void*)flags_d_})
             if (p) (void)hipFree(p);
-        for (void* p : {(void*)hdr_h_, table_h_, (void*)items_h_, (void*)mass_h_, (void*)items2_h_})
+        for (void* p : {(void*)hdr_h_, table_h_, (void*)items_h_, (void*)mass_h_, (void*)items2_h_, (void*)flags_h_})
             if (p) (void)hipHostFree(p);
         for (hipEvent_t e : ev_)
             if (e) (void)hipEventDestroy(e);
@@ -149,9 +155,11 @@ class GpuServe final : public serve::Backend {
         SHIP(hipMemsetAsync(ctr_d_, 0, 2 * sizeof(unsigned), stream_));
         SHIP(hipMalloc(&table_d_, (size_t)cfg_.max_jobs * sizeof(JobArgs<T>)));
         SHIP(hipMalloc(&mass_d_, (size_t)cfg_.max_jobs * 2 * sizeof(double)));
+        SHIP(hipMalloc(&flags_d_, (size_t)cfg_.max_jobs * sizeof(unsigned)));
         SHIP(hipHostMalloc(&hdr_h_, sizeof(ServeHdr), hipHostMallocDefault));
         SHIP(hipHostMalloc(&table_h_, (size_t)cfg_.max_jobs * sizeof(JobArgs<T>), hipHostMallocDefault));
         SHIP(hipHostMalloc(&mass_h_, (size_t)cfg_.max_jobs * 2 * sizeof(double), hipHostMallocDefault));
+        SHIP(hipHostMalloc(&flags_h_, (size_t)cfg_.max_jobs * sizeof(unsigned), hipHostMallocDefault));
         int rc = grow_items(4096, err);
         if (rc != SCL_OK) return rc;
         mass_grid_ = 4 * cus_;
@@ -172,6 +180,15 @@ class GpuServe final : public serve::Backend {
                 return SCL_ERR_ARG;
             }
         } else {
+            // the width is timed; a caller's rows and lookahead are kept
+            if (cfg_.lookahead != 0 && cfg_.lookahead != 1 && cfg_.lookahead != 2) {
+                *err = "lookahead is 1 or 2";
+                return SCL_ERR_ARG;
+            }
+            if (cfg_.tile_rows < 0 || cfg_.tile_rows > 4096) {
+                *err = "tile rows must be in 1..4096";
+                return SCL_ERR_ARG;
+            }
             rc = autotune(err);
             if (rc != SCL_OK) return rc;
         }
@@ -257,7 +274,7 @@ class GpuServe final : public serve::Backend {
             }
         }
         *hdr_h_ = ServeHdr{table_d_, items_d_, partial_d_, mass_d_, n, (int)jobs.size(), 64 * nw_, n_step,
-                           items2_d_, n2};
+                           items2_d_, n2, flags_d_};
         SHIP(hipMemcpyAsync(table_d_, table_h_, jobs.size() * sizeof(JobArgs<T>), hipMemcpyHostToDevice, stream_));
         SHIP(hipMemcpyAsync(items_d_, items_h_, (size_t)n * sizeof(uint64_t), hipMemcpyHostToDevice, stream_));
         if (n2) SHIP(hipMemcpyAsync(items2_d_, items2_h_, (size_t)n2 * sizeof(uint64_t), hipMemcpyHostToDevice,
@@ -270,6 +287,7 @@ class GpuServe final : public serve::Backend {
         SHIP(hipGraphLaunch(ge, stream_));
         SHIP(hipEventRecord(ev_[1], stream_));
         SHIP(hipMemcpyAsync(mass_h_, mass_d_, jobs.size() * 2 * sizeof(double), hipMemcpyDeviceToHost, stream_));
+        SHIP(hipMemcpyAsync(flags_h_, flags_d_, jobs.size() * sizeof(unsigned), hipMemcpyDeviceToHost, stream_));
         SHIP(hipStreamSynchronize(stream_));
         float ms = 0;
         SHIP(hipEventElapsedTime(&ms, ev_[0], ev_[1]));
@@ -280,6 +298,10 @@ class GpuServe final : public serve::Backend {
             jobs[k]->mass[1] = mass_h_[2 * k + 1];
             g->cur ^= launches(steps[k], two[k]) & 1;
             g->ucur = (g->ucur + steps[k]) % 3;
+            g->nstep += (unsigned)steps[k];
+            if (flags_h_[k])
+                jobs[k]->device_error = "zero-density rule: a site's density fell from >= 1e-8 to <= 1e-16 "
+                                        "within one step without its previous equilibrium velocity held";
         }
         return SCL_OK;
     }
@@ -430,8 +452,9 @@ class GpuServe final : public serve::Backend {
             ca.gneg = (T)(-1.0 * s.g_fluid[c]);
             ca.gsneg = (T)(-1.0 * s.g_solid[c]);
             ca.rho0 = (T)(ST::kDev ? s.rho0[c] : 0.0);
-            ca.has_force = s.has_force[c] ? 1 : 0;
-            for (int i = 0; i < Q; ++i) ca.force[i] = (T)s.force[c][i];
+            set_force(ca, s.has_force[c] ? s.force[c] : nullptr, omega);
+            ca.step = g.nstep + 1;
+            ca.stamp = ST::kDev ? nullptr : g.stamp(c);
             ca.ueqx = g.ueq(0, c, 0);
             ca.ueqy = g.ueq(0, c, 1);
             ca.ueqx_in = ca.ueqx;
@@ -474,6 +497,14 @@ class GpuServe final : public serve::Backend {
             for (int c = 0; c < 2; ++c)
                 for (int d = 0; d < 2; ++d, in += ueq)
                     SHIP(hipMemcpyAsync(g->ueq(0, c, d), in, ueq, hipMemcpyHostToDevice, stream_));
+            // the steps continue their numbering; every held equilibrium
+            // velocity restored is valid as of the last step done
+            g->nstep = (unsigned)j.steps_done;
+            const unsigned valid = g->nstep;
+            if (!ST::kDev)
+                for (int c = 0; c < 2; ++c)
+                    SHIP(hipMemcpyAsync(g->stamp(c) + L.plane + 1, &valid, sizeof valid, hipMemcpyHostToDevice,
+                                        stream_));
             SHIP(hipStreamSynchronize(stream_));
             j.saved.clear();
             g->cur = 0;
@@ -648,8 +679,12 @@ class GpuServe final : public serve::Backend {
             rc = SCL_ERR_HIP;
         }
         float best = 1e30f;
-        int bnw = 4, brows = 32, bla = 1;
-        const int nws[] = {2, 4}, las[] = {1, 2}, rowss[] = {16};
+        // rows: the caller's, else 16; lookahead: the caller's, else timed
+        const int rows_fixed = cfg_.tile_rows ? cfg_.tile_rows : 16;
+        int bnw = 4, brows = rows_fixed, bla = cfg_.lookahead ? cfg_.lookahead : 1;
+        std::vector<int> las = {1, 2};
+        if (cfg_.lookahead) las = {cfg_.lookahead};
+        const int nws[] = {2, 4}, rowss[] = {rows_fixed};
         for (int nw : nws)
             for (int la : las)
                 for (int rows : rowss) {
@@ -678,7 +713,8 @@ class GpuServe final : public serve::Backend {
                         n = emit_tiles(k, g->nx, g->ny, nw, rows, items_h_, n);
                         table[k].item_end = n;
                     }
-                    *hdr_h_ = ServeHdr{table_d_, items_d_, partial_d_, mass_d_, n, 2, 64 * nw, n, items2_d_, 0};
+                    *hdr_h_ = ServeHdr{table_d_, items_d_, partial_d_, mass_d_, n, 2, 64 * nw, n, items2_d_, 0,
+                                       flags_d_};
                     if (hipMemcpyAsync(table_d_, table_h_, 2 * sizeof(JobArgs<T>), hipMemcpyHostToDevice,
                                        stream_) != hipSuccess ||
                         hipMemcpyAsync(items_d_, items_h_, (size_t)n * sizeof(uint64_t), hipMemcpyHostToDevice,
@@ -736,6 +772,8 @@ class GpuServe final : public serve::Backend {
     double* partial_d_ = nullptr;
     double* mass_d_ = nullptr;
     double* mass_h_ = nullptr;
+    unsigned* flags_d_ = nullptr;   // zero-density rule flag per job of the round
+    unsigned* flags_h_ = nullptr;
     int nw_ = 4, rows_ = 16, la_ = 1, grid_ = 0, mass_grid_ = 1024;
     bool x2_ = false;                  // two steps per launch for wide jobs (deviation forms)
     int x2rows_ = 32, grid2_ = 0;      // band rows of the two-step tiles; its persistent grid

@@ -43,6 +43,13 @@ struct CompArgs {
     T rho0;            // deviation reference density (deviation forms)
     T force[9];
     int has_force;
+    // FP64: index of the time step this launch computes, and the zero-density
+    // rule's records (ueq_hold / ueq_reuse): stamp[s] = the step whose
+    // equilibrium velocity the held value of site s is; stamp[plane] = error
+    // flag; stamp[plane + 1] = the step as of which every site's held value is
+    // valid (set by the host).  Unused by the deviation forms.
+    unsigned step;
+    unsigned* stamp;
     double* ueqx;      // equilibrium velocity store (dry-site fallback / replay output)
     double* ueqy;
     // Deviation forms: the equilibrium velocities of the previous step, read
@@ -57,6 +64,24 @@ struct CompArgs {
     double* fry;
     double* feq;       // 9 planes
 };
+
+// The body force as the kernels take it (host side): F_i for FP64; for the
+// deviation forms (relax_dev) F_i / omega for i > 0 and the sum of all F_i,
+// the mass it adds, in slot 0.
+template <class T>
+inline void set_force(CompArgs<T>& c, const double* force, double omega) {
+    c.has_force = force ? 1 : 0;
+    if (!force) {
+        for (int i = 0; i < Q; ++i) c.force[i] = T(0);
+    } else if (sizeof(T) == sizeof(double)) {
+        for (int i = 0; i < Q; ++i) c.force[i] = (T)force[i];
+    } else {
+        double sum = 0.0;
+        for (int i = 0; i < Q; ++i) sum += force[i];
+        c.force[0] = (T)sum;
+        for (int i = 1; i < Q; ++i) c.force[i] = (T)(force[i] / omega);
+    }
+}
 
 template <class T>
 struct StepArgs {
@@ -171,17 +196,34 @@ __device__ inline void feq_ref(double rho, double ux, double uy, double (&e)[Q])
 }
 
 // BGK relaxation of one component in deviation form, FP32:
-//   d_i' = d_i + omega (deq_i - d_i) [+ F_i],
+//   d_i' = d_i + omega (deq_i - d_i + F_i / omega),
 //   deq_i = feq_i - w_i rho0 = w_i drho + w_i rho (3 cu + 4.5 cu^2 - 1.5 u^2),
 // with drho = rho - rho0 so small perturbations keep their digits.  Opposite
 // directions share cu^2 (their polynomials differ only in the sign of the
 // 3 cu term).
 //
-// Two choices keep the FP32 result close to FP64 over long runs:
-//  * the update acts on the non-equilibrium part deq_i - d_i, which is small,
-//    rather than adding (1 - omega) d_i and omega deq_i, which nearly cancel
-//    for omega near 2 (a porous two-phase lattice at tau = 0.707 drifted 4x
-//    further from FP64 with the latter form);
+// Which roundings the FP32 result carries decides how close it stays to FP64
+// over long runs, because in a steady state a site's rounding repeats every
+// step.  Four choices:
+//  * the non-equilibrium part deq_i - d_i is formed directly:
+//    fma(w_i, drho, -d_i) rounds once at the size of that (small) difference,
+//    and the velocity terms are added to it, instead of rounding deq_i itself
+//    at the size of the populations.  Rounded at that size, deq_i of
+//    opposite directions carries independent errors, a frozen field of
+//    spurious momentum sources; an FP32 emulation of a porous two-phase
+//    lattice (512x256, 2000 steps, against FP64) ended 2.97e-5 from FP64 in
+//    density that way and 1.97e-7 this way, the level of the FP32 storage
+//    rounding alone;
+//  * the body force enters before the one rounding of d_i' (as F_i / omega
+//    inside the fma).  Added after it, the sum of a stored-grid value and the
+//    fixed F_i rounds with the same bias at every site whose d_i shares a
+//    binade, a systematic error in the force of up to half a unit in the last
+//    place per step (the same emulation: 3e-5 from FP64 with the force added
+//    after rounding, 3e-7 before);
+//  * the update acts on the non-equilibrium part, which is small, rather than
+//    adding (1 - omega) d_i and omega deq_i, which nearly cancel for omega
+//    near 2 (a porous two-phase lattice at tau = 0.707 drifted 4x further
+//    from FP64 with the latter form);
 //  * the rest population closes the site's mass balance:
 //    d_0' = d_0 + sum_{i>0} (d_i - d_i') + sum_i F_i, summing the changes,
 //    which are small, rather than the populations.  Collision conserves
@@ -192,6 +234,15 @@ __device__ inline void feq_ref(double rho, double ux, double uy, double (&e)[Q])
 //    steady state the rounding of those large partial sums repeats every
 //    step (a porous two-phase run lost 2e-5 of a component's mass in 1e4
 //    steps; summing the changes cut an FP32 emulation's drift 1000x).
+// force: the deviation forms' encoding of the body force (set_force):
+// force[i] = F_i / omega for i > 0 and force[0] = sum_i F_i.
+// ST: the storage form.  The mass balance sums the changes of the values as
+// they will be stored (rounded to FP16 for StoreF16), so only the rounding of
+// d_0' itself is left out of it: closed on the FP32 values, every FP16
+// rounding of d_1'..d_8' was a mass error, and where a population changes by
+// less than half its FP16 spacing the rounding has the same sign every step
+// (a 4096^2 spinodal lattice lost 0.45% of its mass in 1000 steps that way).
+template <class ST>
 __device__ inline void relax_dev(const float (&p)[Q], float rho, float drho, float ux, float uy, float omega,
                                  const float* force, bool has_force, float (&out)[Q]) {
     // Every fused multiply-add is written out (the file is built without
@@ -199,33 +250,29 @@ __device__ inline void relax_dev(const float (&p)[Q], float rho, float drho, flo
     // could differ between the kernels this is inlined into, and the
     // single-step and two-step kernels must give the same bits.
     const float usq15 = 1.5f * fmaf(ux, ux, uy * uy);
-    const float wd1 = (1.0f / 9.0f) * drho, wr1 = (1.0f / 9.0f) * rho;
-    const float wd2 = (1.0f / 36.0f) * drho, wr2 = (1.0f / 36.0f) * rho;
+    const float wr1 = (1.0f / 9.0f) * rho, wr2 = (1.0f / 36.0f) * rho;
     float r = 0.f;   // sum over i > 0 of (d_i - d_i')
-    auto pair = [&](int i, int j, float cu, float wd, float wr) {
+    auto pair = [&](int i, int j, float cu, float w, float wr) {
         const float t = fmaf(4.5f * cu, cu, -usq15);
         const float c3 = 3.0f * cu;
-        const float ei = fmaf(wr, t + c3, wd), ej = fmaf(wr, t - c3, wd);
-        float oi = fmaf(omega, ei - p[i], p[i]);
-        float oj = fmaf(omega, ej - p[j], p[j]);
+        // deq - d = (w drho - d) + w rho (t +- 3 cu)
+        float ni = fmaf(wr, t + c3, fmaf(w, drho, -p[i]));
+        float nj = fmaf(wr, t - c3, fmaf(w, drho, -p[j]));
         if (has_force) {
-            oi += force[i];
-            oj += force[j];
+            ni += force[i];
+            nj += force[j];
         }
+        const float oi = ST::rnd(fmaf(omega, ni, p[i]));
+        const float oj = ST::rnd(fmaf(omega, nj, p[j]));
         out[i] = oi;
         out[j] = oj;
         r += (p[i] - oi) + (p[j] - oj);
     };
-    pair(1, 3, ux, wd1, wr1);
-    pair(2, 4, uy, wd1, wr1);
-    pair(5, 7, ux + uy, wd2, wr2);
-    pair(6, 8, uy - ux, wd2, wr2);
-    float fs = 0.f;   // mass the force adds
-    if (has_force) {
-#pragma unroll
-        for (int i = 0; i < Q; ++i) fs += force[i];
-    }
-    out[0] = (p[0] + r) + fs;
+    pair(1, 3, ux, 1.0f / 9.0f, wr1);
+    pair(2, 4, uy, 1.0f / 9.0f, wr1);
+    pair(5, 7, ux + uy, 1.0f / 36.0f, wr2);
+    pair(6, 8, uy - ux, 1.0f / 36.0f, wr2);
+    out[0] = has_force ? (p[0] + r) + force[0] : p[0] + r;
 }
 
 // Reciprocal for the deviation forms (hardware approximation, 1 ulp).
@@ -241,6 +288,25 @@ static_assert((double)kZeroTolF <= kZeroTol && (double)0x1.cd2b2ap-54f > kZeroTo
 __device__ inline bool is_dry(float r) {
     const float m = fabsf(r);
     return m <= kZeroTolF || m == __builtin_inff();
+}
+
+// The zero-density rule (src/lbm.cpp:262-269) keeps the previous step's
+// equilibrium velocity where |rho| <= 1e-16.  The fused FP64 kernel holds it
+// in HBM only where rho < kUeqWatch (ueq_hold), so a site whose density drops
+// from >= kUeqWatch to <= 1e-16 in one step would reuse an older value.  Each
+// hold records its step in stamp[s]; a reuse at step n (ueq_reuse) checks
+// that the value held is step n-1's (or that the host declared every value
+// valid since, stamp[plane + 1]) and otherwise raises the flag stamp[plane],
+// which the host turns into an error (SCL_ERR_INEXACT).  The deviation forms,
+// which are not bit-exact with the reference anyway, hold the velocity below
+// thresholds far above their density noise (StoreF32 / StoreF16 ::kUeqWatch)
+// and keep no records.
+__device__ inline void ueq_hold(unsigned* stamp, long long s_idx, unsigned n) { G(stamp)[s_idx] = n; }
+__device__ inline void ueq_reuse(unsigned* stamp, long long plane, long long s_idx, unsigned n) {
+    const unsigned last = G(stamp)[s_idx], valid = G(stamp)[plane + 1];
+    if ((int)(n - 1u - last) > 0 && (int)(n - 1u - valid) > 0)
+        __hip_atomic_fetch_or(stamp + plane, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    ueq_hold(stamp, s_idx, n);
 }
 
 // Bounce-back mode of the gathering kernels: 0 selects the pull source per
@@ -325,7 +391,7 @@ __device__ inline void finish1(const CompArgs<typename ST::T>& ca, void* dst, lo
             momentum(p, jx, jy);
             const float inv = rcp(rho);
             float out[Q];
-            relax_dev(p, rho, drho, jx * inv, jy * inv, ca.omega, ca.force, ca.has_force, out);
+            relax_dev<ST>(p, rho, drho, jx * inv, jy * inv, ca.omega, ca.force, ca.has_force, out);
 #pragma unroll
             for (int i = 0; i < Q; ++i) st_at<ST>(dst, (OffT)i * plane_b + home, out[i]);
             if constexpr (GW) ghost_stores<ST, OffT>(dst, plane_b, nbr, nb, out);
@@ -366,18 +432,18 @@ __device__ inline UeqPtrs ueq_ptrs(const StepArgs<T>& a) {
 // FORCE = false: no body force on either component (has_force is then not
 // read, so kernels built for force-free lattices carry neither the force
 // values nor the additions).
-template <class T, class U, bool FORCE = true>
+template <class T, class U, bool FORCE = true, class ST = StoreF32>
 __device__ inline void collide2_core(const StepArgs<T>& a, const U& u, int s_idx, const float (&p)[2][Q],
                                      const float (&rc)[2], const float (&nr)[2][Q], unsigned nb, float (&oa)[Q],
                                      float (&ob)[Q]);
 
-template <class T>
+template <class T, class ST>
 __device__ inline void collide2_dev(const StepArgs<T>& a, int s_idx, const float (&p)[2][Q], const float (&rc)[2],
                                     const float (&nr)[2][Q], unsigned nb, float (&oa)[Q], float (&ob)[Q]) {
-    collide2_core(a, ueq_ptrs(a), s_idx, p, rc, nr, nb, oa, ob);
+    collide2_core<T, UeqPtrs, true, ST>(a, ueq_ptrs(a), s_idx, p, rc, nr, nb, oa, ob);
 }
 
-template <class T, class U, bool FORCE>
+template <class T, class U, bool FORCE, class ST>
 __device__ inline void collide2_core(const StepArgs<T>& a, const U& u, int s_idx, const float (&p)[2][Q],
                                      const float (&rc)[2], const float (&nr)[2][Q], unsigned nb, float (&oa)[Q],
                                      float (&ob)[Q]) {
@@ -433,7 +499,7 @@ __device__ inline void collide2_core(const StepArgs<T>& a, const U& u, int s_idx
         const float ir = rcp(ra);
         uea[0] = fmaf(A.inv_omega, fmaf(fa_x, ir, fs_ax), ucx);
         uea[1] = fmaf(A.inv_omega, fmaf(fa_y, ir, fs_ay), ucy);
-        if (fabsf(ra) < (float)kUeqWatch) {
+        if (fabsf(ra) < ST::kUeqWatch) {
             G(u.ox(0))[s_idx] = uea[0];
             G(u.oy(0))[s_idx] = uea[1];
         }
@@ -448,7 +514,7 @@ __device__ inline void collide2_core(const StepArgs<T>& a, const U& u, int s_idx
         const float ir = rcp(rb);
         ueb[0] = fmaf(B.inv_omega, fmaf(fb_x, ir, fs_bx), ucx);
         ueb[1] = fmaf(B.inv_omega, fmaf(fb_y, ir, fs_by), ucy);
-        if (fabsf(rb) < (float)kUeqWatch) {
+        if (fabsf(rb) < ST::kUeqWatch) {
             G(u.ox(1))[s_idx] = ueb[0];
             G(u.oy(1))[s_idx] = ueb[1];
         }
@@ -459,8 +525,8 @@ __device__ inline void collide2_core(const StepArgs<T>& a, const U& u, int s_idx
         G(u.ox(1))[s_idx] = ux0;
         G(u.oy(1))[s_idx] = uy0;
     }
-    relax_dev(p[0], ra, da0, uea[0], uea[1], A.omega, A.force, FORCE && A.has_force, oa);
-    relax_dev(p[1], rb, db0, ueb[0], ueb[1], B.omega, B.force, FORCE && B.has_force, ob);
+    relax_dev<ST>(p[0], ra, da0, uea[0], uea[1], A.omega, A.force, FORCE && A.has_force, oa);
+    relax_dev<ST>(p[1], rb, db0, ueb[0], ueb[1], B.omega, B.force, FORCE && B.has_force, ob);
 }
 
 template <class ST, bool REPLAY, class OffT, bool GW>
@@ -528,10 +594,12 @@ __device__ inline void finish2(const StepArgs<typename ST::T>& a, void* const (&
                     if (!REPLAY && fabs(r) < kUeqWatch) {
                         G(C.ueqx)[s_idx] = ue[c][0];
                         G(C.ueqy)[s_idx] = ue[c][1];
+                        ueq_hold(C.stamp, s_idx, C.step);
                     }
                 } else {
                     ue[c][0] = G(C.ueqx)[s_idx];
                     ue[c][1] = G(C.ueqy)[s_idx];
+                    if (!REPLAY) ueq_reuse(C.stamp, plane, s_idx, C.step);
                 }
             }
         } else if (REPLAY) {
@@ -578,7 +646,7 @@ __device__ inline void finish2(const StepArgs<typename ST::T>& a, void* const (&
         // deviation forms, FP32 arithmetic
         if (!REPLAY && fl) {
             float oa[Q], ob[Q];
-            collide2_dev(a, s_idx, p, rc, nr, nb, oa, ob);
+            collide2_dev<typename ST::T, ST>(a, s_idx, p, rc, nr, nb, oa, ob);
 #pragma unroll
             for (int i = 0; i < Q; ++i) {
                 st_at<ST>(dst[0], (OffT)i * plane_b + home, oa[i]);

@@ -800,7 +800,7 @@ void Scheduler::loop() {
             for (size_t k = 0; k < jobs.size(); ++k) {
                 Job* j = jobs[k];
                 j->steps_done += steps[k];
-                std::string why;
+                std::string why = j->device_error;
                 for (int c = 0; c < 2 && why.empty(); ++c) {
                     if (!std::isfinite(j->mass[c])) {
                         why = "non-finite mass of component " + std::to_string(c) + " after " +

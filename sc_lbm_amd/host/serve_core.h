@@ -114,6 +114,7 @@ struct Job {
     bool suspend = false;             // leave the device at the next round boundary
     std::vector<char> saved;          // exact state of a suspended job (backend format), else empty
     std::string error;
+    std::string device_error;         // set by the backend's round: fails the job at the watchdog
     std::vector<double> rho[2], f[2]; // deliverables (reference layout)
     std::vector<uint64_t> bits;       // packed mask (pack_mask), built before admission
     bool prepared = false;

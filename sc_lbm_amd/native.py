@@ -12,7 +12,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 # SCLBM_LIBRARY selects another build of the library (e.g. to compare builds)
 LIB_PATH = os.environ.get("SCLBM_LIBRARY") or os.path.join(HERE, "libsclbm.so")
 
-SCL_OK, SCL_ERR_ARG, SCL_ERR_HIP, SCL_ERR_RANGE, SCL_ERR_UNSUPPORTED, SCL_ERR_BUSY, SCL_ERR_TIMEOUT = range(7)
+(SCL_OK, SCL_ERR_ARG, SCL_ERR_HIP, SCL_ERR_RANGE, SCL_ERR_UNSUPPORTED, SCL_ERR_BUSY, SCL_ERR_TIMEOUT,
+ SCL_ERR_INEXACT) = range(8)
 FP64, FP32_DEV, FP16_DEV = 0, 1, 2
 F, FEQ, RHO, UX, UY, UEQX, UEQY, FRX, FRY, FSX, FSY = range(11)
 PRECISION = {"fp64": FP64, "fp32dev": FP32_DEV, "fp16dev": FP16_DEV}
@@ -46,6 +47,11 @@ class SclError(RuntimeError):
 
 class RangeError(SclError, IndexError):
     """The reference would throw std::out_of_range here."""
+
+
+class InexactError(SclError):
+    """A fused step reused an equilibrium velocity older than the previous
+    step's at a zero-density site (the data were still delivered)."""
 
 
 _lib = None
@@ -115,6 +121,8 @@ def check(rc):
         msg = load().scl_last_error().decode(errors="replace")
         if rc == SCL_ERR_RANGE:
             raise RangeError(rc, msg)
+        if rc == SCL_ERR_INEXACT:
+            raise InexactError(rc, msg)
         raise SclError(rc, msg)
 
 

@@ -67,3 +67,19 @@ def max_rel(a, b):
     """The reference tests' metric: |a-b| / max(1, |a|, |b|)."""
     d = np.abs(a - b) / np.maximum(1.0, np.maximum(np.abs(a), np.abs(b)))
     return float(np.max(d)) if d.size else 0.0
+
+
+def domain_length(phi):
+    """Domain size of a phase field: 2 pi / k1, k1 the first moment of the
+    radially averaged structure factor of phi - <phi> (phi real, square)."""
+    n = phi.shape[0]
+    s = np.abs(np.fft.rfft2(phi - phi.mean())) ** 2
+    ky = np.fft.fftfreq(n) * 2 * np.pi
+    kx = np.fft.rfftfreq(n) * 2 * np.pi
+    kk = np.sqrt(kx[None, :] ** 2 + ky[:, None] ** 2)
+    w = np.full(kx.size, 2.0)          # rfft columns stand for two of the full plane
+    w[0] = 1.0
+    if n % 2 == 0:
+        w[-1] = 1.0
+    s[0, 0] = 0.0
+    return 2 * np.pi * (w * s).sum() / (w * kk * s).sum()

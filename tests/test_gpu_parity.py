@@ -339,13 +339,20 @@ def test_two_steps_per_launch_equal_single_steps(prec, name):
 
 
 @pytest.mark.parametrize("prec", [nat.FP32_DEV, nat.FP16_DEV])
-def test_multi_step_kernels_at_every_band_height_remainder(prec):
+@pytest.mark.parametrize("name", ["spinodal", "porous"])
+def test_multi_step_kernels_at_every_band_height_remainder(prec, name):
     """The multi-step tiles walk their rows in a loop unrolled by 3 or 6,
     with ring slots fixed per unrolled row: band heights 20..25 end the
     walk at every remainder (a band runs its height plus 8 or 10 rows), and
     the last band of the 300-row lattice is partial.  Two and three steps
-    per launch equal single steps bit for bit at each."""
-    case = wl.spinodal(300, G=1.2, seed=9)
+    per launch equal single steps bit for bit at each, for the solid-free
+    tiles (spinodal) and for the tiles with solids (porous: walls, disks,
+    adhesion +-0.1 and a body force on A)."""
+    if name == "spinodal":
+        case = wl.spinodal(300, G=1.2, seed=9)
+    else:
+        case = wl.porous(nx=300, ny=300, porosity=0.75, seed=13, rmin=5, rmax=12)
+    forces = case.get("force", [None, None])
 
     def run(fusion, band):
         g = nat.Geometry(case["mask"])
@@ -355,7 +362,7 @@ def test_multi_step_kernels_at_every_band_height_remainder(prec):
         for c in range(2):
             fl[c].set_f(case["f"][c])
         nat.solid_surface_force(g, fl[0], fl[1])
-        nat.step2(g, fl[0], fl[1], None, None, 13)   # bootstrap step, then 12
+        nat.step2(g, fl[0], fl[1], forces[0], forces[1], 13)   # bootstrap step, then 12
         out = [f.get_f() for f in fl]
         for f in fl:
             f.close()
@@ -456,3 +463,42 @@ def test_multi_step_kernels_on_edge_shapes(prec, shape):
         assert np.all(np.isfinite(out[0][c]))
         assert np.array_equal(out[0][c], out[1][c])
         assert np.array_equal(out[0][c], out[2][c])
+
+
+def test_zero_density_rule_without_held_velocity_is_reported():
+    """The reference keeps a component's previous equilibrium velocity where
+    |rho| <= 1e-16 (src/lbm.cpp:262-269).  The fused kernels hold it only where
+    rho < 1e-8, so a density that falls from >= 1e-8 to <= 1e-16 within one
+    step is reported (SCL_ERR_INEXACT) instead of silently reusing an older
+    value.  Forced here: one site of component A holds f_E = a, f_W = b with
+    (a - b)/(a + b) = u where 1.5 u^2 rounds to exactly 1, so at tau = 1 its
+    rest population relaxes to feq_0 = (4/9) rho (1 - 1.5 u^2) = 0; every
+    other population of A is 0 and B is empty, so after one step that site's
+    density is exactly 0 after being 1.  The per-operator path keeps every
+    site's velocity and reports nothing."""
+    nx = ny = 16
+    fa = np.zeros((9, ny, nx))
+    fa[1, 8, 8] = 0.908248290463863
+    fa[3, 8, 8] = 0.09175170953613698
+    u = (fa[1, 8, 8] - fa[3, 8, 8]) / (fa[1, 8, 8] + fa[3, 8, 8])
+    assert 1.0 - 1.5 * (u * u) == 0.0
+    case = dict(mask=wl.open_mask(nx, ny), tau=[1.0, 1.0], g_fluid=[0.0, 0.0], g_solid=[0.0, 0.0],
+                force=[None, None], f=[fa.ravel(), np.zeros(9 * nx * ny)])
+    s = _oracle_run(case, 2)
+    site = 8 * nx + 8
+    assert s.get(0, po.RHO)[site] == 0.0 and s.get(0, po.UEQX)[site] == u   # the rule applied at step 2
+    g, fl = gh.run_fused(case, 2)
+    with pytest.raises(nat.InexactError, match="zero-density rule"):
+        fl[0].get_f()
+    # reported once; the populations are the reference's (the density there is 0)
+    for c in range(2):
+        assert np.array_equal(fl[c].get_f(), s.get_f(c))
+    g2, pl = gh.run_perop(case, 2)
+    for c in range(2):
+        assert np.array_equal(pl[c].get_f(), s.get_f(c))
+        assert np.array_equal(pl[c].download(nat.UEQX), s.get(c, po.UEQX))
+    # one more fused step after the per-operator ones: every velocity was held
+    nat.step2(g2, pl[0], pl[1], None, None, 1)
+    s.run(1, None, None)
+    for c in range(2):
+        assert np.array_equal(pl[c].get_f(), s.get_f(c))

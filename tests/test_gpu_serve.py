@@ -52,13 +52,16 @@ def dens_ref(f):
 def footprint(nx, ny, es, stage):
     """Device bytes of one resident job (the backend's block layout: mask,
     2 x 2 population buffers, equilibrium-velocity planes (4 in FP64, 3 x 4
-    rotating in the deviation forms) and 2 density planes in FP64, optional
-    FP64 staging; each region 256-byte aligned)."""
+    rotating in the deviation forms), 2 density planes in FP64, in FP64 the
+    zero-density rule's records (a 32-bit word per site and 2 more per
+    component), optional FP64 staging; each region 256-byte aligned)."""
     a = lambda n: (n + 255) // 256 * 256
     pitch = (nx + 63) // 64 * 64
     plane = pitch * ny
     nueq = 1 if es == 8 else 3
     total = a(ny * pitch // 64 * 8) + 4 * a(9 * plane * es) + (4 * nueq + 2) * a(plane * 8)
+    if es == 8:
+        total += 2 * a((plane + 2) * 4)
     return total + (a(9 * plane * 8) if stage else 0)
 
 
@@ -213,6 +216,12 @@ def test_autotuned_server_runs_and_reports_its_shape():
     fixed = run_alone(dict(nx=200, ny=120, steps=33, ic_seed=9, radius=30, deliver=serve.DELIVER_F, **DROP))
     for c in range(2):
         assert np.array_equal(tuned[c], fixed[c])
+    # with the width timed, a caller's rows and lookahead are kept
+    with serve.Server(precision=nat.FP32_DEV, budget_bytes=1 << 30, tile_rows=24, lookahead=2) as srv:
+        st = srv.stats()
+        assert st["tile_cols"] in (128, 256) and st["tile_rows"] == 24 and st["lookahead"] == 2
+    with pytest.raises(nat.SclError):
+        serve.Server(precision=nat.FP32_DEV, budget_bytes=1 << 30, lookahead=3)
 
 
 @pytest.mark.parametrize("form", ["fp32dev", "fp16dev"])

@@ -355,3 +355,19 @@ def test_observables_contact_angle_of_a_constructed_cap(h, half_base):
     assert abs(theta - np.degrees(2 * np.arctan(2 * H / L))) < 1e-9
     if L == 2 * H:
         assert theta == 90.0
+
+
+def test_domain_length_statistic():
+    """The domain-size statistic of the spinodal tests: for a single Fourier
+    mode phi = cos(2 pi k x / n) the structure factor sits at |k| = 2 pi k / n,
+    so L = n / k exactly, along x and along y; two modes of equal power give
+    the harmonic mean of their wavelengths weighted as the first moment."""
+    from tests.gpu_helpers import domain_length
+    n = 256
+    x = np.arange(n)
+    for k in (4, 16):
+        phi = np.cos(2 * np.pi * k * x / n)[None, :].repeat(n, 0)
+        assert abs(domain_length(phi) - n / k) < 1e-9
+        assert abs(domain_length(phi.T) - n / k) < 1e-9
+    phi = (np.cos(2 * np.pi * 4 * x / n) + np.cos(2 * np.pi * 12 * x / n))[None, :].repeat(n, 0)
+    assert abs(domain_length(phi) - n / 8.0) < 1e-9
