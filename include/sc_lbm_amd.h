@@ -178,6 +178,16 @@ int scl_step2(const scl_geometry* g, scl_field* a, scl_field* b,
  * device in a fixed order: out = {mass, momentum_x, momentum_y}. */
 int scl_moments(const scl_geometry* g, scl_field* f, double out[3]);
 
+/* scl_moments without allocating device memory (no reference counterpart),
+ * for callers that cannot allocate: code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc.  Same result, bit for bit.
+ * scl_moments stages the populations in an FP64 buffer the field allocates
+ * on first use; this variant reads them as stored and keeps its partial sums
+ * in a fixed device array of the library (3 doubles per row, at most 32768
+ * rows: SCL_ERR_ARG beyond), one per device for the life of the process.
+ * Calls that overlap on one device race for that array. */
+int scl_moments_noalloc(const scl_geometry* g, scl_field* f, double out[3]);
+
 /* ---- Droplet observables on the device.  The reference evaluates these in
  *      Python from the density files its drivers write; these entry points
  *      replace those scripts' arithmetic without downloading the fields.

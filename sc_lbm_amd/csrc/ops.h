@@ -207,6 +207,53 @@ __global__ void k_moments_rows(Grid2 g, const double* __restrict__ f, double* __
     }
 }
 
+// The same per-row partial triples as k_to_f64 followed by k_moments_rows
+// (the same per-site FP64 values, summed in the same order), read from the
+// populations as stored, gathered from the post-collision phase when GATHER:
+// no FP64 staging copy (scl_moments_noalloc).
+template <class ST, bool GATHER>
+__global__ void k_moments_direct(Grid2 g, const typename ST::S* __restrict__ src, double rho0,
+                                 double* __restrict__ part) {
+    const int y = blockIdx.x;
+    double m = 0.0, jx = 0.0, jy = 0.0;
+    for (int x = threadIdx.x; x < g.nx; x += blockDim.x) {
+        const unsigned nb = nbits_at(g, x, y);
+        const bool fl = nb & 1u;
+        typename ST::T q[Q];
+        if (!GATHER || fl)
+            load_site<ST, GATHER>(src, g.plane, g.pitch, x, wrap1(x - 1, g.nx), wrap1(x + 1, g.nx), y,
+                                  wrap1(y - 1, g.ny), wrap1(y + 1, g.ny), nb, q);
+        double p[Q];
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            if (GATHER && !fl) p[i] = 0.0;
+            else if (ST::kDev) p[i] = (double)q[i] + weq(i) * rho0;
+            else p[i] = (double)q[i];
+        }
+        double mx, my;
+        momentum(p, mx, my);
+        m += dens(p);
+        jx += mx;
+        jy += my;
+    }
+    __shared__ double red[3][256];
+    red[0][threadIdx.x] = m;
+    red[1][threadIdx.x] = jx;
+    red[2][threadIdx.x] = jy;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) part[3 * y + k] = red[k][0];
+    }
+}
+
 __global__ void k_moments_final(int rows, const double* __restrict__ part, double* __restrict__ out) {
     __shared__ double red[3][256];
     double acc[3] = {0.0, 0.0, 0.0};

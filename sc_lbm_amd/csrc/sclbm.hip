@@ -62,6 +62,15 @@ static int fail(int code, const char* fmt, ...) {
         if (rc_ != SCL_OK) return rc_; \
     } while (0)
 
+// Device memory of the allocation-free entry point (scl_moments_noalloc):
+// the per-row partial sums of the moment reduction and its result, for
+// lattices of up to kScratchRows rows.  Each device holds its own copy until
+// the process ends; every call overwrites what it reads, and calls that
+// overlap on one device race for it.  Host code uses its device address
+// (g_lib.scratch, from hipGetSymbolAddress), never the array's name.
+constexpr int kScratchRows = 32768;
+static __device__ __align__(256) double scratch[3 * kScratchRows + 3];
+
 // ---------------------------------------------------------------------------
 // library state
 // ---------------------------------------------------------------------------
@@ -76,8 +85,10 @@ struct Lib {
     std::vector<int> ev_steps;   // time steps covered by each recorded launch
     int cus = 0;                 // compute units of the device
     int x2_occ[3][2] = {};       // [precision][solid-free]: two-step workgroups per CU (0 = not queried)
+    double* scratch = nullptr;   // address of `scratch` on the library's device (set at initialisation)
 };
 Lib g_lib;
+
 const size_t kMaxTimed = 1 << 15;
 
 int lib_ready() {
@@ -93,6 +104,8 @@ int lib_ready() {
     g_lib.cus = prop.multiProcessorCount;
     // default priority (no priority argument): the library never changes it
     HIPCHK(hipStreamCreateWithFlags(&g_lib.stream, hipStreamNonBlocking));
+    // the allocation-free entry point's array: its address on this device
+    HIPCHK(hipGetSymbolAddress((void**)&g_lib.scratch, HIP_SYMBOL(scratch)));
     g_lib.ready = true;
     return SCL_OK;
 }
@@ -1435,6 +1448,45 @@ int stage_f64(const scl_geometry* g, scl_field* f, Grid2* gr_out) {
     HIPCHK(hipGetLastError());
     if (gr_out) *gr_out = gr;
     return SCL_OK;
+}
+
+// scl_moments without allocating: the populations are read as stored (and
+// gathered on the fly) instead of through the field's FP64 staging buffer,
+// and the partial sums go to the static `scratch` array instead of the
+// geometry's.  Same per-site values and summation order as scl_moments, so
+// the same bits.
+int scl_moments_noalloc(const scl_geometry* g, scl_field* f, double out[3]) {
+    CHK(check_geom(g, f));
+    if (!out) return fail(SCL_ERR_ARG, "null output");
+    if (g->ny > kScratchRows)
+        return fail(SCL_ERR_ARG, "lattice of %d rows: the allocation-free moments hold partial sums for at most %d",
+                    g->ny, kScratchRows);
+    CHK(lib_ready());
+    double* part = g_lib.scratch;
+    const bool gth = f->phase == PH_P;
+    const Grid2 gr = (gth && f->pg ? f->pg : g)->grid();
+    const void* s = f->f[f->cur];
+    const dim3 grid(g->ny), block(256);
+    switch (f->precision) {
+        case SCL_FP64:
+            if (gth) hipLaunchKernelGGL((k_moments_direct<StoreF64, true>), grid, block, 0, g_lib.stream, gr, (const double*)s, 0.0, part);
+            else hipLaunchKernelGGL((k_moments_direct<StoreF64, false>), grid, block, 0, g_lib.stream, gr, (const double*)s, 0.0, part);
+            break;
+        case SCL_FP32_DEV:
+            if (gth) hipLaunchKernelGGL((k_moments_direct<StoreF32, true>), grid, block, 0, g_lib.stream, gr, (const float*)s, f->rho0, part);
+            else hipLaunchKernelGGL((k_moments_direct<StoreF32, false>), grid, block, 0, g_lib.stream, gr, (const float*)s, f->rho0, part);
+            break;
+        default:
+            if (gth) hipLaunchKernelGGL((k_moments_direct<StoreF16, true>), grid, block, 0, g_lib.stream, gr, (const _Float16*)s, f->rho0, part);
+            else hipLaunchKernelGGL((k_moments_direct<StoreF16, false>), grid, block, 0, g_lib.stream, gr, (const _Float16*)s, f->rho0, part);
+            break;
+    }
+    hipLaunchKernelGGL(k_moments_final, dim3(1), dim3(256), 0, g_lib.stream, g->ny, (const double*)part,
+                       part + 3 * (size_t)g->ny);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, part + 3 * (size_t)g->ny, 3 * sizeof(double), hipMemcpyDeviceToHost, g_lib.stream));
+    HIPCHK(hipStreamSynchronize(g_lib.stream));
+    return check_ueq_flag(f);
 }
 
 // Row extents of the two droplet-observable site sets (k_obs_rows) and the

@@ -25,7 +25,7 @@ EXPORTS = [
     "scl_field_download", "scl_compute_density", "scl_compute_velocities",
     "scl_compute_f_equilibrium_geom", "scl_compute_f_equilibrium", "scl_solid_surface_force",
     "scl_fluid_repulsion", "scl_equilibrium_velocities", "scl_collide1", "scl_collide2",
-    "scl_volume_force", "scl_stream1", "scl_stream2", "scl_step1", "scl_step2", "scl_moments",
+    "scl_volume_force", "scl_stream1", "scl_stream2", "scl_step1", "scl_step2", "scl_moments", "scl_moments_noalloc",
     "scl_set_tile", "scl_set_lookahead", "scl_get_tile", "scl_autotune", "scl_timing", "scl_probe_copy",
     "scl_set_step_fusion",
     "scl_get_step_fusion",
@@ -93,6 +93,7 @@ def load():
         "scl_step1": [_vp, _vp, _dp, ctypes.c_int],
         "scl_step2": [_vp, _vp, _vp, _dp, _dp, ctypes.c_int],
         "scl_moments": [_vp, _vp, _dp],
+        "scl_moments_noalloc": [_vp, _vp, _dp],
         "scl_laplace_observables": [_vp, _vp, _vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp],
         "scl_contact_angle": [_vp, _vp, _vp, ctypes.c_double, ctypes.c_double, _dp],
@@ -258,6 +259,12 @@ class Field:
     def moments(self):
         out = np.zeros(3)
         check(load().scl_moments(self.geom.h, self.h, out.ctypes.data_as(_dp)))
+        return out
+
+    def moments_noalloc(self):
+        """scl_moments_noalloc: the same sums without allocating device memory."""
+        out = np.zeros(3)
+        check(load().scl_moments_noalloc(self.geom.h, self.h, out.ctypes.data_as(_dp)))
         return out
 
     def close(self):
