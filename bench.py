@@ -53,6 +53,8 @@ def parse():
     p.add_argument("--serve-load", type=float, default=0.8, help="offered load / measured server capacity")
     p.add_argument("--serve-budget-gib", type=float, default=64.0)
     p.add_argument("--serve-sample", type=int, default=20, help="every k-th job is rerun in FP64 (5%%)")
+    p.add_argument("--serve-part", default="", help="I/P: serve only the I-th of P equal slices of the stream "
+                   "(jobs keep their stream numbers; per-job latencies are added to the output)")
     return p.parse_args()
 
 
@@ -124,6 +126,11 @@ def served_stream(nat, a):
     from sc_lbm_amd import serve
     from sc_lbm_amd import workloads as wl
     jobs = wl.served_stream(a.serve_jobs, seed=2025, step_scale=a.serve_step_scale)
+    first_job = 0
+    if a.serve_part:
+        part, parts = (int(x) for x in a.serve_part.split("/"))
+        first_job = part * len(jobs) // parts
+        jobs = jobs[first_job:(part + 1) * len(jobs) // parts]
     cfg = dict(precision=nat.FP32_DEV, budget_bytes=int(a.serve_budget_gib * (1 << 30)), max_jobs=256,
                max_pending=max(4096, 2 * a.serve_jobs))
     with serve.Server(**cfg) as srv:
@@ -145,7 +152,7 @@ def served_stream(nat, a):
         log("  capacity %.0f MLUPS; %d jobs at %.1f/s (offered load %.2f), mean %.3g LU/job"
             % (cap, len(jobs), rate, a.serve_load, lus.mean()))
         t0 = srv.stats()["clock_ms"] + 50.0
-        sample = set(range(0, len(jobs), a.serve_sample))
+        sample = set(k for k in range(len(jobs)) if (first_job + k) % a.serve_sample == 0)
         ids = []
         for k, j in enumerate(jobs):
             ids.append(srv.submit(release_ms=t0 + arr[k], deliver=serve.DELIVER_RHO if k in sample else 0, **j))
@@ -173,12 +180,12 @@ def served_stream(nat, a):
             e = max(float(np.max(np.abs(s64.result(i, serve.RHO, c) - rho32[k][c]))) for c in range(2))
             errs.append(e)
             j = jobs[k]
-            worst.append(dict(err=e, job=k, nx=j["nx"], ny=j["ny"], steps=j["steps"], tau=round(j["tau"][0], 4),
+            worst.append(dict(err=e, job=first_job + k, nx=j["nx"], ny=j["ny"], steps=j["steps"], tau=round(j["tau"][0], 4),
                               G=round(j["g_fluid"][0], 4), porosity=round(j["porosity"], 4),
                               ic="droplet" if j["ic"] == serve.IC_DROPLET else "stratified",
                               rho_min_fp64=min(float(s64.result(i, serve.RHO, c).min()) for c in range(2))))
     worst.sort(key=lambda w: -w["err"])
-    failed_jobs = [dict(job=k, nx=jobs[k]["nx"], ny=jobs[k]["ny"], steps=jobs[k]["steps"],
+    failed_jobs = [dict(job=first_job + k, nx=jobs[k]["nx"], ny=jobs[k]["ny"], steps=jobs[k]["steps"],
                         tau=round(jobs[k]["tau"][0], 4), G=round(jobs[k]["g_fluid"][0], 4),
                         porosity=round(jobs[k]["porosity"], 4), error=sts[k].get("error", ""),
                         steps_done=sts[k].get("steps_done"))
@@ -186,6 +193,12 @@ def served_stream(nat, a):
     scope = ("the served stream as specified (steps U[2000, 50000])" if a.serve_step_scale == 1.0 else
              "a shortened served stream: the specified steps U[2000, 50000] x %g; not the specified "
              "configuration, whose full run is recorded in profiles/served_stream_full.md" % a.serve_step_scale)
+    extra = {}
+    if a.serve_part:
+        extra = dict(part=a.serve_part, first_job=first_job, sample_errs=errs,
+                     latency_ms=[round(float(s["done_ms"] - s["release_ms"]), 1) if s["state"] == "done" else None
+                                 for s in sts],
+                     lattice_updates=lu_done, first_release_ms=first, last_done_ms=last)
     return dict(scope=scope, jobs=len(jobs), done=len(done), failed=sum(s["state"] == "failed" for s in sts),
                 failed_jobs=failed_jobs,
                 step_scale=a.serve_step_scale, steps_range=[int(round(2000 * a.serve_step_scale)),
@@ -208,7 +221,7 @@ def served_stream(nat, a):
                 sample_jobs=len(errs), sample_max_abs_rho_err=max(errs) if errs else None,
                 sample_median_abs_rho_err=float(np.median(errs)) if errs else None,
                 sample_frac_within_1e5=float(np.mean(np.array(errs) <= 1e-5)) if errs else None,
-                sample_worst=worst[:5])
+                sample_worst=worst[:5], **extra)
 
 
 def cpu_baseline(n, steps):

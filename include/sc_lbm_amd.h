@@ -352,6 +352,8 @@ typedef struct scl_server_config {
                                    (same bits as 1) with a deviation form; 0 = automatic: 2 with
                                    FP32 storage, 1 with FP16 (two measured slower there) */
     int band_rows;              /* rows per band of the two-step tiles; 0 = 32            */
+    int use_graphs;             /* 1: each round replays a captured HIP graph; 0: the same
+                                   launches issued one by one (profilers, debugging)      */
 } scl_server_config;
 
 typedef struct scl_server_stats {
@@ -369,11 +371,12 @@ typedef struct scl_server_stats {
     int tile_cols, tile_rows, lookahead, round_steps, workgroups;
     int steps_per_launch;       /* 2 when wide jobs take two steps per launch          */
     long long fluid_updates;    /* lattice updates of fluid sites only                 */
+    long long pair_updates;     /* lattice updates made by two-step launches           */
 } scl_server_stats;
 
 /* Defaults: FP32 deviations, 8 GiB arena, 64 slots, 4096 pending, rounds of
  * 32 steps, tile timed at start-up (16 rows), mass_tol by precision,
- * max_bypass 16. */
+ * max_bypass 16, graphs on. */
 int scl_server_default_config(scl_server_config* cfg);
 /* Creates the server on the library's device and starts its request loop. */
 int scl_server_create(const scl_server_config* cfg, scl_server** out);

@@ -58,10 +58,13 @@ struct ServeHdr {
     double* mass;               // 2 per job
     int n_items, n_jobs;
     int bw;                     // tile width in columns
-    int n_step;                 // k_serve takes items [0, n_step): the jobs it has steps of
     const uint64_t* items2;     // two-step tiles (strips of x2_out columns)
     int n_items2;
     unsigned* flags;            // per job: the zero-density rule's flags of both components
+    // per step s of the round, three ints P, B, E: k_serve's launch s takes
+    // items [0, P) (the single-step jobs with more than s steps) and [B, E)
+    // (the two-step jobs whose odd last step is s)
+    const int* sched;
 };
 
 // Tile encoding, 16 bits each from the low end: job, strip, y0, y1.
@@ -83,7 +86,9 @@ k_serve(const ServeHdr* __restrict__ hdr, int s, unsigned* __restrict__ ctr) {
     using T = typename ST::T;
     __shared__ DensRing<2, T, NW> lds;
     __shared__ int ticket;
-    const int n = hdr->n_step;
+    const SCL_AS_CONST int* sched = K(hdr->sched);
+    const int P = sched[3 * s], B = sched[3 * s + 1], n = P + (sched[3 * s + 2] - B);
+    if (n == 0) return;   // nothing at step s: the counters stay zero untouched
     const SCL_AS_CONST JobArgs<T>* jobs = K((const JobArgs<T>*)hdr->jobs);
     const SCL_AS_CONST uint64_t* items = K(hdr->items);
     for (;;) {
@@ -94,7 +99,7 @@ k_serve(const ServeHdr* __restrict__ hdr, int s, unsigned* __restrict__ ctr) {
         const int t = __builtin_amdgcn_readfirstlane(ticket);
         if (t >= n) break;
         int j, bx, y0, y1;
-        unpack_item(items[t], j, bx, y0, y1);
+        unpack_item(items[t < P ? t : B + (t - P)], j, bx, y0, y1);
         j = __builtin_amdgcn_readfirstlane(j);
         bx = __builtin_amdgcn_readfirstlane(bx);
         y0 = __builtin_amdgcn_readfirstlane(y0);

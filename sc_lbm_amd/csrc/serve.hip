@@ -43,6 +43,8 @@ namespace {
 
 const char* hip_msg(hipError_t e) { return hipGetErrorString(e); }
 
+constexpr int kMaxRoundSteps = 1024;   // longest round (scl_server_config.round_steps)
+
 #define SHIP(expr)                                                                     \
     do {                                                                               \
         hipError_t e_ = (expr);                                                        \
@@ -124,9 +126,10 @@ class GpuServe final : public serve::Backend {
         if (stream_) (void)hipStreamSynchronize(stream_);
         for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
         for (void* p : {(void*)arena_, (void*)hdr_d_, (void*)ctr_d_, table_d_, (void*)items_d_, (void*)partial_d_,
-                        (void*)mass_d_, (void*)items2_d_, (void*)flags_d_})
+                        (void*)mass_d_, (void*)items2_d_, (void*)flags_d_, (void*)sched_d_})
             if (p) (void)hipFree(p);
-        for (void* p : {(void*)hdr_h_, table_h_, (void*)items_h_, (void*)mass_h_, (void*)items2_h_, (void*)flags_h_})
+        for (void* p : {(void*)hdr_h_, table_h_, (void*)items_h_, (void*)mass_h_, (void*)items2_h_, (void*)flags_h_,
+                        (void*)sched_h_})
             if (p) (void)hipHostFree(p);
         for (hipEvent_t e : ev_)
             if (e) (void)hipEventDestroy(e);
@@ -156,6 +159,8 @@ class GpuServe final : public serve::Backend {
         SHIP(hipMalloc(&table_d_, (size_t)cfg_.max_jobs * sizeof(JobArgs<T>)));
         SHIP(hipMalloc(&mass_d_, (size_t)cfg_.max_jobs * 2 * sizeof(double)));
         SHIP(hipMalloc(&flags_d_, (size_t)cfg_.max_jobs * sizeof(unsigned)));
+        SHIP(hipMalloc(&sched_d_, (size_t)3 * kMaxRoundSteps * sizeof(int)));
+        SHIP(hipHostMalloc(&sched_h_, (size_t)3 * kMaxRoundSteps * sizeof(int), hipHostMallocDefault));
         SHIP(hipHostMalloc(&hdr_h_, sizeof(ServeHdr), hipHostMallocDefault));
         SHIP(hipHostMalloc(&table_h_, (size_t)cfg_.max_jobs * sizeof(JobArgs<T>), hipHostMallocDefault));
         SHIP(hipHostMalloc(&mass_h_, (size_t)cfg_.max_jobs * 2 * sizeof(double), hipHostMallocDefault));
@@ -230,20 +235,28 @@ class GpuServe final : public serve::Backend {
     int run_round(const std::vector<Job*>& jobs, const std::vector<int>& steps, int rsteps,
                   std::string* err) override {
         SHIP(hipSetDevice(dev_));
-        // two-step jobs: at least 256 columns and two steps in the round
+        // two-step jobs: two steps in the round and at least kX2MinCols
+        // columns (a job narrower than a strip takes one strip, whose lanes
+        // past its columns and halos idle)
         std::vector<char> two(jobs.size(), 0);
         for (size_t k = 0; k < jobs.size(); ++k)
-            two[k] = x2_ && rsteps >= 2 && steps[k] >= 2 && jobs[k]->spec.nx >= kX2Threads;
-        // work-list order: first the jobs k_serve has steps of (largest first),
-        // then the two-step jobs with an even step count (their tiles only
-        // serve the mass reduction)
+            two[k] = x2_ && rsteps >= 2 && steps[k] >= 2 && jobs[k]->spec.nx >= kX2MinCols;
+        // work-list order: (0) the single-step jobs, most steps first; (1) the
+        // two-step jobs with an odd step count, fewest steps first; (2) the
+        // other two-step jobs (their tiles only serve the mass reduction).
+        // k_serve's launch s then takes a prefix of group 0 and the group-1
+        // jobs whose last step is s, both contiguous.
         std::vector<int> order(jobs.size());
         for (size_t k = 0; k < order.size(); ++k) order[k] = (int)k;
-        auto single = [&](int k) { return !two[k] || (steps[k] & 1); };
+        auto group = [&](int k) { return !two[k] ? 0 : ((steps[k] & 1) ? 1 : 2); };
         std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-            if (single(a) != single(b)) return single(a);
+            if (group(a) != group(b)) return group(a) < group(b);
+            if (group(a) == 0 && steps[a] != steps[b]) return steps[a] > steps[b];
+            if (group(a) == 1 && steps[a] != steps[b]) return steps[a] < steps[b];
             return jobs[a]->sites() > jobs[b]->sites();
         });
+        int* sched = sched_h_;
+        for (int s = 0; s < 3 * rsteps; ++s) sched[s] = 0;
         size_t need = 0, need2 = 0;
         for (size_t k = 0; k < jobs.size(); ++k) {
             need += tiles_of(jobs[k]->spec.nx, jobs[k]->spec.ny, nw_, rows_);
@@ -253,7 +266,7 @@ class GpuServe final : public serve::Backend {
         if (rc == SCL_OK) rc = grow_items2(need2, err);
         if (rc != SCL_OK) return rc;
         JobArgs<T>* table = (JobArgs<T>*)table_h_;
-        int n = 0, n_step = 0, n2 = 0;
+        int n = 0, n2 = 0;
         bool force2 = false;   // a two-step job of the round has a body force
         for (int k : order) {
             const GJob* g = (const GJob*)jobs[k]->dev;
@@ -267,24 +280,38 @@ class GpuServe final : public serve::Backend {
             ja.item_begin = n;
             n = emit_tiles(k, g->nx, g->ny, nw_, rows_, items_h_, n);
             ja.item_end = n;
-            if (single(k)) n_step = n;
+            if (group(k) == 0) {
+                for (int s = 0; s < steps[k]; ++s) sched[3 * s] = n;
+            } else if (group(k) == 1) {
+                const int s = steps[k] - 1;
+                if (sched[3 * s + 2] == 0) sched[3 * s + 1] = ja.item_begin;
+                sched[3 * s + 2] = n;
+            }
             if (two[k]) {
                 n2 = emit_tiles2(k, g->nx, g->ny, x2rows_, items2_h_, n2);
                 force2 = force2 || jobs[k]->spec.has_force[0] || jobs[k]->spec.has_force[1];
             }
         }
-        *hdr_h_ = ServeHdr{table_d_, items_d_, partial_d_, mass_d_, n, (int)jobs.size(), 64 * nw_, n_step,
-                           items2_d_, n2, flags_d_};
+        *hdr_h_ = ServeHdr{table_d_, items_d_, partial_d_, mass_d_, n, (int)jobs.size(), 64 * nw_,
+                           items2_d_, n2, flags_d_, sched_d_};
+        SHIP(hipMemcpyAsync(sched_d_, sched_h_, (size_t)3 * rsteps * sizeof(int), hipMemcpyHostToDevice, stream_));
         SHIP(hipMemcpyAsync(table_d_, table_h_, jobs.size() * sizeof(JobArgs<T>), hipMemcpyHostToDevice, stream_));
         SHIP(hipMemcpyAsync(items_d_, items_h_, (size_t)n * sizeof(uint64_t), hipMemcpyHostToDevice, stream_));
         if (n2) SHIP(hipMemcpyAsync(items2_d_, items2_h_, (size_t)n2 * sizeof(uint64_t), hipMemcpyHostToDevice,
                                     stream_));
         SHIP(hipMemcpyAsync(hdr_d_, hdr_h_, sizeof(ServeHdr), hipMemcpyHostToDevice, stream_));
         hipGraphExec_t ge = nullptr;
-        rc = graph(rsteps, n2 > 0, force2, &ge, err);
-        if (rc != SCL_OK) return rc;
+        if (cfg_.use_graphs) {
+            rc = graph(rsteps, n2 > 0, force2, &ge, err);
+            if (rc != SCL_OK) return rc;
+        }
         SHIP(hipEventRecord(ev_[0], stream_));
-        SHIP(hipGraphLaunch(ge, stream_));
+        if (ge) {
+            SHIP(hipGraphLaunch(ge, stream_));
+        } else {
+            launch_round(rsteps, n2 > 0, force2);
+            SHIP(hipGetLastError());
+        }
         SHIP(hipEventRecord(ev_[1], stream_));
         SHIP(hipMemcpyAsync(mass_h_, mass_d_, jobs.size() * 2 * sizeof(double), hipMemcpyDeviceToHost, stream_));
         SHIP(hipMemcpyAsync(flags_h_, flags_d_, jobs.size() * sizeof(unsigned), hipMemcpyDeviceToHost, stream_));
@@ -299,6 +326,7 @@ class GpuServe final : public serve::Backend {
             g->cur ^= launches(steps[k], two[k]) & 1;
             g->ucur = (g->ucur + steps[k]) % 3;
             g->nstep += (unsigned)steps[k];
+            if (two[k]) pair_updates_ += (long long)(steps[k] & ~1) * g->nx * g->ny;
             if (flags_h_[k])
                 jobs[k]->device_error = "zero-density rule: a site's density fell from >= 1e-8 to <= 1e-16 "
                                         "within one step without its previous equilibrium velocity held";
@@ -348,6 +376,7 @@ class GpuServe final : public serve::Backend {
     int workgroups() const override { return grid_; }
     int steps_per_launch() const override { return x2_ ? 2 : 1; }
     double device_ms() const override { return device_ms_.load(); }
+    long long pair_updates() const override { return pair_updates_.load(); }
 
   private:
     static void fill_ueq(JobArgs<T>& ja, const GJob& g) {
@@ -376,6 +405,9 @@ class GpuServe final : public serve::Backend {
     // Launches that swap a job's buffers in a round of `steps` steps.
     static int launches(int steps, bool two) { return two ? steps / 2 + (steps & 1) : steps; }
 
+    // Narrowest job of the two-step kernel: a strip's halo lanes wrap at
+    // most once (2 * K - 1 = 3 columns each side).
+    static constexpr int kX2MinCols = 8;
     // output columns of a two-step strip in this storage form (the tile's)
     static constexpr int kOut2 = x2_out<ST>();
     static size_t tiles2_of(int nx, int ny, int rows) {
@@ -600,12 +632,7 @@ class GpuServe final : public serve::Backend {
             return SCL_OK;
         }
         SHIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-        for (int s = 0; s < rsteps; ++s) {
-            if (two && !(s & 1) && s + 1 < rsteps) launch_pair(s / 2, force);
-            launch_step(nw_, la_, grid_, s);
-        }
-        hipLaunchKernelGGL(k_serve_mass<ST>, dim3(mass_grid_), dim3(256), 0, stream_, hdr_d_);
-        hipLaunchKernelGGL(k_serve_mass_final<T>, dim3(cfg_.max_jobs), dim3(256), 0, stream_, hdr_d_);
+        launch_round(rsteps, two, force);
         hipGraph_t gr = nullptr;
         hipError_t e = hipStreamEndCapture(stream_, &gr);
         if (e != hipSuccess) {
@@ -622,6 +649,17 @@ class GpuServe final : public serve::Backend {
         graphs_[key] = ge;
         *out = ge;
         return SCL_OK;
+    }
+
+    // The launches of a round: `rsteps` steps (pairs of steps through
+    // k_serve2x where `two`), then the per-job FP64 mass.
+    void launch_round(int rsteps, bool two, bool force) {
+        for (int s = 0; s < rsteps; ++s) {
+            if (two && !(s & 1) && s + 1 < rsteps) launch_pair(s / 2, force);
+            launch_step(nw_, la_, grid_, s);
+        }
+        hipLaunchKernelGGL(k_serve_mass<ST>, dim3(mass_grid_), dim3(256), 0, stream_, hdr_d_);
+        hipLaunchKernelGGL(k_serve_mass_final<T>, dim3(cfg_.max_jobs), dim3(256), 0, stream_, hdr_d_);
     }
 
     void launch_pair(int q, bool force) {
@@ -713,13 +751,19 @@ class GpuServe final : public serve::Backend {
                         n = emit_tiles(k, g->nx, g->ny, nw, rows, items_h_, n);
                         table[k].item_end = n;
                     }
-                    *hdr_h_ = ServeHdr{table_d_, items_d_, partial_d_, mass_d_, n, 2, 64 * nw, n, items2_d_, 0,
-                                       flags_d_};
+                    *hdr_h_ = ServeHdr{table_d_, items_d_, partial_d_, mass_d_, n, 2, 64 * nw, items2_d_, 0,
+                                       flags_d_, sched_d_};
+                    for (int s = 0; s < 10; ++s) {
+                        sched_h_[3 * s] = n;
+                        sched_h_[3 * s + 1] = sched_h_[3 * s + 2] = 0;
+                    }
                     if (hipMemcpyAsync(table_d_, table_h_, 2 * sizeof(JobArgs<T>), hipMemcpyHostToDevice,
                                        stream_) != hipSuccess ||
                         hipMemcpyAsync(items_d_, items_h_, (size_t)n * sizeof(uint64_t), hipMemcpyHostToDevice,
                                        stream_) != hipSuccess ||
                         hipMemcpyAsync(hdr_d_, hdr_h_, sizeof(ServeHdr), hipMemcpyHostToDevice, stream_) !=
+                            hipSuccess ||
+                        hipMemcpyAsync(sched_d_, sched_h_, 30 * sizeof(int), hipMemcpyHostToDevice, stream_) !=
                             hipSuccess) {
                         *err = "autotune upload failed";
                         rc = SCL_ERR_HIP;
@@ -763,6 +807,8 @@ class GpuServe final : public serve::Backend {
     char* arena_ = nullptr;
     ServeHdr* hdr_d_ = nullptr;
     ServeHdr* hdr_h_ = nullptr;
+    int* sched_d_ = nullptr;   // k_serve's item ranges per step of the round (ServeHdr::sched)
+    int* sched_h_ = nullptr;
     unsigned* ctr_d_ = nullptr;
     void* table_d_ = nullptr;
     void* table_h_ = nullptr;
@@ -783,6 +829,7 @@ class GpuServe final : public serve::Backend {
     std::map<int, hipGraphExec_t> graphs_;
     hipEvent_t ev_[2] = {nullptr, nullptr};
     std::atomic<double> device_ms_{0.0};   // written by the request loop, read by stats
+    std::atomic<long long> pair_updates_{0};
 };
 
 template <class ST>
@@ -805,7 +852,8 @@ extern "C" int scl_server_create(const scl_server_config* cfg, scl_server** out)
         return serve::report(SCL_ERR_ARG, "bad precision " + std::to_string(c.precision));
     if (c.max_jobs < 1 || c.max_jobs > 65535) return serve::report(SCL_ERR_ARG, "max_jobs must be in 1..65535");
     if (c.max_pending < 1) return serve::report(SCL_ERR_ARG, "max_pending must be >= 1");
-    if (c.round_steps < 1 || c.round_steps > 1024) return serve::report(SCL_ERR_ARG, "round_steps must be in 1..1024");
+    if (c.round_steps < 1 || c.round_steps > kMaxRoundSteps)
+        return serve::report(SCL_ERR_ARG, "round_steps must be in 1..1024");
     if (!(c.mass_tol >= 0)) return serve::report(SCL_ERR_ARG, "mass_tol must be >= 0 (0: by precision)");
     if (c.max_bypass < 0) return serve::report(SCL_ERR_ARG, "max_bypass must be >= 0");
     int rc = runtime_ready();

@@ -213,11 +213,17 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
 
     int t = threadIdx.x;   // (fresh_lane)
     const int x0 = bx * OUT;
-    int X = x0 - H + t;   // one wrap at most (nx >= T)
-    X = X < 0 ? X + nx : (X >= nx ? X - nx : X);
-    const int Xm = wrap1(X - 1, nx), Xp = wrap1(X + 1, nx);
     const int nout = min(OUT, nx - x0);
     const bool out_lane = t >= H && t < H + nout;
+    // Lanes the strip needs: its outputs and H halo columns on each side.
+    // The lanes past them (a few in every FP32 strip, whole waves in a
+    // lattice's narrow last strip) collide nothing: every stage's lane bounds
+    // below end at tl instead of T.  They load the last needed lane's column,
+    // so their loads fall in that lane's cache lines and cost no traffic.
+    const int tl = min(T, nout + 2 * H);
+    int X = x0 - H + min(t, tl - 1);   // one wrap at most (nx >= T)
+    X = X < 0 ? X + nx : (X >= nx ? X - nx : X);
+    const int Xm = wrap1(X - 1, nx), Xp = wrap1(X + 1, nx);
     auto wr = [&](int r) {
         r %= ny;
         return r < 0 ? r + ny : r;
@@ -363,7 +369,7 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
             for (int s = 2; s <= K; ++s) {
                 const int q = s - 2;
                 const int g = r - 3 * (s - 1) + 1;
-                if (g >= y0 - 2 * (K - s) - 1 && g <= y1 + 2 * (K - s) && t >= 2 * s - 2 && t < T - 2 * s + 2) {
+                if (g >= y0 - 2 * (K - s) - 1 && g <= y1 + 2 * (K - s) && t >= 2 * s - 2 && t < tl - 2 * s + 2) {
                     const unsigned nbg = nbw[3 * s - 2];
                     if (OPEN || (nbg & 1u)) {
 #pragma unroll
@@ -388,7 +394,7 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
             }
             __syncthreads();
             // (B) stage 1: collide row r into ring 0
-            if (r <= r1_last && t >= 1 && t < T - 1 && (OPEN || (nbw[2] & 1u))) {
+            if (r <= r1_last && t >= 1 && t < tl - 1 && (OPEN || (nbw[2] & 1u))) {
                 float nr[2][Q], rc[2], oa[Q], ob[Q];
                 nbr_dens(an, 0, r, nr, rc);
                 collide2_core<float, UeqStage, FORCE, ST>(a, UeqStage{A.u, 0}, wr(r) * a.pitch + X, pa, rc, nr,
@@ -409,7 +415,7 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
                 const int h = r - 3 * (s - 1);
                 const unsigned nbh = nbw[3 * s - 1];
                 if (s < K) {
-                    if (h >= y0 - 2 * (K - s) && h <= y1 + 2 * (K - s) - 1 && t >= 2 * s - 1 && t < T - 2 * s + 1 &&
+                    if (h >= y0 - 2 * (K - s) && h <= y1 + 2 * (K - s) - 1 && t >= 2 * s - 1 && t < tl - 2 * s + 1 &&
                         (OPEN || (nbh & 1u))) {
                         float nr[2][Q], rc[2], oa[Q], ob[Q];
                         nbr_dens(an, s - 1, h, nr, rc);
@@ -514,7 +520,7 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
             for (int c = 0; c < 2; ++c) L.d[0][r3(an, r + 1)][c][t] = dens(pb[c]);
             __syncthreads();
             // stage 1: collide row r into ring 0
-            if (t >= 1 && t < T - 1 && (OPEN || (nbw[2] & 1u))) {
+            if (t >= 1 && t < tl - 1 && (OPEN || (nbw[2] & 1u))) {
                 float nr[2][Q], rc[2], oa[Q], ob[Q];
                 nbr_dens(an, 0, r, nr, rc);
                 collide2_core<float, UeqStage, FORCE, ST>(a, UeqStage{A.u, 0}, wr(r) * a.pitch + X, pa, rc, nr, OPEN ? 0x1FFu : nbw[2], oa, ob);
@@ -534,7 +540,7 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
                 const int q = s - 2;   // register / ring index of this stage's inputs
                 // gather of stage s at row g from ring q (stage s-1)
                 const int g = r - 2 * s + 3;
-                if (g >= y0 - 2 * (K - s) - 1 && t >= 2 * s - 2 && t < T - 2 * s + 2) {
+                if (g >= y0 - 2 * (K - s) - 1 && t >= 2 * s - 2 && t < tl - 2 * s + 2) {
                     const unsigned nbg = nbw[2 * s - 1];
                     if (OPEN || (nbg & 1u)) {
 #pragma unroll
@@ -561,7 +567,7 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
                 const int h = r - 2 * (s - 1);
                 const unsigned nbh = nbw[2 * s];
                 if (s < K) {
-                    if (h >= y0 - 2 * (K - s) && t >= 2 * s - 1 && t < T - 2 * s + 1 && (OPEN || (nbh & 1u))) {
+                    if (h >= y0 - 2 * (K - s) && t >= 2 * s - 1 && t < tl - 2 * s + 1 && (OPEN || (nbh & 1u))) {
                         float nr[2][Q], rc[2], oa[Q], ob[Q];
                         nbr_dens(an, s - 1, h, nr, rc);
                         collide2_core<float, UeqStage, FORCE, ST>(a, UeqStage{A.u, s - 1}, wr(h) * a.pitch + X, sx[q], rc, nr, OPEN ? 0x1FFu : nbh, oa, ob);

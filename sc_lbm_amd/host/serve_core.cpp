@@ -617,6 +617,7 @@ int Scheduler::stats(scl_server_stats* st) {
     st->round_steps = cfg_.round_steps;
     st->workgroups = be_->workgroups();
     st->steps_per_launch = be_->steps_per_launch();
+    st->pair_updates = be_->pair_updates();
     return SCL_OK;
 }
 
@@ -886,6 +887,7 @@ int scl_server_default_config(scl_server_config* cfg) {
     cfg->round_steps = 32;
     cfg->mass_tol = 0.0;   // by precision
     cfg->max_bypass = 16;
+    cfg->use_graphs = 1;
     return SCL_OK;
 }
 

@@ -166,6 +166,8 @@ class Backend {
     virtual int steps_per_launch() const { return 1; }
     // Device time of the rounds so far, in milliseconds (0 if not measured).
     virtual double device_ms() const { return 0.0; }
+    // Lattice updates made by two-step launches so far.
+    virtual long long pair_updates() const { return 0; }
 };
 
 // ---------------------------------------------------------------------------

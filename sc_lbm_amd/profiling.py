@@ -7,7 +7,12 @@
         two per launch) and prints the mean kernel time per launch;
     python -m sc_lbm_amd.profiling copy --bytes 2147483648 --reps 10
         runs the plain copy kernel k_probe_copy (reads and writes `bytes`
-        each launch) so the HBM byte counters can be calibrated.
+        each launch) so the HBM byte counters can be calibrated;
+    python -m sc_lbm_amd.profiling serve --jobs 64 --step-scale 0.025 [--form fp32dev]
+        runs a batch of jobs of the served mix (submitted together) through a
+        server that issues its launches one by one instead of from captured
+        graphs (profilers trace graph launches poorly) and prints the
+        lattice updates per second of the batch.
 """
 import argparse
 import json
@@ -37,10 +42,36 @@ def main(argv=None):
     c = sub.add_parser("copy")
     c.add_argument("--bytes", type=float, default=2 ** 31)
     c.add_argument("--reps", type=int, default=10)
+    v = sub.add_parser("serve")
+    v.add_argument("--form", default="fp32dev")
+    v.add_argument("--jobs", type=int, default=64)
+    v.add_argument("--seed", type=int, default=77)
+    v.add_argument("--step-scale", type=float, default=0.025)
+    v.add_argument("--graphs", type=int, default=0)
     a = p.parse_args(argv)
 
     from sc_lbm_amd import native as nat
     nat.init(0)
+    if a.cmd == "serve":
+        from sc_lbm_amd import serve
+        from sc_lbm_amd import workloads as wl
+        jobs = wl.served_stream(a.jobs, seed=a.seed, step_scale=a.step_scale)
+        with serve.Server(precision=a.form, budget_bytes=64 << 30, max_jobs=256, use_graphs=a.graphs) as srv:
+            st0 = srv.stats()
+            ids = [srv.submit(**j) for j in jobs]
+            sts = [srv.wait(i) for i in ids]
+            st1 = srv.stats()
+        lu = st1["lattice_updates"] - st0["lattice_updates"]
+        ms = st1["clock_ms"] - st0["clock_ms"]
+        print(json.dumps({"form": a.form, "jobs": a.jobs, "step_scale": a.step_scale, "graphs": a.graphs,
+                          "done": sum(x["state"] == "done" for x in sts), "lattice_updates": lu, "ms": ms,
+                          "mlups": lu / ms / 1e3, "device_mlups": lu / st1["device_ms"] / 1e3,
+                          "tile": [st1["tile_cols"], st1["tile_rows"], st1["lookahead"]],
+                          "steps_per_launch": st1["steps_per_launch"], "rounds": st1["rounds"],
+                          "pair_updates": st1["pair_updates"] - st0["pair_updates"],
+                          "sites": int(sum(j["nx"] * j["ny"] for j in jobs)),
+                          "device_ms": st1["device_ms"] - st0["device_ms"]}))
+        return
     if a.cmd == "copy":
         ms = nat.probe_copy(int(a.bytes), a.reps)
         print(json.dumps({"kernel": "k_probe_copy", "bytes_read": int(a.bytes) // 16 * 16,

@@ -60,6 +60,7 @@ class Config(ctypes.Structure):
         ("max_pending", ctypes.c_int), ("round_steps", ctypes.c_int), ("tile_cols", ctypes.c_int),
         ("tile_rows", ctypes.c_int), ("lookahead", ctypes.c_int), ("mass_tol", ctypes.c_double),
         ("max_bypass", ctypes.c_int), ("steps_per_launch", ctypes.c_int), ("band_rows", ctypes.c_int),
+        ("use_graphs", ctypes.c_int),
     ]
 
 
@@ -75,6 +76,7 @@ class Stats(ctypes.Structure):
         ("arena_peak", ctypes.c_size_t), ("tile_cols", ctypes.c_int), ("tile_rows", ctypes.c_int),
         ("lookahead", ctypes.c_int), ("round_steps", ctypes.c_int), ("workgroups", ctypes.c_int),
         ("steps_per_launch", ctypes.c_int), ("fluid_updates", ctypes.c_longlong),
+        ("pair_updates", ctypes.c_longlong),
     ]
 
     def as_dict(self):

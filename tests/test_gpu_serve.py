@@ -226,10 +226,10 @@ def test_autotuned_server_runs_and_reports_its_shape():
 
 @pytest.mark.parametrize("form", ["fp32dev", "fp16dev"])
 def test_two_steps_per_launch_in_the_server_keep_results(form):
-    """Deviation forms: jobs at least 256 columns wide advance two steps per
-    launch of the persistent kernel; an odd step count, a round shorter than
-    two steps and narrow jobs take single steps.  The result is the same bits
-    with one step per launch, in a mix, and on the single-lattice path."""
+    """Deviation forms: jobs advance two steps per launch of the persistent
+    kernel; an odd step count and a round shorter than two steps take single
+    steps.  The result is the same bits with one step per launch, in a mix,
+    and on the single-lattice path."""
     wide = dict(POROUS, nx=300, ny=120, steps=151, geom_seed=4, ic_seed=6, deliver=serve.DELIVER_F)
     two = run_alone(wide, form, round_steps=16, steps_per_launch=2)
     one = run_alone(wide, form, round_steps=16, steps_per_launch=1)
@@ -260,6 +260,32 @@ def test_two_steps_per_launch_in_the_server_keep_results(form):
         fl[c].set_f(case["f"][c])
     nat.solid_surface_force(g, fl[0], fl[1])
     nat.step2(g, fl[0], fl[1], case["force"][0], case["force"][1], wide["steps"])
+    for c in range(2):
+        assert np.array_equal(fl[c].get_f().reshape(-1), two[c].reshape(-1))
+
+
+@pytest.mark.parametrize("form", ["fp32dev", "fp16dev"])
+@pytest.mark.parametrize("nx", [8, 37, 128, 224, 225, 255])
+def test_narrow_jobs_in_two_step_strips(form, nx):
+    """A job narrower than a two-step strip (224 output columns in FP32, 250
+    in FP16) takes one strip, or a full one and a narrow one, whose lanes
+    past the job's columns and halos idle: the same bits as one step per
+    launch and as the single-lattice path's single steps."""
+    job = dict(POROUS, nx=nx, ny=96, steps=41, rmin=2, rmax=5, radius=min(nx, 96) / 4.0, geom_seed=5,
+               ic_seed=9, deliver=serve.DELIVER_F)
+    two = run_alone(job, form, round_steps=16, steps_per_launch=2)
+    one = run_alone(job, form, round_steps=16, steps_per_launch=1)
+    for c in range(2):
+        assert np.array_equal(two[c], one[c])
+    case = wl.served_case(**{k: v for k, v in job.items() if k not in ("steps", "deliver")})
+    g = nat.Geometry(case["mask"])
+    g.set_step_fusion(1)
+    fl = [nat.Field(g, case["tau"][c], nat.PRECISION[form], served_rho0(job), g_solid=case["g_solid"][c],
+                    g_fluid=case["g_fluid"][c]) for c in range(2)]
+    for c in range(2):
+        fl[c].set_f(case["f"][c])
+    nat.solid_surface_force(g, fl[0], fl[1])
+    nat.step2(g, fl[0], fl[1], case["force"][0], case["force"][1], job["steps"])
     for c in range(2):
         assert np.array_equal(fl[c].get_f().reshape(-1), two[c].reshape(-1))
 
@@ -316,3 +342,13 @@ def test_suspended_job_resumes_bit_for_bit(form):
         got = [srv.result(jid, serve.F, c) for c in range(2)]
     for c in range(2):
         assert np.array_equal(got[c], alone[c])
+
+
+def test_launches_without_graphs_give_the_same_bits():
+    """use_graphs = 0 (the launches of each round issued one by one, for
+    profilers) runs the same kernels in the same order: same populations."""
+    job = dict(POROUS, nx=300, steps=77, deliver=serve.DELIVER_F)   # wide: two-step tiles too
+    want = run_alone(job, "fp32dev", round_steps=16)
+    got = run_alone(job, "fp32dev", round_steps=16, use_graphs=0)
+    for c in range(2):
+        assert np.array_equal(got[c], want[c])
