@@ -162,7 +162,7 @@ def served_stream(nat, a):
             if (k + 1) % 100 == 0:
                 log("  %d jobs finished" % (k + 1))
         rho32 = {k: [srv.result(ids[k], serve.RHO, c) for c in range(2)]
-                 for k in sample if sts[k]["state"] == "done"}
+                 for k in sorted(sample) if sts[k]["state"] == "done"}
         st2 = srv.stats()
     done = [k for k, s in enumerate(sts) if s["state"] == "done"]
     lat = np.array([sts[k]["done_ms"] - sts[k]["release_ms"] for k in done])
@@ -170,7 +170,7 @@ def served_stream(nat, a):
     last = max(sts[k]["done_ms"] for k in done)
     lu_done = float(lus[done].sum())
     # FP64 rerun of the sample
-    errs, worst = [], []
+    errs, worst, errs_by_job = [], [], []
     with serve.Server(precision=nat.FP64, budget_bytes=int(a.serve_budget_gib * (1 << 30)), max_jobs=256) as s64:
         ids64 = {k: s64.submit(deliver=serve.DELIVER_RHO, **jobs[k]) for k in rho32}
         for k, i in ids64.items():
@@ -179,6 +179,7 @@ def served_stream(nat, a):
                 continue
             e = max(float(np.max(np.abs(s64.result(i, serve.RHO, c) - rho32[k][c]))) for c in range(2))
             errs.append(e)
+            errs_by_job.append([first_job + k, e])
             j = jobs[k]
             worst.append(dict(err=e, job=first_job + k, nx=j["nx"], ny=j["ny"], steps=j["steps"], tau=round(j["tau"][0], 4),
                               G=round(j["g_fluid"][0], 4), porosity=round(j["porosity"], 4),
@@ -195,7 +196,7 @@ def served_stream(nat, a):
              "configuration, whose full run is recorded in profiles/served_stream_full.md" % a.serve_step_scale)
     extra = {}
     if a.serve_part:
-        extra = dict(part=a.serve_part, first_job=first_job, sample_errs=errs,
+        extra = dict(part=a.serve_part, first_job=first_job, sample_errs=sorted(errs_by_job),
                      latency_ms=[round(float(s["done_ms"] - s["release_ms"]), 1) if s["state"] == "done" else None
                                  for s in sts],
                      lattice_updates=lu_done, first_release_ms=first, last_done_ms=last)

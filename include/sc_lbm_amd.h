@@ -417,6 +417,31 @@ int scl_server_destroy(scl_server* s);
 /* Host helper (no device needed): the mask a job would use, nx*ny ints. */
 int scl_job_mask(const scl_job* job, int* mask, size_t n);
 
+/* The geometry pipeline on the device.  Bit masks are row-major, ny rows of
+ * ceil(nx/64) 64-bit words, bit x & 63 of word x >> 6 set where site x is
+ * fluid (bits past nx zero): the layout the kernels read.
+ *
+ * scl_mask_raster replaces Geometry::add_walls followed by add_circle calls
+ * (src/geometry.cpp:34-67, 132-175; the node sets of
+ * src/geom_object/ellipse.cpp:12-33) on an all-fluid nx x ny lattice:
+ * walls_along 0 none, 1 the reference's "x" (wall_thickness rows at the
+ * bottom and the top), 2 its "y" (columns at the left and the right); then
+ * n_circles triples (D, xc, yc).  Walls and circles are checked on the host
+ * as the reference checks them (SCL_ERR_ARG with its message where it
+ * throws; circles of diameter above 65535: SCL_ERR_RANGE), rasterised on the
+ * device, copied to bits (n_words = ny * ceil(nx/64)) and their fluid sites
+ * counted there.  device_ms (may be NULL): the device time of the raster and
+ * count.  The circles may overlap. */
+int scl_mask_raster(int nx, int ny, long long wall_thickness, int walls_along, const long long* circles,
+                    int n_circles, uint64_t* bits, size_t n_words, long long* fluid_sites, double* device_ms);
+/* A served job's mask through the server's own pipeline: the seeded disk
+ * sampler on the host (centres and sizes only), the rasterising, packing and
+ * fluid count on the device; SCL_ERR_HIP if the device count differs from
+ * the sampler's.  The same bits as scl_job_mask packed.  A job with its own
+ * mask gets that mask packed (no device work).  ms (may be NULL): [0] the
+ * sampler's host time, [1] the device time. */
+int scl_job_mask_bits(const scl_job* job, uint64_t* bits, size_t n_words, long long* fluid_sites, double* ms);
+
 #ifdef __cplusplus
 }
 #endif

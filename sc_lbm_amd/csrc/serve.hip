@@ -22,6 +22,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -31,6 +32,7 @@
 
 #include "../host/serve_core.h"
 #include "convert.h"
+#include "geom.h"
 #include "runtime.h"
 #include "serve.h"
 #include "step.h"
@@ -121,10 +123,12 @@ class GpuServe final : public serve::Backend {
 
   public:
     explicit GpuServe(const scl_server_config& cfg) : cfg_(cfg) {}
+    bool device_masks() const override { return true; }
     ~GpuServe() override {
         (void)hipSetDevice(dev_);
         if (stream_) (void)hipStreamSynchronize(stream_);
         for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
+        geo_.release();
         for (void* p : {(void*)arena_, (void*)hdr_d_, (void*)ctr_d_, table_d_, (void*)items_d_, (void*)partial_d_,
                         (void*)mass_d_, (void*)items2_d_, (void*)flags_d_, (void*)sched_d_})
             if (p) (void)hipFree(p);
@@ -506,17 +510,25 @@ class GpuServe final : public serve::Backend {
         g->ny = s.ny;
         const Layout& L = g->L;
         if (!j.prepared) {   // jobs built outside the scheduler (the autotuner's)
-            std::vector<int> m;
-            serve::job_mask(s, &m);
-            serve::pack_mask(m, s.nx, s.ny, &j.bits);
+            if (s.mask.empty()) {
+                j.fluid_sites = serve::job_disks(s, &j.disks);
+                j.device_mask = true;
+            } else {
+                serve::pack_mask(s.mask, s.nx, s.ny, &j.bits);
+            }
             j.prepared = true;
         }
-        if ((int)(j.bits.size() / s.ny) != L.wpr || j.bits.size() != (size_t)s.ny * L.wpr) {
-            *err = "packed mask does not match the lattice";
-            return SCL_ERR_ARG;
-        }
         SHIP(hipMemsetAsync(base, 0, L.total, stream_));
-        SHIP(hipMemcpyAsync(base + L.mask, j.bits.data(), j.bits.size() * 8, hipMemcpyHostToDevice, stream_));
+        if (j.device_mask) {
+            const int rc = raster_job(j, (uint64_t*)(base + L.mask), err);
+            if (rc != SCL_OK) return rc;
+        } else {
+            if ((int)(j.bits.size() / s.ny) != L.wpr || j.bits.size() != (size_t)s.ny * L.wpr) {
+                *err = "packed mask does not match the lattice";
+                return SCL_ERR_ARG;
+            }
+            SHIP(hipMemcpyAsync(base + L.mask, j.bits.data(), j.bits.size() * 8, hipMemcpyHostToDevice, stream_));
+        }
         if (!j.saved.empty()) {   // a resumed job: its saved state (save) into buffer 0 and set 0
             if (j.saved.size() != saved_bytes(L)) {
                 *err = "saved state does not match the lattice";
@@ -564,6 +576,28 @@ class GpuServe final : public serve::Backend {
         SHIP(hipStreamSynchronize(stream_));   // host vectors above must outlive the copies
         g->cur = 1;
         j.dev = g.release();
+        return SCL_OK;
+    }
+
+    // A generated mask on the device: the sampler's disks (walls: rows 0 and
+    // ny - 1) rasterised into `bits`, its fluid sites counted and checked
+    // against the sampler's count.
+    int raster_job(Job& j, uint64_t* bits, std::string* err) {
+        const JobSpec& s = j.spec;
+        static_assert(sizeof(serve::Disk) == sizeof(RasterDisk), "the sampler's disks are the raster's circles");
+        SHIP(geo_.upload(reinterpret_cast<const RasterDisk*>(j.disks.data()), j.disks.size(), stream_));
+        SHIP(raster_mask(stream_, bits, s.nx, s.ny, s.walls ? 1 : 0, 1, geo_.disks(), geo_.rows(),
+                         (int)j.disks.size(), geo_.n_rows, geo_.count()));
+        unsigned long long n = 0;
+        SHIP(hipMemcpyAsync(&n, geo_.count(), sizeof n, hipMemcpyDeviceToHost, stream_));
+        SHIP(hipStreamSynchronize(stream_));
+        if ((long long)n != j.fluid_sites) {
+            *err = "device mask has " + std::to_string(n) + " fluid sites, the sampler counted " +
+                   std::to_string(j.fluid_sites);
+            return SCL_ERR_HIP;
+        }
+        j.disks.clear();
+        j.disks.shrink_to_fit();
         return SCL_OK;
     }
 
@@ -809,6 +843,7 @@ class GpuServe final : public serve::Backend {
     ServeHdr* hdr_h_ = nullptr;
     int* sched_d_ = nullptr;   // k_serve's item ranges per step of the round (ServeHdr::sched)
     int* sched_h_ = nullptr;
+    GeoStage geo_;             // disks of the mask being rasterised
     unsigned* ctr_d_ = nullptr;
     void* table_d_ = nullptr;
     void* table_h_ = nullptr;
@@ -869,5 +904,130 @@ extern "C" int scl_server_create(const scl_server_config* cfg, scl_server** out)
     auto s = std::unique_ptr<scl_server>(new scl_server);
     s->sched.reset(new serve::Scheduler(std::move(be), c));
     *out = s.release();
+    return SCL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// The geometry pipeline on the device, as calls of its own
+// ---------------------------------------------------------------------------
+namespace {
+
+// Rasterises walls and circles into an ny x ceil(nx/64) bit mask on the
+// library's device, copies it to `bits` and counts its fluid sites.
+int raster_to_host(int nx, int ny, int dh, int where_x, const std::vector<RasterDisk>& disks, uint64_t* bits,
+                   size_t n_words, long long* fluid, double* dev_ms, std::string* err) {
+    const int wpr = (nx + 63) / 64;
+    if (n_words != (size_t)ny * wpr) {
+        *err = "bits must hold ny * ceil(nx / 64) = " + std::to_string((size_t)ny * wpr) + " words";
+        return SCL_ERR_ARG;
+    }
+    int rc = runtime_ready();
+    if (rc != SCL_OK) return rc;
+    SHIP(hipSetDevice(runtime_device()));
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    uint64_t* bits_d = nullptr;
+    GeoStage geo;
+    unsigned long long n = 0;
+    float ms = 0.f;
+    hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);   // default priority
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipMalloc(&bits_d, n_words * sizeof(uint64_t));
+    if (e == hipSuccess) e = geo.upload(disks.data(), disks.size(), st);
+    if (e == hipSuccess) e = hipEventRecord(e0, st);
+    if (e == hipSuccess)
+        e = raster_mask(st, bits_d, nx, ny, dh, where_x, geo.disks(), geo.rows(), (int)disks.size(), geo.n_rows,
+                        geo.count());
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(bits, bits_d, n_words * sizeof(uint64_t), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(&n, geo.count(), sizeof n, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (st) (void)hipStreamSynchronize(st);
+    geo.release();
+    if (bits_d) (void)hipFree(bits_d);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (e != hipSuccess) {
+        *err = std::string("device geometry failed: ") + hip_msg(e);
+        return SCL_ERR_HIP;
+    }
+    if (fluid) *fluid = (long long)n;
+    if (dev_ms) *dev_ms = ms;
+    return SCL_OK;
+}
+
+}  // namespace
+
+extern "C" int scl_mask_raster(int nx, int ny, long long wall_thickness, int walls_along, const long long* circles,
+                               int n_circles, uint64_t* bits, size_t n_words, long long* fluid_sites,
+                               double* device_ms) {
+    if (nx < 1 || ny < 1 || n_circles < 0 || (n_circles > 0 && !circles) || !bits)
+        return serve::report(SCL_ERR_ARG, "bad lattice size, circle list or output");
+    if (walls_along < 0 || walls_along > 2) return serve::report(SCL_ERR_ARG, "walls_along is 0, 1 (x) or 2 (y)");
+    if (wall_thickness < 0) return serve::report(SCL_ERR_ARG, "negative wall thickness");
+    // Geometry::add_walls (src/geometry.cpp:34-67)
+    if (walls_along == 2 && wall_thickness > nx)
+        return serve::report(SCL_ERR_ARG, "Wall thickness larger than domain width (x)");
+    if (walls_along == 1 && wall_thickness > ny)
+        return serve::report(SCL_ERR_ARG, "Wall thickness larger than domain height (y)");
+    std::vector<RasterDisk> disks((size_t)n_circles);
+    for (int k = 0; k < n_circles; ++k) {
+        const long long D = circles[3 * k], xc = circles[3 * k + 1], yc = circles[3 * k + 2];
+        if (D < 1 || xc < 0 || yc < 0) return serve::report(SCL_ERR_ARG, "circle " + std::to_string(k) + ": bad size or centre");
+        // Geometry::check_object_bounds (src/geometry.cpp:150-162)
+        if (xc - D / 2 < 0 || yc - D / 2 < 0)
+            return serve::report(SCL_ERR_ARG, "Circle lower bounds do not fit the domain.");
+        if (xc + D / 2 >= nx || yc + D / 2 >= ny)
+            return serve::report(SCL_ERR_ARG, "Circle upper bounds do not fit the domain.");
+        const long long L = D % 2 == 0 ? D - 1 : D;
+        if (L > kMaxRasterL) return serve::report(SCL_ERR_RANGE, "circle diameter above 65535");
+        disks[(size_t)k] = RasterDisk{(int)xc, (int)yc, (int)L};
+    }
+    std::string err;
+    const int rc = raster_to_host(nx, ny, walls_along ? (int)wall_thickness : 0, walls_along == 1, disks, bits, n_words,
+                                  fluid_sites, device_ms, &err);
+    return rc == SCL_OK ? rc : serve::report(rc, err);
+}
+
+extern "C" int scl_job_mask_bits(const scl_job* job, uint64_t* bits, size_t n_words, long long* fluid_sites,
+                                 double* ms) {
+    if (!job || !bits) return serve::report(SCL_ERR_ARG, "null job or output");
+    JobSpec spec;
+    std::string err;
+    int rc = serve::spec_from_job(job, &spec, &err);
+    if (rc != SCL_OK) return serve::report(rc, err);
+    const int wpr = (spec.nx + 63) / 64;
+    if (n_words != (size_t)spec.ny * wpr) return serve::report(SCL_ERR_ARG, "bits must hold ny * ceil(nx / 64) words");
+    if (!spec.mask.empty()) {   // the caller's mask, packed
+        std::vector<uint64_t> b;
+        serve::pack_mask(spec.mask, spec.nx, spec.ny, &b);
+        std::memcpy(bits, b.data(), n_words * sizeof(uint64_t));
+        long long nf = 0;
+        for (uint64_t w : b) nf += __builtin_popcountll(w);
+        if (fluid_sites) *fluid_sites = nf;
+        if (ms) ms[0] = ms[1] = 0.0;
+        return SCL_OK;
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<serve::Disk> d;
+    const long long expect = serve::job_disks(spec, &d);
+    const double sample_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    std::vector<RasterDisk> disks(d.size());
+    for (size_t k = 0; k < d.size(); ++k) disks[k] = RasterDisk{d[k].xc, d[k].yc, d[k].L};
+    long long nf = 0;
+    double dev = 0.0;
+    rc = raster_to_host(spec.nx, spec.ny, spec.walls ? 1 : 0, 1, disks, bits, n_words, &nf, &dev, &err);
+    if (rc != SCL_OK) return serve::report(rc, err);
+    if (nf != expect)
+        return serve::report(SCL_ERR_HIP, "device mask has " + std::to_string(nf) + " fluid sites, the sampler counted " +
+                                              std::to_string(expect));
+    if (fluid_sites) *fluid_sites = nf;
+    if (ms) {
+        ms[0] = sample_ms;
+        ms[1] = dev;
+    }
     return SCL_OK;
 }

@@ -3,10 +3,15 @@
 // Every device operator of the reference API is appended to a per-thread
 // list instead of running at once.  stream() closes a time step: when the
 // list holds exactly the reference's step loop (the sequences documented in
-// include/lbm.h) it becomes one fused launch (scl_step1 / scl_step2), which
+// include/lbm.h) it becomes a fused step (scl_step1 / scl_step2), which
 // reads and writes the populations once; otherwise each recorded operator
-// runs as its own kernel, in order.  Any host read of a Fluid array runs the
-// list first, so results never depend on the choice.
+// runs as its own kernel, in order.  Fused steps are not issued at once:
+// consecutive identical ones (same fluids, geometry and forces) are counted
+// and issued as one scl_step1/2(n) when anything else needs the device or a
+// host array (any other operator, a host read, sclbm_flush, a geometry or
+// fluid change), so the deviation forms reach their multi-step kernels.  Any
+// host read runs what is pending first, so results never depend on the
+// choice.
 #include <algorithm>
 #include <stdexcept>
 #include <string>
@@ -24,9 +29,24 @@ struct Op {
     std::vector<double> force;
 };
 
+// A run of identical fused steps not yet issued.
+struct Pending {
+    int n = 0;   // steps (0: none)
+    Fluid* a = nullptr;
+    Fluid* b = nullptr;   // null: one component
+    scl_geometry* g = nullptr;
+    bool has_fa = false, has_fb = false;
+    std::vector<double> fa, fb;
+    bool same(const Pending& o) const {
+        return a == o.a && b == o.b && g == o.g && has_fa == o.has_fa && has_fb == o.has_fb && fa == o.fa &&
+               fb == o.fb;
+    }
+};
+
 struct Engine {
     static thread_local std::vector<Op> log;
     static thread_local bool running;
+    static thread_local Pending pend;
 
     static void prepare(Fluid* f, const Geometry* g) {
         if (!f->dev) {
@@ -41,11 +61,14 @@ struct Engine {
 
     static void run_one(const Op& op);
     static bool try_fused();
+    static void defer(Pending c);
+    static void run_pending();
     static void flush();
 };
 
 thread_local std::vector<Op> Engine::log;
 thread_local bool Engine::running = false;
+thread_local Pending Engine::pend;
 
 void raise_status(int rc) {
     if (rc == SCL_OK) return;
@@ -102,9 +125,18 @@ bool Engine::try_fused() {
         if (k != log.size() - 1) return false;
         const Geometry* g = last.g;
         if (log[0].g != g) return false;
-        prepare(f, g);
-        raise_status(scl_step1(g->device(), f->dev, force ? force->data() : nullptr, 1));
-        f->device_wrote();
+        Pending c;
+        c.a = f;
+        c.g = g->device();
+        if (force) {
+            c.has_fa = true;
+            c.fa = *force;
+        }
+        if (!(pend.n > 0 && pend.same(c))) {
+            run_pending();
+            prepare(f, g);
+        }
+        defer(std::move(c));
         return true;
     }
     if (last.kind != OP_STREAM2) return false;
@@ -148,20 +180,59 @@ bool Engine::try_fused() {
         ++k;
     }
     if (k != log.size() - 1) return false;
-    prepare(A, g);
-    prepare(B, g);
-    const int rc = scl_step2(g->device(), A->dev, B->dev, fa ? fa->data() : nullptr, fb ? fb->data() : nullptr, 1);
-    if (rc == SCL_ERR_UNSUPPORTED) return false;
-    raise_status(rc);
-    A->device_wrote();
-    B->device_wrote();
+    Pending c;
+    c.a = A;
+    c.b = B;
+    c.g = g->device();
+    if (fa) {
+        c.has_fa = true;
+        c.fa = *fa;
+    }
+    if (fb) {
+        c.has_fb = true;
+        c.fb = *fb;
+    }
+    if (!(pend.n > 0 && pend.same(c))) {
+        run_pending();
+        prepare(A, g);
+        prepare(B, g);
+    }
+    defer(std::move(c));
     return true;
 }
 
+// Counts a fused step onto the pending run (the same step as the run's) or
+// issues the run and starts a new one.  The fluids' host mirrors are stale
+// from here on, so a host read runs the steps first.
+void Engine::defer(Pending c) {
+    if (pend.n > 0 && pend.same(c)) {
+        ++pend.n;
+    } else {
+        run_pending();
+        pend = std::move(c);
+        pend.n = 1;
+    }
+    pend.a->device_wrote();
+    if (pend.b) pend.b->device_wrote();
+}
+
+void Engine::run_pending() {
+    if (pend.n == 0) return;
+    Pending p = std::move(pend);
+    pend = Pending();
+    const double* fa = p.has_fa ? p.fa.data() : nullptr;
+    const double* fb = p.has_fb ? p.fb.data() : nullptr;
+    const int rc = p.b ? scl_step2(p.g, p.a->dev, p.b->dev, fa, fb, p.n) : scl_step1(p.g, p.a->dev, fa, p.n);
+    raise_status(rc);
+    p.a->device_wrote();
+    if (p.b) p.b->device_wrote();
+}
+
 void Engine::flush() {
-    if (running || log.empty()) return;
+    if (running || (log.empty() && pend.n == 0)) return;
     running = true;
     try {
+        run_pending();
         std::vector<Op> ops;
         ops.swap(log);
         for (const Op& op : ops) run_one(op);
@@ -176,6 +247,15 @@ void flush_log() { Engine::flush(); }
 
 void forget(const Fluid* f) {
     if (Engine::running) return;
+    if (Engine::pend.n > 0 && (Engine::pend.a == f || Engine::pend.b == f)) {
+        try {
+            Engine::flush();
+        } catch (...) {
+            Engine::pend = Pending();
+            Engine::log.clear();
+        }
+        return;
+    }
     for (const Op& op : Engine::log)
         if (op.a == f || op.b == f) {
             try {

@@ -215,26 +215,27 @@ class Draws {
 };
 
 // add_circle node set (src/geometry.cpp:132-147 with
-// src/geom_object/ellipse.cpp:12-33): an even diameter shrinks by one; node
-// (ix, iy) of the box xc +- L/2, yc +- L/2 is solid when
+// src/geom_object/ellipse.cpp:12-33) of odd diameter L: node (ix, iy) of the
+// box xc +- L/2, yc +- L/2 is solid when
 // ((ix-xc)^2*L^2)/4 + ((iy-yc)^2*L^2)/4 <= L^4/16 in integer arithmetic.
-long long carve_disk(std::vector<int>& m, int nx, long long d, long long xc, long long yc) {
-    const long long L = d % 2 == 0 ? d - 1 : d;
-    const long long h = L / 2, L2 = L * L, rhs = L2 * L2 / 16;
-    long long carved = 0;
-    for (long long y = yc - h; y <= yc + h; ++y)
-        for (long long x = xc - h; x <= xc + h; ++x) {
-            const long long lhs = ((x - xc) * (x - xc) * L2) / 4 + ((y - yc) * (y - yc) * L2) / 4;
-            if (lhs <= rhs) {
-                int& v = m[(size_t)y * nx + (size_t)x];
-                carved += v;
-                v = 0;
-            }
+void carve_disk(std::vector<int>& m, int nx, const Disk& d) {
+    const long long L = d.L, h = L / 2, L2 = L * L, rhs = L2 * L2 / 16;
+    for (long long y = d.yc - h; y <= d.yc + h; ++y)
+        for (long long x = d.xc - h; x <= d.xc + h; ++x) {
+            const long long lhs = ((x - d.xc) * (x - d.xc) * L2) / 4 + ((y - d.yc) * (y - d.yc) * L2) / 4;
+            if (lhs <= rhs) m[(size_t)y * nx + (size_t)x] = 0;
         }
-    return carved;
 }
 
 }  // namespace
+
+long long disk_nodes(long long L) {
+    const long long h = L / 2, L2 = L * L, rhs = L2 * L2 / 16;
+    long long n = 0;
+    for (long long dy = -h; dy <= h; ++dy)
+        for (long long dx = -h; dx <= h; ++dx) n += (dx * dx * L2) / 4 + (dy * dy * L2) / 4 <= rhs;
+    return n;
+}
 
 void job_mask(const JobSpec& s, std::vector<int>* out) {
     const int nx = s.nx, ny = s.ny;
@@ -242,17 +243,24 @@ void job_mask(const JobSpec& s, std::vector<int>* out) {
         *out = s.mask;
         return;
     }
+    std::vector<Disk> disks;
+    job_disks(s, &disks);
     std::vector<int>& m = *out;
     m.assign((size_t)nx * ny, 1);
-    long long fluid = (long long)nx * ny;
-    if (s.walls) {
+    if (s.walls)
         for (int x = 0; x < nx; ++x) {
             m[x] = 0;
             m[(size_t)(ny - 1) * nx + x] = 0;
         }
-        fluid -= 2LL * nx;
-    }
-    if (s.porosity >= 1.0) return;
+    for (const Disk& d : disks) carve_disk(m, nx, d);
+}
+
+long long job_disks(const JobSpec& s, std::vector<Disk>* out) {
+    const int nx = s.nx, ny = s.ny;
+    out->clear();
+    long long fluid = (long long)nx * ny;
+    if (s.walls) fluid -= 2LL * nx;
+    if (s.porosity >= 1.0) return fluid;
     const double total = (double)nx * (double)ny;
     const int y_lo = s.walls ? 1 : 0, y_hi = s.walls ? ny - 2 : ny - 1;
     // disks in a uniform grid of cells at least as wide as the largest
@@ -260,10 +268,11 @@ void job_mask(const JobSpec& s, std::vector<int>* out) {
     const int cell = 2 * s.rmax + 2;
     const int gx = nx / cell + 1, gy = ny / cell + 1;
     std::vector<std::vector<int>> grid((size_t)gx * gy);
-    struct Disk {
+    struct Placed {
         long long r, xc, yc;
     };
-    std::vector<Disk> disks;
+    std::vector<Placed> disks;
+    std::vector<long long> nodes(s.rmax + 1, -1);   // node count by radius
     Draws st(s.geom_seed);
     const int max_tries = 200000;
     for (int tries = 0; (double)fluid / total > s.porosity && tries < max_tries;) {
@@ -279,7 +288,7 @@ void job_mask(const JobSpec& s, std::vector<int>* out) {
         for (int j = std::max(0, cy - 1); j <= std::min(gy - 1, cy + 1) && ok; ++j)
             for (int i = std::max(0, cx - 1); i <= std::min(gx - 1, cx + 1) && ok; ++i)
                 for (int k : grid[(size_t)j * gx + i]) {
-                    const Disk& o = disks[k];
+                    const Placed& o = disks[k];
                     const long long dx = xc - o.xc, dy = yc - o.yc, rr = r + o.r + 1;
                     if (dx * dx + dy * dy <= rr * rr) {
                         ok = false;
@@ -287,10 +296,13 @@ void job_mask(const JobSpec& s, std::vector<int>* out) {
                     }
                 }
         if (!ok) continue;
-        fluid -= carve_disk(m, nx, 2 * r + 1, xc, yc);
+        if (nodes[r] < 0) nodes[r] = disk_nodes(2 * r + 1);
+        fluid -= nodes[r];
         grid[(size_t)cy * gx + cx].push_back((int)disks.size());
-        disks.push_back(Disk{r, xc, yc});
+        disks.push_back(Placed{r, xc, yc});
+        out->push_back(Disk{(int)xc, (int)yc, (int)(2 * r + 1)});
     }
+    return fluid;
 }
 
 void pack_mask(const std::vector<int>& m, int nx, int ny, std::vector<uint64_t>* bits) {
@@ -353,9 +365,11 @@ Scheduler::~Scheduler() {
     if (thr_.joinable()) thr_.join();
 }
 
-// Mask builders: host work that precedes admission (the seeded disk sampler
-// takes tens of milliseconds at 2048^2), done off the request loop so the
-// device keeps running rounds meanwhile.  Jobs are prepared in release order.
+// Mask builders: host work that precedes admission, done off the request
+// loop so the device keeps running rounds meanwhile: a caller's mask is
+// packed here; a generated one is only sampled (the disks' centres and
+// sizes, no rasterising), and the backend rasterises it on the device when
+// it builds the job.  Jobs are prepared in release order.
 void Scheduler::prep_loop() {
     std::unique_lock<std::mutex> lk(mu_);
     for (;;) {
@@ -376,16 +390,24 @@ void Scheduler::prep_loop() {
         g.walls = src.walls;
         g.geom_seed = src.geom_seed;
         lk.unlock();
-        std::vector<int> m;
-        job_mask(g, &m);
         std::vector<uint64_t> bits;
-        pack_mask(m, g.nx, g.ny, &bits);
+        std::vector<Disk> disks;
         long long nf = 0;
-        for (uint64_t w : bits) nf += __builtin_popcountll(w);
+        const bool device_mask = g.mask.empty() && be_->device_masks();
+        if (device_mask) {
+            nf = job_disks(g, &disks);
+        } else {
+            std::vector<int> m;
+            job_mask(g, &m);
+            pack_mask(m, g.nx, g.ny, &bits);
+            for (uint64_t w : bits) nf += __builtin_popcountll(w);
+        }
         lk.lock();
         it = jobs_.find(id);
         if (it != jobs_.end() && !finished(it->second->state)) {
             it->second->bits.swap(bits);
+            it->second->disks.swap(disks);
+            it->second->device_mask = device_mask;
             it->second->fluid_sites = nf;
             it->second->prepared = true;
             cv_work_.notify_one();

@@ -85,6 +85,21 @@ int spec_from_job(const scl_job* j, JobSpec* out, std::string* err);
 // rejection until the fluid fraction is at most `porosity`).
 void job_mask(const JobSpec& s, std::vector<int>* mask);
 
+// An add_circle call of the sampler: centre (xc, yc), odd diameter L.
+struct Disk {
+    int xc, yc, L;
+};
+
+// The generated geometry of a job (no caller mask) without rasterising it:
+// the sampler's disks in placement order, and the fluid sites of the mask
+// they and the walls leave (the disks never overlap each other or the walls,
+// so each removes exactly its node count).  job_mask is these disks carved
+// from the walled lattice.
+long long job_disks(const JobSpec& s, std::vector<Disk>* disks);
+
+// Nodes of an add_circle node set of odd diameter L.
+long long disk_nodes(long long L);
+
 // Bit-packed mask: row-major, ceil(nx/64) 64-bit words per row, bit x & 63
 // of word x >> 6 is site x (1 = fluid).
 void pack_mask(const std::vector<int>& m, int nx, int ny, std::vector<uint64_t>* bits);
@@ -116,9 +131,11 @@ struct Job {
     std::string error;
     std::string device_error;         // set by the backend's round: fails the job at the watchdog
     std::vector<double> rho[2], f[2]; // deliverables (reference layout)
-    std::vector<uint64_t> bits;       // packed mask (pack_mask), built before admission
+    std::vector<uint64_t> bits;       // packed mask (pack_mask) of a caller's mask, built before admission
+    std::vector<Disk> disks;          // or: the sampler's disks, rasterised by the backend (device_mask)
+    bool device_mask = false;
     bool prepared = false;
-    long long fluid_sites = 0;        // fluid sites of the mask (set with bits)
+    long long fluid_sites = 0;        // fluid sites of the mask (set with bits or disks)
     void* dev = nullptr;              // backend state
     long long sites() const { return (long long)spec.nx * spec.ny; }
 };
@@ -168,6 +185,8 @@ class Backend {
     virtual double device_ms() const { return 0.0; }
     // Lattice updates made by two-step launches so far.
     virtual long long pair_updates() const { return 0; }
+    // Whether setup() rasterises a generated mask from Job::disks itself.
+    virtual bool device_masks() const { return false; }
 };
 
 // ---------------------------------------------------------------------------

@@ -36,6 +36,8 @@ EXPORTS = [
     "scl_server_wait", "scl_server_cancel", "scl_server_suspend", "scl_server_resume", "scl_server_result",
     "scl_server_release", "scl_server_drain",
     "scl_server_get_stats", "scl_server_destroy", "scl_job_mask",
+    # the geometry pipeline on the device (bound in serve.py)
+    "scl_mask_raster", "scl_job_mask_bits",
 ]
 
 

@@ -12,7 +12,16 @@
         runs a batch of jobs of the served mix (submitted together) through a
         server that issues its launches one by one instead of from captured
         graphs (profilers trace graph launches poorly) and prints the
-        lattice updates per second of the batch.
+        lattice updates per second of the batch;
+    python -m sc_lbm_amd.profiling geometry --n 2048 --porosity 0.6 --reps 5
+        builds a served job's porous mask on the host (sampler, rasteriser,
+        packing) and through the device pipeline (host sampler, device
+        raster, pack and count) and prints the times of both;
+    python -m sc_lbm_amd.profiling noise --jobs 860,140 [--eps 6e-8]
+        for jobs of the served stream (2000 jobs, seed 2025, full step
+        counts) prints the FP32-deviation run's max density error against
+        FP64 and the FP64 noise floor: FP64 from the same populations
+        perturbed by a relative eps against FP64 from the unperturbed ones.
 """
 import argparse
 import json
@@ -48,6 +57,15 @@ def main(argv=None):
     v.add_argument("--seed", type=int, default=77)
     v.add_argument("--step-scale", type=float, default=0.025)
     v.add_argument("--graphs", type=int, default=0)
+    z = sub.add_parser("noise")
+    z.add_argument("--jobs", default="860")
+    z.add_argument("--eps", type=float, default=6e-8)
+    q = sub.add_parser("geometry")
+    q.add_argument("--n", type=int, default=2048)
+    q.add_argument("--porosity", type=float, default=0.6)
+    q.add_argument("--rmin", type=int, default=8)
+    q.add_argument("--rmax", type=int, default=20)
+    q.add_argument("--reps", type=int, default=5)
     a = p.parse_args(argv)
 
     from sc_lbm_amd import native as nat
@@ -71,6 +89,65 @@ def main(argv=None):
                           "pair_updates": st1["pair_updates"] - st0["pair_updates"],
                           "sites": int(sum(j["nx"] * j["ny"] for j in jobs)),
                           "device_ms": st1["device_ms"] - st0["device_ms"]}))
+        return
+    if a.cmd == "noise":
+        from sc_lbm_amd import serve
+        from sc_lbm_amd import workloads as wl
+        stream = wl.served_stream(2000, seed=2025, step_scale=1.0)
+        ks = [int(k) for k in a.jobs.split(",")]
+        runs = {}
+        for prec in (nat.FP64, nat.FP32_DEV):
+            with serve.Server(precision=prec, budget_bytes=64 << 30, max_jobs=256) as srv:
+                ids = {}
+                for k in ks:
+                    j = stream[k]
+                    ids[(k, "gen")] = srv.submit(deliver=serve.DELIVER_RHO, **j)
+                    if prec != nat.FP64:
+                        continue
+                    m = serve.job_mask(**j)
+                    ra, rb = wl.served_density(j["nx"], j["ny"], j["ic"], j["rho_hi"], j["rho_lo"], j["radius"],
+                                               j["noise"], j["ic_seed"])
+                    f0 = [wl.f_from_rho(ra, m), wl.f_from_rho(rb, m)]
+                    u = [wl.uniform01(1000 + 2 * k + c, f0[c].size) for c in range(2)]
+                    fp = [f0[c] * (1.0 + a.eps * (2.0 * u[c] - 1.0)) for c in range(2)]
+                    base = {kk: v for kk, v in j.items() if kk not in ("porosity", "geom_seed", "ic", "ic_seed")}
+                    ids[(k, "exp")] = srv.submit(deliver=serve.DELIVER_RHO, mask=m, f=f0, **base)
+                    ids[(k, "pert")] = srv.submit(deliver=serve.DELIVER_RHO, mask=m, f=fp, **base)
+                for key, i in ids.items():
+                    st = srv.wait(i)
+                    runs[(prec,) + key] = ([srv.result(i, serve.RHO, c) for c in range(2)]
+                                           if st["state"] == "done" else None)
+        for k in ks:
+            g64, e64, p64 = (runs[(nat.FP64, k, t)] for t in ("gen", "exp", "pert"))
+            g32 = runs[(nat.FP32_DEV, k, "gen")]
+            err = lambda x, y: (max(float(np.max(np.abs(x[c] - y[c]))) for c in range(2))
+                                if x is not None and y is not None else None)
+            j = stream[k]
+            print(json.dumps({"job": k, "nx": j["nx"], "ny": j["ny"], "steps": j["steps"], "eps": a.eps,
+                              "fp32_vs_fp64": err(g32, g64), "fp64_noise_floor": err(p64, e64),
+                              "explicit_equals_generated": bool(e64 is not None and g64 is not None and
+                                                                all(np.array_equal(e64[c], g64[c]) for c in range(2)))}),
+                  flush=True)
+        return
+    if a.cmd == "geometry":
+        import time
+        from sc_lbm_amd import serve
+        spec = dict(nx=a.n, ny=a.n, steps=1, porosity=a.porosity, rmin=a.rmin, rmax=a.rmax, walls=True)
+        host_ms, samp_ms, dev_ms, wall_ms = [], [], [], []
+        for k in range(a.reps + 1):
+            t = time.perf_counter()
+            m = serve.job_mask(geom_seed=k, **spec)
+            host_ms.append((time.perf_counter() - t) * 1e3)
+            t = time.perf_counter()
+            bits, fluid, (ts, td) = serve.job_mask_bits(geom_seed=k, **spec)
+            wall_ms.append((time.perf_counter() - t) * 1e3)
+            samp_ms.append(ts)
+            dev_ms.append(td)
+            assert fluid == int(m.sum())
+        med = lambda v: float(np.median(v[1:]))   # the first repetition warms up
+        print(json.dumps({"n": a.n, "porosity": a.porosity, "reps": a.reps, "fluid_fraction": fluid / a.n ** 2,
+                          "host_mask_ms": med(host_ms), "sampler_ms": med(samp_ms), "device_raster_ms": med(dev_ms),
+                          "device_call_ms": med(wall_ms)}))
         return
     if a.cmd == "copy":
         ms = nat.probe_copy(int(a.bytes), a.reps)

@@ -265,3 +265,53 @@ def job_mask(job=None, lib=None, **kw):
     out = np.empty(j.nx * j.ny, dtype=np.int32)
     _check(lib, lib.scl_job_mask(ctypes.byref(j), out.ctypes.data_as(_ip), out.size))
     return out.reshape(j.ny, j.nx)
+
+
+def _bind_geometry(lib):
+    u64p = ctypes.POINTER(ctypes.c_uint64)
+    llp = ctypes.POINTER(ctypes.c_longlong)
+    lib.scl_mask_raster.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_longlong, ctypes.c_int, llp, ctypes.c_int,
+                                    u64p, ctypes.c_size_t, llp, _dp]
+    lib.scl_job_mask_bits.argtypes = [ctypes.POINTER(Job), u64p, ctypes.c_size_t, llp, _dp]
+    for fn in (lib.scl_mask_raster, lib.scl_job_mask_bits):
+        fn.restype = ctypes.c_int
+    lib.scl_last_error.restype = ctypes.c_char_p
+    return lib
+
+
+def unpack_bits(bits, nx):
+    """A bit mask (ny rows of ceil(nx/64) uint64 words, bit x & 63 of word
+    x >> 6 = site x) as ny x nx ints."""
+    b = np.ascontiguousarray(bits, dtype=np.uint64)
+    by = b.view(np.uint8).reshape(b.shape[0], -1)
+    return np.unpackbits(by, axis=1, bitorder="little")[:, :nx].astype(np.int32)
+
+
+def job_mask_bits(job=None, **kw):
+    """A job's mask through the device pipeline (host sampler, device raster,
+    pack and count): (bits ny x ceil(nx/64) uint64, fluid sites, (sampler
+    ms, device ms))."""
+    lib = _bind_geometry(nat.load())
+    j = job if job is not None else make_job(**kw)
+    out = np.zeros((j.ny, (j.nx + 63) // 64), dtype=np.uint64)
+    nf = ctypes.c_longlong(0)
+    ms = (ctypes.c_double * 2)()
+    _check(lib, lib.scl_job_mask_bits(ctypes.byref(j), out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                      out.size, ctypes.byref(nf), ms))
+    return out, nf.value, (ms[0], ms[1])
+
+
+def mask_raster(nx, ny, circles=(), wall_thickness=0, walls_along=0):
+    """Geometry::add_walls + add_circle calls rasterised on the device:
+    (bits, fluid sites, device ms).  walls_along 0 none, 1 "x", 2 "y";
+    circles (D, xc, yc) triples."""
+    lib = _bind_geometry(nat.load())
+    c = np.ascontiguousarray(np.asarray(circles, dtype=np.int64).reshape(-1))
+    out = np.zeros((ny, (nx + 63) // 64), dtype=np.uint64)
+    nf = ctypes.c_longlong(0)
+    ms = ctypes.c_double(0.0)
+    _check(lib, lib.scl_mask_raster(int(nx), int(ny), int(wall_thickness), int(walls_along),
+                                    c.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), len(c) // 3,
+                                    out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), out.size,
+                                    ctypes.byref(nf), ctypes.byref(ms)))
+    return out, nf.value, ms.value

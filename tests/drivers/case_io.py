@@ -74,8 +74,40 @@ def build():
            "-Wl,-rpath,$ORIGIN/../../../sc_lbm_amd"]
     subprocess.check_call(cmd)
     build_dropin_host()
+    build_dropin_bench()
     build_serve_cpu()
     return BIN
+
+
+DROPIN_BENCH = os.path.join(HERE, "bin", "dropin_bench")
+
+
+def build_dropin_bench():
+    """The reference's phase-separation loop on the drop-in classes, timed."""
+    os.makedirs(os.path.dirname(DROPIN_BENCH), exist_ok=True)
+    lib = os.path.join(ROOT, "sc_lbm_amd")
+    cmd = ["g++", "-std=c++14", "-O2", "-Wall", "-I" + os.path.join(ROOT, "include"),
+           os.path.join(HERE, "dropin_bench.cpp"), "-o", DROPIN_BENCH, "-L" + lib, "-lsclbm",
+           "-Wl,-rpath,$ORIGIN/../../../sc_lbm_amd"]
+    subprocess.check_call(cmd)
+    return DROPIN_BENCH
+
+
+def run_dropin_bench(workdir, n, steps, precision, read_every, out=True):
+    """Runs tests/drivers/bin/dropin_bench; returns (its JSON line as a
+    dict, [f_A, f_B] or None)."""
+    import json
+    path = os.path.join(workdir, "dropin_bench.bin")
+    args = [DROPIN_BENCH, str(n), str(steps), str(precision), str(read_every)] + ([path] if out else [])
+    p = subprocess.run(args, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=600, cwd=workdir)
+    if p.returncode != 0:
+        raise RuntimeError("dropin_bench failed (%d): %s" % (p.returncode, p.stdout[-2000:]))
+    res = json.loads(p.stdout.strip().splitlines()[-1])
+    fs = None
+    if out:
+        raw = np.fromfile(path, dtype=np.float64)
+        fs = [raw[:raw.size // 2], raw[raw.size // 2:]]
+    return res, fs
 
 
 DROPIN_HOST = os.path.join(HERE, "bin", "dropin_host")

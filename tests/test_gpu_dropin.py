@@ -123,3 +123,24 @@ def test_text_restart_after_gpu_steps_rounds_oracle_state(tmp_path):
     for c in range(2):
         want = np.char.mod("%g", s.get_f(c)).astype(np.float64)
         assert np.array_equal(fs[c], want), "component %d" % c
+
+
+@pytest.mark.parametrize("precision", [0, 1, 2])
+def test_deferred_steps_equal_steps_issued_one_by_one(precision, tmp_path):
+    """The drop-in queues identical fused steps and issues them as one
+    scl_step2(n) at the next host read (tests/drivers/dropin_bench.cpp, the
+    reference's phase-separation loop): with no read until the end, a read
+    after every step (each step issued alone), a read every 5 steps and a
+    flush after every step, the populations are the same bits, in FP64 and
+    both deviation forms."""
+    if not os.path.exists(case_io.DROPIN_BENCH):
+        case_io.build_dropin_bench()
+    runs = {}
+    for every in (0, 1, 5, -1):
+        print("dropin_bench precision %d read_every %d" % (precision, every), flush=True)
+        res, fs = case_io.run_dropin_bench(str(tmp_path), 96, 23, precision, every)
+        runs[every] = fs
+    for every in (1, 5, -1):
+        for c in range(2):
+            assert np.array_equal(runs[0][c], runs[every][c]), (every, c)
+    assert np.all(np.isfinite(runs[0][0])) and np.all(np.isfinite(runs[0][1]))
