@@ -96,38 +96,42 @@ def main(argv=None):
         stream = wl.served_stream(2000, seed=2025, step_scale=1.0)
         ks = [int(k) for k in a.jobs.split(",")]
         runs = {}
-        for prec in (nat.FP64, nat.FP32_DEV):
-            with serve.Server(precision=prec, budget_bytes=64 << 30, max_jobs=256) as srv:
-                ids = {}
-                for k in ks:
-                    j = stream[k]
-                    ids[(k, "gen")] = srv.submit(deliver=serve.DELIVER_RHO, **j)
-                    if prec != nat.FP64:
-                        continue
-                    m = serve.job_mask(**j)
-                    ra, rb = wl.served_density(j["nx"], j["ny"], j["ic"], j["rho_hi"], j["rho_lo"], j["radius"],
-                                               j["noise"], j["ic_seed"])
-                    f0 = [wl.f_from_rho(ra, m), wl.f_from_rho(rb, m)]
-                    u = [wl.uniform01(1000 + 2 * k + c, f0[c].size) for c in range(2)]
-                    fp = [f0[c] * (1.0 + a.eps * (2.0 * u[c] - 1.0)) for c in range(2)]
-                    base = {kk: v for kk, v in j.items() if kk not in ("porosity", "geom_seed", "ic", "ic_seed")}
-                    ids[(k, "exp")] = srv.submit(deliver=serve.DELIVER_RHO, mask=m, f=f0, **base)
-                    ids[(k, "pert")] = srv.submit(deliver=serve.DELIVER_RHO, mask=m, f=fp, **base)
-                for key, i in ids.items():
-                    st = srv.wait(i)
-                    runs[(prec,) + key] = ([srv.result(i, serve.RHO, c) for c in range(2)]
-                                           if st["state"] == "done" else None)
-        for k in ks:
-            g64, e64, p64 = (runs[(nat.FP64, k, t)] for t in ("gen", "exp", "pert"))
-            g32 = runs[(nat.FP32_DEV, k, "gen")]
-            err = lambda x, y: (max(float(np.max(np.abs(x[c] - y[c]))) for c in range(2))
-                                if x is not None and y is not None else None)
-            j = stream[k]
-            print(json.dumps({"job": k, "nx": j["nx"], "ny": j["ny"], "steps": j["steps"], "eps": a.eps,
-                              "fp32_vs_fp64": err(g32, g64), "fp64_noise_floor": err(p64, e64),
-                              "explicit_equals_generated": bool(e64 is not None and g64 is not None and
-                                                                all(np.array_equal(e64[c], g64[c]) for c in range(2)))}),
-                  flush=True)
+
+        def collect(srv, ids, prec):
+            for key, i in ids.items():
+                st = srv.wait(i)
+                runs[(prec,) + key] = ([srv.result(i, serve.RHO, c) for c in range(2)]
+                                       if st["state"] == "done" else None)
+                srv.release(i)
+                print("precision %d job %d %s: %s" % (prec, key[0], key[1], st["state"]), file=sys.stderr,
+                      flush=True)
+
+        with serve.Server(precision=nat.FP32_DEV, budget_bytes=64 << 30, max_jobs=256) as srv:
+            collect(srv, {(k, "gen"): srv.submit(deliver=serve.DELIVER_RHO, **stream[k]) for k in ks}, nat.FP32_DEV)
+        with serve.Server(precision=nat.FP64, budget_bytes=64 << 30, max_jobs=256) as srv:
+            for k in ks:   # one job's three runs at a time
+                j = stream[k]
+                m = serve.job_mask(**j)
+                ra, rb = wl.served_density(j["nx"], j["ny"], j["ic"], j["rho_hi"], j["rho_lo"], j["radius"],
+                                           j["noise"], j["ic_seed"])
+                f0 = [wl.f_from_rho(ra, m), wl.f_from_rho(rb, m)]
+                u = [wl.uniform01(1000 + 2 * k + c, f0[c].size) for c in range(2)]
+                fp = [f0[c] * (1.0 + a.eps * (2.0 * u[c] - 1.0)) for c in range(2)]
+                base = {kk: v for kk, v in j.items() if kk not in ("porosity", "geom_seed", "ic", "ic_seed")}
+                collect(srv, {(k, "gen"): srv.submit(deliver=serve.DELIVER_RHO, **j),
+                              (k, "exp"): srv.submit(deliver=serve.DELIVER_RHO, mask=m, f=f0, **base),
+                              (k, "pert"): srv.submit(deliver=serve.DELIVER_RHO, mask=m, f=fp, **base)}, nat.FP64)
+                g64, e64, p64 = (runs[(nat.FP64, k, t)] for t in ("gen", "exp", "pert"))
+                g32 = runs[(nat.FP32_DEV, k, "gen")]
+                err = lambda x, y: (max(float(np.max(np.abs(x[c] - y[c]))) for c in range(2))
+                                    if x is not None and y is not None else None)
+                print(json.dumps({"job": k, "nx": j["nx"], "ny": j["ny"], "steps": j["steps"], "eps": a.eps,
+                                  "fp32_vs_fp64": err(g32, g64), "fp64_noise_floor": err(p64, e64),
+                                  "explicit_equals_generated": bool(
+                                      e64 is not None and g64 is not None and
+                                      all(np.array_equal(e64[c], g64[c]) for c in range(2)))}), flush=True)
+                for t in ("gen", "exp", "pert"):
+                    runs.pop((nat.FP64, k, t), None)
         return
     if a.cmd == "geometry":
         import time
