@@ -596,8 +596,8 @@ class GpuServe final : public serve::Backend {
                    std::to_string(j.fluid_sites);
             return SCL_ERR_HIP;
         }
-        j.disks.clear();
-        j.disks.shrink_to_fit();
+        // The disks stay with the job (a few KB): a suspended job's lattice
+        // is rebuilt from them when it resumes.
         return SCL_OK;
     }
 
