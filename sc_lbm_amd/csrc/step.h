@@ -494,36 +494,46 @@ __device__ inline void collide2_core(const StepArgs<T>& a, const U& u, int s_idx
     const float inv_den = rcp(fmaf(ra, A.omega, rb * B.omega));
     const float ucx = fmaf(jax, A.omega, jbx * B.omega) * inv_den;
     const float ucy = fmaf(jay, A.omega, jby * B.omega) * inv_den;
+    // Equilibrium velocities; at dry sites (|rho| <= 1e-16) the held ones
+    // replace them below.
     float uea[2], ueb[2];
-    if (!is_dry(ra)) {
+    {
         const float ir = rcp(ra);
         uea[0] = fmaf(A.inv_omega, fmaf(fa_x, ir, fs_ax), ucx);
         uea[1] = fmaf(A.inv_omega, fmaf(fa_y, ir, fs_ay), ucy);
-        if (fabsf(ra) < ST::kUeqWatch) {
-            G(u.ox(0))[s_idx] = uea[0];
-            G(u.oy(0))[s_idx] = uea[1];
-        }
-    } else {
-        const double ux0 = G(u.ix(0))[s_idx], uy0 = G(u.iy(0))[s_idx];
-        uea[0] = (float)ux0;
-        uea[1] = (float)uy0;
-        G(u.ox(0))[s_idx] = ux0;
-        G(u.oy(0))[s_idx] = uy0;
     }
-    if (!is_dry(rb)) {
+    {
         const float ir = rcp(rb);
         ueb[0] = fmaf(B.inv_omega, fmaf(fb_x, ir, fs_bx), ucx);
         ueb[1] = fmaf(B.inv_omega, fmaf(fb_y, ir, fs_by), ucy);
-        if (fabsf(rb) < ST::kUeqWatch) {
+    }
+    // The zero-density rule, behind one wave-uniform branch: sites below the
+    // watch threshold are rare, and per-lane branches for each component cost
+    // the common path their compares, exec-mask updates and the buffer
+    // addresses.
+    const bool dry_a = is_dry(ra), dry_b = is_dry(rb);
+    const bool held_a = dry_a || fabsf(ra) < ST::kUeqWatch, held_b = dry_b || fabsf(rb) < ST::kUeqWatch;
+    if (__builtin_amdgcn_ballot_w64(held_a || held_b) != 0) {
+        if (dry_a) {
+            const double ux0 = G(u.ix(0))[s_idx], uy0 = G(u.iy(0))[s_idx];
+            uea[0] = (float)ux0;
+            uea[1] = (float)uy0;
+            G(u.ox(0))[s_idx] = ux0;
+            G(u.oy(0))[s_idx] = uy0;
+        } else if (held_a) {
+            G(u.ox(0))[s_idx] = uea[0];
+            G(u.oy(0))[s_idx] = uea[1];
+        }
+        if (dry_b) {
+            const double ux0 = G(u.ix(1))[s_idx], uy0 = G(u.iy(1))[s_idx];
+            ueb[0] = (float)ux0;
+            ueb[1] = (float)uy0;
+            G(u.ox(1))[s_idx] = ux0;
+            G(u.oy(1))[s_idx] = uy0;
+        } else if (held_b) {
             G(u.ox(1))[s_idx] = ueb[0];
             G(u.oy(1))[s_idx] = ueb[1];
         }
-    } else {
-        const double ux0 = G(u.ix(1))[s_idx], uy0 = G(u.iy(1))[s_idx];
-        ueb[0] = (float)ux0;
-        ueb[1] = (float)uy0;
-        G(u.ox(1))[s_idx] = ux0;
-        G(u.oy(1))[s_idx] = uy0;
     }
     relax_dev<ST>(p[0], ra, da0, uea[0], uea[1], A.omega, A.force, FORCE && A.has_force, oa);
     relax_dev<ST>(p[1], rb, db0, ueb[0], ueb[1], B.omega, B.force, FORCE && B.has_force, ob);
