@@ -457,7 +457,8 @@ int launch_fused(const scl_geometry* g, scl_field* const* fl, const double* cons
 // from P_{n-1} in one launch (k_step2x).
 bool x2_supported(const scl_geometry* g, int precision) {
     static const size_t esz[3] = {8, 4, 2};
-    return precision != SCL_FP64 && g->nx >= kX2Threads && !needs_wide(g, esz[precision]);
+    return precision != SCL_FP64 && g->nx >= kX2Threads && g->ny >= xk_min_rows<2>() &&
+           !needs_wide(g, esz[precision]);
 }
 
 // whether two-component steps of `precision` on g go two per launch
@@ -561,7 +562,7 @@ int launch_x2(const scl_geometry* g, scl_field* const* fl, const double* const* 
 // else three per launch with FP32 storage (72k against 61k MLUPS for two on
 // a 4096^2 lattice) and two with FP16 storage (74k against 82k).
 bool x3_wanted(const scl_geometry* g, int precision) {
-    if (!x2_supported(g, precision) || g->fusion == 1 || g->fusion == 2) return false;
+    if (!x2_supported(g, precision) || g->ny < xk_min_rows<3>() || g->fusion == 1 || g->fusion == 2) return false;
     if (g->fusion == 3) return true;
     if (g->x3_tuned[precision]) return g->x3_tuned[precision] > 0;
     return precision == SCL_FP32_DEV;

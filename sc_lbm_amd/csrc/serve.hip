@@ -239,12 +239,13 @@ class GpuServe final : public serve::Backend {
     int run_round(const std::vector<Job*>& jobs, const std::vector<int>& steps, int rsteps,
                   std::string* err) override {
         SHIP(hipSetDevice(dev_));
-        // two-step jobs: two steps in the round and at least kX2MinCols
-        // columns (a job narrower than a strip takes one strip, whose lanes
-        // past its columns and halos idle)
+        // two-step jobs: two steps in the round, at least kX2MinCols columns
+        // (a job narrower than a strip takes one strip, whose lanes past its
+        // columns and halos idle) and xk_min_rows<2>() rows
         std::vector<char> two(jobs.size(), 0);
         for (size_t k = 0; k < jobs.size(); ++k)
-            two[k] = x2_ && rsteps >= 2 && steps[k] >= 2 && jobs[k]->spec.nx >= kX2MinCols;
+            two[k] = x2_ && rsteps >= 2 && steps[k] >= 2 && jobs[k]->spec.nx >= kX2MinCols &&
+                     jobs[k]->spec.ny >= xk_min_rows<2>();
         // work-list order: (0) the single-step jobs, most steps first; (1) the
         // two-step jobs with an odd step count, fewest steps first; (2) the
         // other two-step jobs (their tiles only serve the mass reduction).

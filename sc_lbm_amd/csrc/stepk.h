@@ -38,6 +38,14 @@
 
 namespace scl {
 
+// Fewest lattice rows a K-step tile takes: its rows (halos and mask rows
+// included) reach 2K + 1 rows past the lattice on either side, and a single
+// conditional add or subtract must wrap them.
+template <int K>
+__host__ __device__ constexpr int xk_min_rows() {
+    return 2 * K + 2;
+}
+
 template <class ST, int K, int T>
 __host__ __device__ constexpr int xk_out() {
     return sizeof(typename ST::S) == 4 ? (T - 2 * (2 * K - 1)) / 32 * 32 : T - 2 * (2 * K - 1);
@@ -224,9 +232,14 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
     int X = x0 - H + min(t, tl - 1);   // one wrap at most (nx >= T)
     X = X < 0 ? X + nx : (X >= nx ? X - nx : X);
     const int Xm = wrap1(X - 1, nx), Xp = wrap1(X + 1, nx);
+    // Periodic row index.  Every row a tile touches (mask rows included) lies
+    // within 2K + 1 rows of the lattice, and the lattice has at least
+    // xk_min_rows<K>() = 2K + 2 rows (the host runs single steps on shorter
+    // ones), so one conditional add or subtract wraps it.  (A remainder
+    // costs 17 scalar instructions, three row indices a row.)
     auto wr = [&](int r) {
-        r %= ny;
-        return r < 0 ? r + ny : r;
+        r = r < 0 ? r + ny : r;
+        return r >= ny ? r - ny : r;
     };
     // Ring slots.  Row `row` sits in slot (row - ra) mod 3 of a three-row
     // ring and (row - ra) mod 2 of a two-row ring, ra the first row stage 1
