@@ -432,18 +432,18 @@ __device__ inline UeqPtrs ueq_ptrs(const StepArgs<T>& a) {
 // FORCE = false: no body force on either component (has_force is then not
 // read, so kernels built for force-free lattices carry neither the force
 // values nor the additions).
-template <class T, class U, bool FORCE = true, class ST = StoreF32>
+template <class T, class U, bool FORCE = true, class ST = StoreF32, bool OPEN = false>
 __device__ inline void collide2_core(const StepArgs<T>& a, const U& u, int s_idx, const float (&p)[2][Q],
                                      const float (&rc)[2], const float (&nr)[2][Q], unsigned nb, float (&oa)[Q],
                                      float (&ob)[Q]);
 
-template <class T, class ST>
+template <class T, class ST, bool OPEN = false>
 __device__ inline void collide2_dev(const StepArgs<T>& a, int s_idx, const float (&p)[2][Q], const float (&rc)[2],
                                     const float (&nr)[2][Q], unsigned nb, float (&oa)[Q], float (&ob)[Q]) {
-    collide2_core<T, UeqPtrs, true, ST>(a, ueq_ptrs(a), s_idx, p, rc, nr, nb, oa, ob);
+    collide2_core<T, UeqPtrs, true, ST, OPEN>(a, ueq_ptrs(a), s_idx, p, rc, nr, nb, oa, ob);
 }
 
-template <class T, class U, bool FORCE, class ST>
+template <class T, class U, bool FORCE, class ST, bool OPEN>
 __device__ inline void collide2_core(const StepArgs<T>& a, const U& u, int s_idx, const float (&p)[2][Q],
                                      const float (&rc)[2], const float (&nr)[2][Q], unsigned nb, float (&oa)[Q],
                                      float (&ob)[Q]) {
@@ -480,11 +480,17 @@ __device__ inline void collide2_core(const StepArgs<T>& a, const U& u, int s_idx
     const float da0 = rc[0], db0 = rc[1];
     const float ra = A.rho0 + da0, rb = B.rho0 + db0;
     // psi_other over fluid neighbours = sum w c drho + rho0_other * sum w c
+    // OPEN (the kernels of lattices without solids): sum w c over the fluid
+    // neighbours is +0, so the rho0 terms and the adhesion drop out.
+    // fma(a, b, +-0) and a * b differ only in the sign of a zero result,
+    // which the next sum with a nonzero term erases, so both forms give the
+    // same populations.
+    constexpr bool open = OPEN;
     const float ga = A.gneg * ra, gb = B.gneg * rb;
-    const float fa_x = ga * fmaf(B.rho0, sfx, sd[0][0]);
-    const float fa_y = ga * fmaf(B.rho0, sfy, sd[0][1]);
-    const float fb_x = gb * fmaf(A.rho0, sfx, sd[1][0]);
-    const float fb_y = gb * fmaf(A.rho0, sfy, sd[1][1]);
+    const float fa_x = ga * (open ? sd[0][0] : fmaf(B.rho0, sfx, sd[0][0]));
+    const float fa_y = ga * (open ? sd[0][1] : fmaf(B.rho0, sfy, sd[0][1]));
+    const float fb_x = gb * (open ? sd[1][0] : fmaf(A.rho0, sfx, sd[1][0]));
+    const float fb_y = gb * (open ? sd[1][1] : fmaf(A.rho0, sfy, sd[1][1]));
     // adhesion: sum over solid nbrs of w c = -(sum over fluid nbrs)
     const float fs_ax = A.gsneg * -sfx, fs_ay = A.gsneg * -sfy;
     const float fs_bx = B.gsneg * -sfx, fs_by = B.gsneg * -sfy;
@@ -499,13 +505,13 @@ __device__ inline void collide2_core(const StepArgs<T>& a, const U& u, int s_idx
     float uea[2], ueb[2];
     {
         const float ir = rcp(ra);
-        uea[0] = fmaf(A.inv_omega, fmaf(fa_x, ir, fs_ax), ucx);
-        uea[1] = fmaf(A.inv_omega, fmaf(fa_y, ir, fs_ay), ucy);
+        uea[0] = fmaf(A.inv_omega, open ? fa_x * ir : fmaf(fa_x, ir, fs_ax), ucx);
+        uea[1] = fmaf(A.inv_omega, open ? fa_y * ir : fmaf(fa_y, ir, fs_ay), ucy);
     }
     {
         const float ir = rcp(rb);
-        ueb[0] = fmaf(B.inv_omega, fmaf(fb_x, ir, fs_bx), ucx);
-        ueb[1] = fmaf(B.inv_omega, fmaf(fb_y, ir, fs_by), ucy);
+        ueb[0] = fmaf(B.inv_omega, open ? fb_x * ir : fmaf(fb_x, ir, fs_bx), ucx);
+        ueb[1] = fmaf(B.inv_omega, open ? fb_y * ir : fmaf(fb_y, ir, fs_by), ucy);
     }
     // The zero-density rule, behind one wave-uniform branch: sites below the
     // watch threshold are rare, and per-lane branches for each component cost
@@ -539,7 +545,7 @@ __device__ inline void collide2_core(const StepArgs<T>& a, const U& u, int s_idx
     relax_dev<ST>(p[1], rb, db0, ueb[0], ueb[1], B.omega, B.force, FORCE && B.has_force, ob);
 }
 
-template <class ST, bool REPLAY, class OffT, bool GW>
+template <class ST, bool REPLAY, class OffT, bool GW, bool OPEN = false>
 __device__ inline void finish2(const StepArgs<typename ST::T>& a, void* const (&dst)[2], OffT plane_b, int s_idx,
                                OffT home,
                                const typename ST::T (&p)[2][Q], const typename ST::T (&rc)[2],
@@ -656,7 +662,7 @@ __device__ inline void finish2(const StepArgs<typename ST::T>& a, void* const (&
         // deviation forms, FP32 arithmetic
         if (!REPLAY && fl) {
             float oa[Q], ob[Q];
-            collide2_dev<typename ST::T, ST>(a, s_idx, p, rc, nr, nb, oa, ob);
+            collide2_dev<typename ST::T, ST, OPEN>(a, s_idx, p, rc, nr, nb, oa, ob);
 #pragma unroll
             for (int i = 0; i < Q; ++i) {
                 st_at<ST>(dst[0], (OffT)i * plane_b + home, oa[i]);
@@ -906,7 +912,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<typename ST::T>& a, con
                     r0[c] = lds[sC][c][tid + 1];
                     nr[c][0] = r0[c];
                 }
-                finish2<ST, REPLAY, OffT, GW>(a, b.dst, plane_b, s_idx, home, p0, r0, nr, OPEN ? 0x1FFu : nb0, nbr);
+                finish2<ST, REPLAY, OffT, GW, OPEN>(a, b.dst, plane_b, s_idx, home, p0, r0, nr, OPEN ? 0x1FFu : nb0, nbr);
             }
         }
         if (LA == 2 && more) {

@@ -410,7 +410,7 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
             if (r <= r1_last && t >= 1 && t < tl - 1 && (OPEN || (nbw[2] & 1u))) {
                 float nr[2][Q], rc[2], oa[Q], ob[Q];
                 nbr_dens(an, 0, r, nr, rc);
-                collide2_core<float, UeqStage, FORCE, ST>(a, UeqStage{A.u, 0}, wr(r) * a.pitch + X, pa, rc, nr,
+                collide2_core<float, UeqStage, FORCE, ST, OPEN>(a, UeqStage{A.u, 0}, wr(r) * a.pitch + X, pa, rc, nr,
                                                       OPEN ? 0x1FFu : nbw[2], oa, ob);
 #pragma unroll
                 for (int i = 1; i < Q; ++i) {
@@ -432,7 +432,7 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
                         (OPEN || (nbh & 1u))) {
                         float nr[2][Q], rc[2], oa[Q], ob[Q];
                         nbr_dens(an, s - 1, h, nr, rc);
-                        collide2_core<float, UeqStage, FORCE, ST>(a, UeqStage{A.u, s - 1}, wr(h) * a.pitch + X, sx[q], rc,
+                        collide2_core<float, UeqStage, FORCE, ST, OPEN>(a, UeqStage{A.u, s - 1}, wr(h) * a.pitch + X, sx[q], rc,
                                                               nr, OPEN ? 0x1FFu : nbh, oa, ob);
 #pragma unroll
                         for (int i = 1; i < Q; ++i) {
@@ -445,7 +445,7 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
                 } else if (h >= y0 && h < y1 && out_lane && (OPEN || (nbh & 1u))) {
                     float nr[2][Q], rc[2], oa[Q], ob[Q];
                     nbr_dens(an, s - 1, h, nr, rc);
-                    collide2_core<float, UeqStage, FORCE, ST>(a, UeqStage{A.u, s - 1}, h * a.pitch + X, sx[q], rc, nr,
+                    collide2_core<float, UeqStage, FORCE, ST, OPEN>(a, UeqStage{A.u, s - 1}, h * a.pitch + X, sx[q], rc, nr,
                                                           OPEN ? 0x1FFu : nbh, oa, ob);
                     const OffT home = (OffT)h * pitch_b + (OffT)X * ES;
 #pragma unroll
@@ -536,7 +536,7 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
             if (t >= 1 && t < tl - 1 && (OPEN || (nbw[2] & 1u))) {
                 float nr[2][Q], rc[2], oa[Q], ob[Q];
                 nbr_dens(an, 0, r, nr, rc);
-                collide2_core<float, UeqStage, FORCE, ST>(a, UeqStage{A.u, 0}, wr(r) * a.pitch + X, pa, rc, nr, OPEN ? 0x1FFu : nbw[2], oa, ob);
+                collide2_core<float, UeqStage, FORCE, ST, OPEN>(a, UeqStage{A.u, 0}, wr(r) * a.pitch + X, pa, rc, nr, OPEN ? 0x1FFu : nbw[2], oa, ob);
                 if constexpr (K > 1) {
 #pragma unroll
                     for (int i = 1; i < Q; ++i) {
@@ -583,7 +583,7 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
                     if (h >= y0 - 2 * (K - s) && t >= 2 * s - 1 && t < tl - 2 * s + 1 && (OPEN || (nbh & 1u))) {
                         float nr[2][Q], rc[2], oa[Q], ob[Q];
                         nbr_dens(an, s - 1, h, nr, rc);
-                        collide2_core<float, UeqStage, FORCE, ST>(a, UeqStage{A.u, s - 1}, wr(h) * a.pitch + X, sx[q], rc, nr, OPEN ? 0x1FFu : nbh, oa, ob);
+                        collide2_core<float, UeqStage, FORCE, ST, OPEN>(a, UeqStage{A.u, s - 1}, wr(h) * a.pitch + X, sx[q], rc, nr, OPEN ? 0x1FFu : nbh, oa, ob);
 #pragma unroll
                         for (int i = 1; i < Q; ++i) {
                             pr(an, s - 1, h, 0, i, t) = ST::enc(oa[i]);
@@ -596,7 +596,7 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
                 } else if (h >= y0 && h < y1 && out_lane && (OPEN || (nbh & 1u))) {
                     float nr[2][Q], rc[2], oa[Q], ob[Q];
                     nbr_dens(an, s - 1, h, nr, rc);
-                    collide2_core<float, UeqStage, FORCE, ST>(a, UeqStage{A.u, s - 1}, h * a.pitch + X, sx[q], rc, nr, OPEN ? 0x1FFu : nbh, oa, ob);
+                    collide2_core<float, UeqStage, FORCE, ST, OPEN>(a, UeqStage{A.u, s - 1}, h * a.pitch + X, sx[q], rc, nr, OPEN ? 0x1FFu : nbh, oa, ob);
                     const OffT home = (OffT)h * pitch_b + (OffT)X * ES;
 #pragma unroll
                     for (int i = 0; i < Q; ++i) {   // 32-bit offsets: a component's buffer is < 4 GiB
