@@ -112,7 +112,8 @@ def time_form(nat, wl, case, form, steps, warmup, tile, fusion=""):
     # (kernel MLUPS x bytes per update), what one step per launch would move
     return dict(form=form, seconds=dt, ms_per_step=dt * 1e3 / steps, mlups=mlups,
                 bytes_per_launch=n_sites * BYTES_PER_LU[form],
-                kernel={1: "k_step", 2: "k_step2x", 3: "k_stepk"}[spl], steps_per_launch=spl, band_rows=band,
+                kernel={1: "k_step", 2: "k_step2d" if form == "fp64" else "k_step2x", 3: "k_stepk"}[spl],
+                steps_per_launch=spl, band_rows=band,
                 kernel_avg_ms=launch_ms, kernel_step_ms=avg_ms, kernel_launches=cnt, tile=list(tile_used),
                 effective_gbs=n_sites * BYTES_PER_LU[form] / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else None,
                 launch_pass_gbs=n_sites * BYTES_PER_LU[form] / (launch_ms * 1e-3) / 1e9 if launch_ms > 0 else None,

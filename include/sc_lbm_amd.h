@@ -223,15 +223,24 @@ int scl_set_tile(scl_geometry* g, int cols, int rows);
  * do not depend on it. */
 int scl_set_lookahead(scl_geometry* g, int rows_ahead);
 /* Time steps per launch on g (no reference counterpart).  Two-component
- * deviation-form fields on lattices at least 256 columns wide can advance two
- * or three steps per launch (the intermediate populations stay on chip;
- * results are bit for bit those of single steps).  0 (the default): what
- * scl_autotune measured fastest, else three per launch with FP32 storage and
- * two with FP16 storage; 1, 2 or 3: that many per launch wherever supported
- * (a call's remaining steps go two or one per launch).  rows: rows per band
- * of the multi-step kernels (0 = autotuned, else the height that makes the
- * grid two whole waves of resident workgroups). */
+ * fields on lattices at least 256 columns and 2K + 2 rows wide can advance
+ * K = 2 (every form) or 3 (deviation forms) steps per launch (the
+ * intermediate populations stay on chip; results are bit for bit those of
+ * single steps).  0 (the default): what scl_autotune measured fastest, else
+ * three per launch with FP32 storage and two with FP16 storage or FP64; 1, 2
+ * or 3: that many per launch wherever supported (a call's remaining steps go
+ * two or one per launch; FP64 takes two while at least three remain, so the
+ * call's last step is a single one).  FP64 fields hand over to single steps
+ * where a density falls below 1e-8 (the zero-density rule's held velocities
+ * need them; the launch that met it is repeated).  rows: rows per band of
+ * the multi-step kernels (0 = autotuned, else the height that makes the grid
+ * two whole waves of resident workgroups). */
 int scl_set_step_fusion(scl_geometry* g, int steps_per_launch, int rows);
+/* Density below which FP64 steps on g hand over from two per launch to one
+ * (no reference counterpart; default and least value 1e-8, the zero-density
+ * rule's watch threshold).  A larger value only hands over earlier; results
+ * do not depend on it. */
+int scl_set_fp64_fusion_watch(scl_geometry* g, double rho);
 /* What two-component steps of `precision` on g use: steps per launch (1, 2
  * or 3) and the band rows of the multi-step kernel (0 when one per
  * launch). */

@@ -86,6 +86,7 @@ struct Lib {
     int cus = 0;                 // compute units of the device
     int x2_occ[3][2] = {};       // [precision][solid-free]: two-step workgroups per CU (0 = not queried)
     double* scratch = nullptr;   // address of `scratch` on the library's device (set at initialisation)
+    unsigned* f64_flag = nullptr;   // FP64 two-step launches: number of the first that met a dilute site
 };
 Lib g_lib;
 
@@ -106,6 +107,8 @@ int lib_ready() {
     HIPCHK(hipStreamCreateWithFlags(&g_lib.stream, hipStreamNonBlocking));
     // the allocation-free entry point's array: its address on this device
     HIPCHK(hipGetSymbolAddress((void**)&g_lib.scratch, HIP_SYMBOL(scratch)));
+    HIPCHK(hipMalloc((void**)&g_lib.f64_flag, 256));
+    HIPCHK(hipMemset(g_lib.f64_flag, 0, 256));
     g_lib.ready = true;
     return SCL_OK;
 }
@@ -141,6 +144,7 @@ struct scl_geometry {
     int x2_rows = 0;                    // rows per band of the multi-step kernels (0 = automatic)
     int x2_tuned[3] = {};               // [precision]: autotuned band rows; -1: single steps were faster
     int x3_tuned[3] = {};               // [precision]: the same for three steps per launch; -1: slower
+    double x2d_watch = kUeqWatch64;     // FP64 two-step launches hand over below this density
     double* part = nullptr;             // moment partials (3 per row + 3)
     int* obs = nullptr;                 // droplet-observable row extents (6 per row), allocated on first use
     // Fields keep their geometry alive: destroying a geometry that fields
@@ -184,6 +188,9 @@ struct scl_field {
     // streamed per operator).
     unsigned* stamp = nullptr;
     unsigned nstep = 0;
+    // FP64: a two-step launch met a site below the zero-density rule's watch
+    // threshold; this field's steps go one per launch from then on
+    bool dilute = false;
     // reference per-site arrays (FP64, pitched)
     double *rho = nullptr, *ux = nullptr, *uy = nullptr, *ueqx = nullptr, *ueqy = nullptr;
     double *frx = nullptr, *fry = nullptr, *fsx = nullptr, *fsy = nullptr, *feq = nullptr;
@@ -473,13 +480,17 @@ bool x2_wanted(const scl_geometry* g, int precision) {
 int x2_occupancy(int precision, bool open) {
     int& o = g_lib.x2_occ[precision][open ? 1 : 0];
     if (!o) {
-        const void* k = precision == SCL_FP32_DEV
+        const void* k = precision == SCL_FP64
+                            ? (open ? reinterpret_cast<const void*>(&k_step2d<true, false>)
+                                    : reinterpret_cast<const void*>(&k_step2d<false, false>))
+                        : precision == SCL_FP32_DEV
                             ? (open ? reinterpret_cast<const void*>(&k_step2x<StoreF32, true, false>)
                                     : reinterpret_cast<const void*>(&k_step2x<StoreF32, false, false>))
                             : (open ? reinterpret_cast<const void*>(&k_step2x<StoreF16, true, false>)
                                     : reinterpret_cast<const void*>(&k_step2x<StoreF16, false, false>));
+        const int threads = precision == SCL_FP64 ? kX2dThreads : kX2Threads;
         int n = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, kX2Threads, 0) != hipSuccess || n < 1) n = 2;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, threads, 0) != hipSuccess || n < 1) n = 1;
         o = n;
     }
     return o;
@@ -488,7 +499,9 @@ int x2_occupancy(int precision, bool open) {
 // Band height that makes the grid `waves` whole waves of resident workgroups
 // (a partial last wave leaves CUs idle at the end of every launch).
 int x2_wave_rows(const scl_geometry* g, int precision, int waves) {
-    const int out = precision == SCL_FP32_DEV ? x2_out<StoreF32>() : x2_out<StoreF16>();
+    const int out = precision == SCL_FP64       ? xk_out<StoreF64, 2, kX2dThreads>()
+                    : precision == SCL_FP32_DEV ? x2_out<StoreF32>()
+                                                : x2_out<StoreF16>();
     const int strips = (g->nx + out - 1) / out;
     const int slots = x2_occupancy(precision, g->open) * std::max(g_lib.cus, 1);
     const int bands = std::max(1, waves * slots / strips);
@@ -637,6 +650,75 @@ int launch_x3(const scl_geometry* g, scl_field* const* fl, const double* const* 
     return g->open ? launch_x3_t<ST, true>(g, fl, forces) : launch_x3_t<ST, false>(g, fl, forces);
 }
 
+// Two fused FP64 steps (k_step2d): P_{n+1} from P_{n-1} with the reference's
+// arithmetic, bit for bit two single steps.  A site below the zero-density
+// rule's watch threshold (whose held equilibrium velocity lives in HBM, with
+// the records the single-step kernel keeps) stops the sequence: the launch
+// writes its number into g_lib.f64_flag, later launches return at once, and
+// fused_steps repeats the steps from that launch's input one per launch.
+bool x2d_supported(const scl_geometry* g) {
+    return g->nx >= kX2dThreads && g->ny >= xk_min_rows<2>() && !needs_wide(g, sizeof(double));
+}
+
+bool x2d_wanted(const scl_geometry* g) {
+    if (g->fusion == 1 || !x2d_supported(g)) return false;
+    return g->fusion >= 2 || g->x2_tuned[SCL_FP64] >= 0;
+}
+
+bool x2d_ok(const scl_geometry* g, scl_field* const* fl, int nc) {
+    return nc == 2 && fl[0]->precision == SCL_FP64 && fl[0]->phase == PH_P && fl[1]->phase == PH_P &&
+           !fl[0]->dilute && !fl[1]->dilute && x2d_wanted(g);
+}
+
+int launch_x2d(const scl_geometry* g, scl_field* const* fl, const double* const* forces, unsigned launch) {
+    constexpr int OUT = xk_out<StoreF64, 2, kX2dThreads>();
+    const int rows = x2_rows_for(g, SCL_FP64);
+    XkArgs<2, double> A{};
+    A.a = make_args<double>(g, rows, 4);
+    A.a.gx = (g->nx + OUT - 1) / OUT;
+    for (int c = 0; c < 2; ++c) {
+        fill_comp(A.a.c[c], fl[c], fl[c]->cur, forces[c]);
+        A.dst[c] = fl[c]->f[fl[c]->cur ^ 1];
+    }
+    A.flag = g_lib.f64_flag;
+    A.launch = launch;
+    A.watch = g->x2d_watch;
+    const int nblk = A.a.gx * ((g->ny + rows - 1) / rows);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (g_lib.timing && g_lib.ev.size() < kMaxTimed) {
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, g_lib.stream));
+    }
+    const bool force = forces[0] || forces[1];
+    const dim3 grid(nblk), block(kX2dThreads);
+    if (g->open) {
+        if (force) hipLaunchKernelGGL((k_step2d<true, true>), grid, block, 0, g_lib.stream, A);
+        else hipLaunchKernelGGL((k_step2d<true, false>), grid, block, 0, g_lib.stream, A);
+    } else {
+        if (force) hipLaunchKernelGGL((k_step2d<false, true>), grid, block, 0, g_lib.stream, A);
+        else hipLaunchKernelGGL((k_step2d<false, false>), grid, block, 0, g_lib.stream, A);
+    }
+    HIPCHK(hipGetLastError());
+    if (e0) {
+        HIPCHK(hipEventRecord(e1, g_lib.stream));
+        g_lib.ev.emplace_back(e0, e1);
+        g_lib.ev_steps.push_back(2);
+    }
+    return SCL_OK;
+}
+
+// The number of the first FP64 two-step launch since the last call that met
+// a dilute site (0: none); clears it.
+int x2d_failed(unsigned* launch) {
+    unsigned v = 0;
+    HIPCHK(hipMemcpyAsync(&v, g_lib.f64_flag, sizeof v, hipMemcpyDeviceToHost, g_lib.stream));
+    HIPCHK(hipStreamSynchronize(g_lib.stream));
+    if (v) HIPCHK(hipMemsetAsync(g_lib.f64_flag, 0, sizeof v, g_lib.stream));
+    *launch = v;
+    return SCL_OK;
+}
+
 template <int NC, bool GATHER>
 void replay_t(const Replay& r, int nblk, bool wide) {
     if (wide)
@@ -766,8 +848,47 @@ int fused_steps(const scl_geometry* g, scl_field** fl, const double* const* forc
         CHK(ensure_s(fl[0]));
         CHK(ensure_s(fl[1]));
     }
+    // FP64 two-step launches go in runs that end in a check of the flag
+    // (x2d_failed): the run's first step, buffers and step counts, to go
+    // back to the input of a launch that met a dilute site
+    unsigned run = 0;
+    int run_t = 0, run_cur[NC];
+    unsigned run_nstep[NC];
     for (int t = 0; t < nsteps;) {
         if constexpr (NC == 2) {
+            // FP64: two steps per launch while at least three remain (the last
+            // step of a call records the replay of the per-operator getters)
+            if (nsteps - t >= 3 && x2d_ok(g, fl, NC)) {
+                if (run == 0) {
+                    run_t = t;
+                    for (int c = 0; c < NC; ++c) {
+                        run_cur[c] = fl[c]->cur;
+                        run_nstep[c] = fl[c]->nstep;
+                    }
+                }
+                CHK(launch_x2d(g, fl, forces, ++run));
+                for (int c = 0; c < NC; ++c) {
+                    fl[c]->cur ^= 1;
+                    advance_ueq(fl[c], 2);
+                }
+                t += 2;
+                continue;
+            }
+            if (run) {
+                unsigned bad = 0;
+                CHK(x2d_failed(&bad));
+                if (bad) {   // repeat from the input of launch `bad`, one step per launch
+                    const int j = (int)bad - 1;
+                    t = run_t + 2 * j;
+                    for (int c = 0; c < NC; ++c) {
+                        fl[c]->cur = run_cur[c] ^ (j & 1);
+                        fl[c]->nstep = run_nstep[c] + 2u * (unsigned)j;
+                        fl[c]->dilute = true;
+                    }
+                }
+                run = 0;
+                continue;
+            }
             if (nsteps - t >= 3 && x2_ok(g, fl, NC) && x3_wanted(g, prec)) {
                 const int rc = prec == SCL_FP32_DEV ? launch_x3<StoreF32>(g, fl, forces)
                                                     : launch_x3<StoreF16>(g, fl, forces);
@@ -935,9 +1056,22 @@ int scl_set_step_fusion(scl_geometry* g, int steps_per_launch, int rows) {
     return SCL_OK;
 }
 
+int scl_set_fp64_fusion_watch(scl_geometry* g, double rho) {
+    if (!g) return fail(SCL_ERR_ARG, "null geometry");
+    if (!(rho >= kUeqWatch64) || !(rho <= 1e300)) return fail(SCL_ERR_ARG, "watch density must be finite and >= 1e-8");
+    g->x2d_watch = rho;
+    return SCL_OK;
+}
+
 int scl_get_step_fusion(const scl_geometry* g, int precision, int* steps_per_launch, int* rows) {
     if (!g || !steps_per_launch || !rows) return fail(SCL_ERR_ARG, "null argument");
     if (precision < SCL_FP64 || precision > SCL_FP16_DEV) return fail(SCL_ERR_ARG, "bad precision");
+    if (precision == SCL_FP64) {
+        const bool two = x2d_wanted(g);
+        *steps_per_launch = two ? 2 : 1;
+        *rows = two ? x2_rows_for(g, precision) : 0;
+        return SCL_OK;
+    }
     if (x3_wanted(g, precision)) {
         *steps_per_launch = 3;
         *rows = x3_rows_for(g, precision);
@@ -1106,6 +1240,7 @@ int scl_field_upload(scl_field* f, int which, const double* src, size_t n) {
             HIPCHK(hipGetLastError());
         }
         f->phase = PH_S;
+        f->dilute = false;
         CHK(set_ueq_valid(f, f->nstep));
         HIPCHK(hipStreamSynchronize(g_lib.stream));
         return SCL_OK;
@@ -1708,13 +1843,16 @@ int scl_autotune(scl_geometry* g, int ncomp, int precision, int* cols, int* rows
             }
         }
     }
-    // two steps per launch: rows per band, kept only if faster than single steps
-    if (rc == SCL_OK && bc && ncomp == 2 && x2_supported(g, precision)) {
+    // two steps per launch: rows per band, kept only if faster than single
+    // steps.  (FP64 calls end in a single step: odd step counts, so that
+    // all but one go two per launch.)
+    const bool f64 = precision == SCL_FP64;
+    double best2 = 1e300;
+    if (rc == SCL_OK && bc && ncomp == 2 && (f64 ? x2d_supported(g) : x2_supported(g, precision))) {
         slot[0] = bc;
         slot[1] = br;
         slot[2] = bl;
         g->fusion = 2;
-        double best2 = 1e300;
         int b2 = 0;
         // whole waves of workgroups (1..4), and two fixed heights
         int bandset[6] = {16, 32, x2_wave_rows(g, precision, 1), x2_wave_rows(g, precision, 2),
@@ -1725,8 +1863,8 @@ int scl_autotune(scl_geometry* g, int ncomp, int precision, int* cols, int* rows
             if (seen) continue;
             g->x2_rows = bandset[bi];
             const double* none[2] = {nullptr, nullptr};
-            rc = fused_steps<2>(g, fl, none, 2);
-            int reps = 4;
+            rc = fused_steps<2>(g, fl, none, f64 ? 3 : 2);
+            int reps = f64 ? 5 : 4;
             float ms = 0.f;
             for (int pass = 0; pass < 2 && rc == SCL_OK; ++pass) {
                 if (hipEventRecord(e0, g_lib.stream) != hipSuccess) rc = fail(SCL_ERR_HIP, "event record failed");
@@ -1739,7 +1877,7 @@ int scl_autotune(scl_geometry* g, int ncomp, int precision, int* cols, int* rows
                     const double per = ms / reps;
                     int want = per > 0 ? (int)(3.0 / per) : 50;
                     want = want < 4 ? 4 : (want > 50 ? 50 : want);
-                    reps = want & ~1;
+                    reps = f64 ? (want | 1) : (want & ~1);
                 }
             }
             if (rc == SCL_OK && ms / reps < best2) {
@@ -1748,7 +1886,9 @@ int scl_autotune(scl_geometry* g, int ncomp, int precision, int* cols, int* rows
             }
         }
         if (rc == SCL_OK) g->x2_tuned[precision] = best2 < best ? b2 : -1;
-        // three steps per launch, kept only if faster than both
+    }
+    // three steps per launch (deviation forms), kept only if faster than both
+    if (rc == SCL_OK && bc && ncomp == 2 && !f64 && x2_supported(g, precision)) {
         g->fusion = 3;
         double best3 = 1e300;
         int b3 = 0;

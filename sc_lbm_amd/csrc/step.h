@@ -545,6 +545,105 @@ __device__ inline void collide2_core(const StepArgs<T>& a, const U& u, int s_idx
     relax_dev<ST>(p[1], rb, db0, ueb[0], ueb[1], B.omega, B.force, FORCE && B.has_force, ob);
 }
 
+// The reference's two-component arithmetic in FP64, operation by operation
+// (src/lbm.cpp:110-140, 163-204, 250-269, 308-309, 335-336), in pieces that
+// the single-step kernel (finish2) and the two-step FP64 tile share, so that
+// both evaluate the same expressions.
+//
+// Fluid-fluid repulsion and fluid-solid adhesion of a fluid site: rep[c] =
+// (sum over fluid neighbours of (w_i c_i) psi_other) * (G_neg,c psi_c),
+// fs[c] = G_sneg,c * (sum over solid neighbours of w_i c_i).  A skipped
+// (solid) neighbour adds +0.0, which leaves the sum unchanged: it starts at
+// +0.0 and a round-to-nearest sum is -0.0 only when both terms are, so
+// selects replace the reference's `continue` exactly.
+__device__ inline void forces2_ref(const StepArgs<double>& a, const double (&rc)[2], const double (&nr)[2][Q],
+                                   unsigned nb, double (&rep)[2][2], double (&fs)[2][2]) {
+    const auto& A = a.c[0];
+    const auto& B = a.c[1];
+    const double ra = rc[0], rb = rc[1];
+    double sax = 0.0, say = 0.0, sbx = 0.0, sby = 0.0;
+#pragma unroll
+    for (int i = 1; i < Q; ++i) {
+        const bool f = (nb >> i) & 1u;
+        const double pa = nr[0][i], pb = nr[1][i];
+        if (cx(i) != 0) {
+            const double wc = wint(i) * cx(i);
+            sax += f ? wc * pb : 0.0;
+            sbx += f ? wc * pa : 0.0;
+        }
+        if (cy(i) != 0) {
+            const double wc = wint(i) * cy(i);
+            say += f ? wc * pb : 0.0;
+            sby += f ? wc * pa : 0.0;
+        }
+    }
+    rep[0][0] = sax * (A.gneg * ra);
+    rep[0][1] = say * (A.gneg * ra);
+    rep[1][0] = sbx * (B.gneg * rb);
+    rep[1][1] = sby * (B.gneg * rb);
+    double ssx = 0.0, ssy = 0.0;
+#pragma unroll
+    for (int i = 1; i < Q; ++i) {
+        const bool solid = !((nb >> i) & 1u);
+        ssx += solid ? wint(i) * cx(i) : 0.0;
+        ssy += solid ? wint(i) * cy(i) : 0.0;
+    }
+    fs[0][0] = A.gsneg * ssx;
+    fs[0][1] = A.gsneg * ssy;
+    fs[1][0] = B.gsneg * ssx;
+    fs[1][1] = B.gsneg * ssy;
+}
+// Composite velocity u' = (j_A w_A + j_B w_B) / (rho_A w_A + rho_B w_B).
+__device__ inline void composite_ref(const StepArgs<double>& a, const double (&jm)[2][2], double ra, double rb,
+                                     double& ucx, double& ucy) {
+    const auto& A = a.c[0];
+    const auto& B = a.c[1];
+    const double den = ra * A.omega + rb * B.omega;
+    ucx = (jm[0][0] * A.omega + jm[1][0] * B.omega) / den;
+    ucy = (jm[0][1] * A.omega + jm[1][1] * B.omega) / den;
+}
+// Equilibrium velocity of a non-dry component, one direction.
+__device__ inline double ueq_ref(const CompArgs<double>& C, double uc, double rep, double fs, double r) {
+    return uc + rep * C.inv_omega / r + fs * C.inv_omega;
+}
+// BGK relaxation and the volume force of one population.
+__device__ inline double bgk_ref(const CompArgs<double>& C, int i, double p, double e, bool force) {
+    double v = C.omc * p + C.omega * e;
+    if (force) v += C.force[i];
+    return v;
+}
+
+// The two-step FP64 tile's collision of one fluid site: the same
+// expressions as finish2, with the equilibrium velocities taken only from
+// the step's own state.  The zero-density rule's held velocities live in
+// HBM for sites below kUeqWatch, and a tile's halo sites are computed by
+// two workgroups, so where either component's density is below `watch`
+// (>= kUeqWatch; or dry, or not finite) the site raises *flag (= launch)
+// instead, and the host repeats the launch's steps one per launch (whose
+// kernel keeps the records).  Elsewhere the result is bit for bit finish2's.
+template <bool FORCE>
+__device__ inline void collide2_f64(const StepArgs<double>& a, unsigned* flag, unsigned launch, double watch,
+                                    const double (&p)[2][Q], const double (&rc)[2], const double (&nr)[2][Q],
+                                    unsigned nb, double (&oa)[Q], double (&ob)[Q]) {
+    double rep[2][2], fs[2][2], jm[2][2], ucx, ucy;
+    forces2_ref(a, rc, nr, nb, rep, fs);
+    momentum(p[0], jm[0][0], jm[0][1]);
+    momentum(p[1], jm[1][0], jm[1][1]);
+    composite_ref(a, jm, rc[0], rc[1], ucx, ucy);
+    const double ma = fabs(rc[0]), mb = fabs(rc[1]);
+    const bool plain = ma >= watch && ma <= __DBL_MAX__ && mb >= watch && mb <= __DBL_MAX__;
+    if (!plain) __hip_atomic_fetch_or(flag, launch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const auto& C = a.c[c];
+        double e[Q];
+        feq_ref(rc[c], ueq_ref(C, ucx, rep[c][0], fs[c][0], rc[c]), ueq_ref(C, ucy, rep[c][1], fs[c][1], rc[c]), e);
+        double (&o)[Q] = c == 0 ? oa : ob;
+#pragma unroll
+        for (int i = 0; i < Q; ++i) o[i] = bgk_ref(C, i, p[c][i], e[i], FORCE && C.has_force);
+    }
+}
+
 template <class ST, bool REPLAY, class OffT, bool GW, bool OPEN = false>
 __device__ inline void finish2(const StepArgs<typename ST::T>& a, void* const (&dst)[2], OffT plane_b, int s_idx,
                                OffT home,
@@ -557,56 +656,23 @@ __device__ inline void finish2(const StepArgs<typename ST::T>& a, void* const (&
         const auto& B = a.c[1];
         const double ra = rc[0], rb = rc[1];
         double rep[2][2] = {{0.0, 0.0}, {0.0, 0.0}};   // [comp][x/y]
+        double fs[2][2];
         double jm[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
         double ue[2][2];
         double e[2][Q];
         if (fl) {
-            // repulsion: sum over fluid neighbours of (w_i c_i) psi_other
-            // A skipped (solid) neighbour adds +0.0, which leaves the sum
-            // unchanged: it starts at +0.0 and a round-to-nearest sum is -0.0
-            // only when both terms are, so selects replace the reference's
-            // `continue` exactly.
-            double sax = 0.0, say = 0.0, sbx = 0.0, sby = 0.0;
-#pragma unroll
-            for (int i = 1; i < Q; ++i) {
-                const bool f = (nb >> i) & 1u;
-                const double pa = nr[0][i], pb = nr[1][i];
-                if (cx(i) != 0) {
-                    const double wc = wint(i) * cx(i);
-                    sax += f ? wc * pb : 0.0;
-                    sbx += f ? wc * pa : 0.0;
-                }
-                if (cy(i) != 0) {
-                    const double wc = wint(i) * cy(i);
-                    say += f ? wc * pb : 0.0;
-                    sby += f ? wc * pa : 0.0;
-                }
-            }
-            rep[0][0] = sax * (A.gneg * ra);
-            rep[0][1] = say * (A.gneg * ra);
-            rep[1][0] = sbx * (B.gneg * rb);
-            rep[1][1] = sby * (B.gneg * rb);
-            // adhesion: weighted lattice vectors towards solid neighbours
-            double ssx = 0.0, ssy = 0.0;
-#pragma unroll
-            for (int i = 1; i < Q; ++i) {
-                const bool solid = !((nb >> i) & 1u);
-                ssx += solid ? wint(i) * cx(i) : 0.0;
-                ssy += solid ? wint(i) * cy(i) : 0.0;
-            }
-            const double fs[2][2] = {{A.gsneg * ssx, A.gsneg * ssy}, {B.gsneg * ssx, B.gsneg * ssy}};
+            forces2_ref(a, rc, nr, nb, rep, fs);
             momentum(p[0], jm[0][0], jm[0][1]);
             momentum(p[1], jm[1][0], jm[1][1]);
-            const double den = ra * A.omega + rb * B.omega;
-            const double ucx = (jm[0][0] * A.omega + jm[1][0] * B.omega) / den;
-            const double ucy = (jm[0][1] * A.omega + jm[1][1] * B.omega) / den;
+            double ucx, ucy;
+            composite_ref(a, jm, ra, rb, ucx, ucy);
 #pragma unroll
             for (int c = 0; c < 2; ++c) {
                 const auto& C = a.c[c];
                 const double r = rc[c];
                 if (!is_dry(r)) {
-                    ue[c][0] = ucx + rep[c][0] * C.inv_omega / r + fs[c][0] * C.inv_omega;
-                    ue[c][1] = ucy + rep[c][1] * C.inv_omega / r + fs[c][1] * C.inv_omega;
+                    ue[c][0] = ueq_ref(C, ucx, rep[c][0], fs[c][0], r);
+                    ue[c][1] = ueq_ref(C, ucy, rep[c][1], fs[c][1], r);
                     if (!REPLAY && fabs(r) < kUeqWatch) {
                         G(C.ueqx)[s_idx] = ue[c][0];
                         G(C.ueqy)[s_idx] = ue[c][1];
@@ -651,8 +717,7 @@ __device__ inline void finish2(const StepArgs<typename ST::T>& a, void* const (&
                 double v[Q];
 #pragma unroll
                 for (int i = 0; i < Q; ++i) {
-                    v[i] = C.omc * p[c][i] + C.omega * e[c][i];
-                    if (C.has_force) v[i] += C.force[i];
+                    v[i] = bgk_ref(C, i, p[c][i], e[c][i], C.has_force);
                     st_at<ST>(dst[c], (OffT)i * plane_b + home, v[i]);
                 }
                 if constexpr (GW) ghost_stores<ST, OffT>(dst[c], plane_b, nbr, nb, v);

@@ -61,7 +61,7 @@ __host__ __device__ constexpr int x3_threads() {
 
 template <class ST, bool OPEN, int K, int T>
 struct XkLds {
-    float d[K][3][2][T];                                 // densities of stage s
+    typename ST::T d[K][3][2][T];                        // densities of stage s
     typename ST::S pu[K - 1][3][2][3][T];                // post-collision, c_y = +1 (2, 5, 6)
     typename ST::S pz[K - 1][2][2][2][T];                // c_y = 0 (1, 3)
     typename ST::S pd[K - 1][OPEN ? 1 : 2][2][3][T];     // c_y = -1 (4, 7, 8)
@@ -98,24 +98,32 @@ struct UeqStage {
 
 // Arguments of one K-step launch: physics, geometry and P_{n-1} (a.c[c].src)
 // from `a` (its equilibrium-velocity pointers are unused); P_{n+K-1} into
-// dst; the equilibrium-velocity sets in u.
-template <int K>
+// dst; the equilibrium-velocity sets in u (deviation forms).  FP64: the
+// launch's number (nonzero) and the word a site below the zero-density
+// rule's watch threshold sets to it (collide2_f64); a launch that finds the
+// word set returns at once.
+template <int K, class R = float>
 struct XkArgs {
-    StepArgs<float> a;
+    StepArgs<R> a;
     void* dst[2];
     UeqRing u;
+    unsigned* flag;
+    unsigned launch;
+    double watch;   // >= kUeqWatch64
 };
 
 // The row schedule of a tile.  Two phases (stage s lags stage 1 by 3(s - 1)
 // rows, two barriers per row) where measured faster: FP32 storage, no
-// solids, three steps (76.7k against 72.8k MLUPS at 4096^2).  Elsewhere the
-// stages follow each other within the row (2K - 1 barriers): there the
+// solids, three steps (76.7k against 72.8k MLUPS at 4096^2), and FP64
+// without solids (two steps: 1312 -> 1257 us per launch at 4096^2; one
+// wave per SIMD, held there by LDS, so its registers are spare).  Elsewhere
+// the stages follow each other within the row (2K - 1 barriers): there the
 // two-phase schedule's longer register lifetimes cost occupancy (FP16 two
 // steps: 72.4k against 82.9k; FP32 three steps with solids: 26.0k against
 // 41.7k).  Both give the same bits.
 template <class ST, bool OPEN, int K>
 __host__ __device__ constexpr bool two_phase() {
-    return sizeof(typename ST::S) == 4 && OPEN && K == 3;
+    return OPEN && ((sizeof(typename ST::S) == 4 && K == 3) || sizeof(typename ST::S) == 8);
 }
 
 // Whether the row loop is unrolled by 6, so that the two register arrays of
@@ -199,14 +207,15 @@ __device__ __attribute__((always_inline)) inline void walk_rows(const int r_firs
 }
 
 template <class ST, bool OPEN, int K, int T, bool FORCE>
-__device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0, const int y1,
+__device__ inline void stepk_tile(const XkArgs<K, typename ST::T>& A, const int bx, const int y0, const int y1,
                                   XkLds<ST, OPEN, K, T>& L) {
     using S = typename ST::S;
+    using R = typename ST::T;   // arithmetic: float (deviation forms) or double
     using OffT = uint32_t;
     constexpr OffT ES = sizeof(S);
     constexpr int H = 2 * K - 1;            // halo columns on each side
     constexpr int OUT = xk_out<ST, K, T>();
-    const StepArgs<float>& a = A.a;
+    const StepArgs<R>& a = A.a;
     const int nx = a.nx, ny = a.ny, wpr = a.wpr;
     const OffT plane_b = (OffT)a.plane * ES, pitch_b = (OffT)a.pitch * ES;
     const uint64_t* __restrict__ mask = a.mask;
@@ -265,7 +274,7 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
     };
     // S_n(X, r) = Gather(P_{n-1}), zero at solid sites (a solid site loads its
     // own cell and keeps zeros: no branch round the loads)
-    auto gather1 = [&](int r, unsigned nb, float (&p)[2][Q]) {
+    auto gather1 = [&](int r, unsigned nb, R (&p)[2][Q]) {
         OffT off[Q];
         pull_offsets<ST, true, OffT>(plane_b, pitch_b, X, Xm, Xp, wr(r), wr(r - 1), wr(r + 1), nb, off);
         const bool fluid = OPEN || (nb & 1u);
@@ -274,8 +283,8 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
         for (int c = 0; c < 2; ++c)
 #pragma unroll
             for (int i = 0; i < Q; ++i) {
-                const float v = ld_at<ST, OffT>(a.c[c].src, fluid ? off[i] : home);
-                p[c][i] = fluid ? v : 0.f;
+                const R v = ld_at<ST, OffT>(a.c[c].src, fluid ? off[i] : home);
+                p[c][i] = fluid ? v : R(0);
             }
     };
     uint64_t mw[3] = {0, 0, 0};   // mask words of row r + 4, loading during row r
@@ -293,7 +302,7 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
             return (unsigned)((mw[0] >> (Xm & 63)) & 1ull) | ((unsigned)((mw[1] >> (X & 63)) & 1ull) << 1) |
                    ((unsigned)((mw[2] >> (Xp & 63)) & 1ull) << 2);
     };
-    auto nbr_dens = [&](const RowAnchor& an, int q, int r, float (&nr)[2][Q], float (&rc)[2]) {
+    auto nbr_dens = [&](const RowAnchor& an, int q, int r, R (&nr)[2][Q], R (&rc)[2]) {
         const int sS = r3(an, r - 1), sC = r3(an, r), sN = r3(an, r + 1);
 #pragma unroll
         for (int i = 1; i < Q; ++i) {
@@ -306,6 +315,16 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
             rc[c] = L.d[q][sC][c][t];
             nr[c][0] = rc[c];
         }
+    };
+
+    // One fluid site's collision in stage q (0-based): the deviation forms'
+    // collide2_core, or the reference's FP64 arithmetic (collide2_f64).
+    auto collide = [&](int q, int s_idx, const R (&p)[2][Q], const R (&rc)[2], const R (&nr)[2][Q], unsigned nb,
+                       R (&oa)[Q], R (&ob)[Q]) {
+        if constexpr (ST::kDev)
+            collide2_core<float, UeqStage, FORCE, ST, OPEN>(a, UeqStage{A.u, q}, s_idx, p, rc, nr, nb, oa, ob);
+        else
+            collide2_f64<FORCE>(a, A.flag, A.launch, A.watch, p, rc, nr, nb, oa, ob);
     };
 
     if constexpr (two_phase<ST, OPEN, K>()) {
@@ -323,11 +342,11 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
         const int load_last = y1 + 2 * K - 2;              // last row loaded
         // S_n populations rotate through three arrays, the rows that stages
         // s >= 2 gather and collide through two (walk_rows)
-        float pX[2][Q], pY[2][Q], pZ[2][Q];
-        float sA[K - 1][2][Q], sB[K - 1][2][Q];
+        R pX[2][Q], pY[2][Q], pZ[2][Q];
+        R sA[K - 1][2][Q], sB[K - 1][2][Q];
         // rest populations of stage s - 1 by the iteration that collided them:
         // [0] this one, [1] one earlier, [2] two earlier (the row gathered now)
-        float rest[K - 1][2][3];
+        R rest[K - 1][2][3];
         // neighbourhood window: nbw[j] = bits of row r + 2 - j
         constexpr int NB = 3 * K;
         unsigned nbw[NB];
@@ -337,9 +356,9 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
         for (int q = 0; q < K - 1; ++q)
 #pragma unroll
             for (int c = 0; c < 2; ++c) {
-                rest[q][c][0] = rest[q][c][1] = rest[q][c][2] = 0.f;
+                rest[q][c][0] = rest[q][c][1] = rest[q][c][2] = R(0);
 #pragma unroll
-                for (int i = 0; i < Q; ++i) sA[q][c][i] = sB[q][c][i] = 0.f;
+                for (int i = 0; i < Q; ++i) sA[q][c][i] = sB[q][c][i] = R(0);
             }
         // prologue: densities of rows r_first - 1, r_first; populations of row
         // r_first (pX); loads of row r_first + 1 (pY) in flight
@@ -361,8 +380,8 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
 
         // sg: stage s >= 2 gathers into it (the row it collides next
         // iteration); sx: the row it collides now, gathered one iteration earlier
-        auto row = [&](auto ph, const int r, float (&pa)[2][Q], float (&pb)[2][Q], float (&pn)[2][Q],
-                       float (&sg)[K - 1][2][Q], float (&sx)[K - 1][2][Q]) {
+        auto row = [&](auto ph, const int r, R (&pa)[2][Q], R (&pb)[2][Q], R (&pn)[2][Q],
+                       R (&sg)[K - 1][2][Q], R (&sx)[K - 1][2][Q]) {
             const RowAnchor an{r, decltype(ph)::value};
             if constexpr (fresh_lane<ST, OPEN, K>()) asm volatile("" : "+v"(t));
             // (A) stage 1: loads of row r + 2 (past the band the last row
@@ -399,7 +418,7 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
 #pragma unroll
                         for (int c = 0; c < 2; ++c)
 #pragma unroll
-                            for (int i = 0; i < Q; ++i) sg[q][c][i] = 0.f;
+                            for (int i = 0; i < Q; ++i) sg[q][c][i] = R(0);
                     }
 #pragma unroll
                     for (int c = 0; c < 2; ++c) L.d[s - 1][r3(an, g)][c][t] = dens(sg[q][c]);
@@ -408,9 +427,9 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
             __syncthreads();
             // (B) stage 1: collide row r into ring 0
             if (r <= r1_last && t >= 1 && t < tl - 1 && (OPEN || (nbw[2] & 1u))) {
-                float nr[2][Q], rc[2], oa[Q], ob[Q];
+                R nr[2][Q], rc[2], oa[Q], ob[Q];
                 nbr_dens(an, 0, r, nr, rc);
-                collide2_core<float, UeqStage, FORCE, ST, OPEN>(a, UeqStage{A.u, 0}, wr(r) * a.pitch + X, pa, rc, nr,
+                collide(0, wr(r) * a.pitch + X, pa, rc, nr,
                                                       OPEN ? 0x1FFu : nbw[2], oa, ob);
 #pragma unroll
                 for (int i = 1; i < Q; ++i) {
@@ -430,9 +449,9 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
                 if (s < K) {
                     if (h >= y0 - 2 * (K - s) && h <= y1 + 2 * (K - s) - 1 && t >= 2 * s - 1 && t < tl - 2 * s + 1 &&
                         (OPEN || (nbh & 1u))) {
-                        float nr[2][Q], rc[2], oa[Q], ob[Q];
+                        R nr[2][Q], rc[2], oa[Q], ob[Q];
                         nbr_dens(an, s - 1, h, nr, rc);
-                        collide2_core<float, UeqStage, FORCE, ST, OPEN>(a, UeqStage{A.u, s - 1}, wr(h) * a.pitch + X, sx[q], rc,
+                        collide(s - 1, wr(h) * a.pitch + X, sx[q], rc,
                                                               nr, OPEN ? 0x1FFu : nbh, oa, ob);
 #pragma unroll
                         for (int i = 1; i < Q; ++i) {
@@ -443,9 +462,9 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
                         rest[s - 1][1][0] = ST::dec(ST::enc(ob[0]));
                     }
                 } else if (h >= y0 && h < y1 && out_lane && (OPEN || (nbh & 1u))) {
-                    float nr[2][Q], rc[2], oa[Q], ob[Q];
+                    R nr[2][Q], rc[2], oa[Q], ob[Q];
                     nbr_dens(an, s - 1, h, nr, rc);
-                    collide2_core<float, UeqStage, FORCE, ST, OPEN>(a, UeqStage{A.u, s - 1}, h * a.pitch + X, sx[q], rc, nr,
+                    collide(s - 1, h * a.pitch + X, sx[q], rc, nr,
                                                           OPEN ? 0x1FFu : nbh, oa, ob);
                     const OffT home = (OffT)h * pitch_b + (OffT)X * ES;
 #pragma unroll
@@ -482,9 +501,9 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
         const int load_last = y1 + 2 * K - 2;                          // last row loaded
         // S_n populations rotate through three arrays, the rows that stages
         // s >= 2 gather and collide through two (walk_rows)
-        float pX[2][Q], pY[2][Q], pZ[2][Q];
-        float sA[K - 1][2][Q], sB[K - 1][2][Q];
-        float rest_now[K - 1][2], rest_prev[K - 1][2];
+        R pX[2][Q], pY[2][Q], pZ[2][Q];
+        R sA[K - 1][2][Q], sB[K - 1][2][Q];
+        R rest_now[K - 1][2], rest_prev[K - 1][2];
         // neighbourhood window: nbw[j] = bits of row r + 2 - j
         unsigned nbw[2 * K + 1];
 #pragma unroll
@@ -493,9 +512,9 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
         for (int q = 0; q < K - 1; ++q)
 #pragma unroll
             for (int c = 0; c < 2; ++c) {
-                rest_now[q][c] = rest_prev[q][c] = 0.f;
+                rest_now[q][c] = rest_prev[q][c] = R(0);
 #pragma unroll
-                for (int i = 0; i < Q; ++i) sA[q][c][i] = sB[q][c][i] = 0.f;
+                for (int i = 0; i < Q; ++i) sA[q][c][i] = sB[q][c][i] = R(0);
             }
         // prologue: densities of rows r_first - 1, r_first; populations of row
         // r_first (pX); loads of row r_first + 1 (pY) in flight
@@ -517,8 +536,8 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
 
         // sg: stage s >= 2 gathers into it (the row it collides next
         // iteration); sx: the row it collides now, gathered one iteration earlier
-        auto row = [&](auto ph, const int r, float (&pa)[2][Q], float (&pb)[2][Q], float (&pn)[2][Q],
-                       float (&sg)[K - 1][2][Q], float (&sx)[K - 1][2][Q]) {
+        auto row = [&](auto ph, const int r, R (&pa)[2][Q], R (&pb)[2][Q], R (&pn)[2][Q],
+                       R (&sg)[K - 1][2][Q], R (&sx)[K - 1][2][Q]) {
             const RowAnchor an{r, decltype(ph)::value};
             if constexpr (fresh_lane<ST, OPEN, K>()) asm volatile("" : "+v"(t));
             // (a) stage 1: loads of row r + 2 (the last row reloads load_last,
@@ -534,9 +553,9 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
             __syncthreads();
             // stage 1: collide row r into ring 0
             if (t >= 1 && t < tl - 1 && (OPEN || (nbw[2] & 1u))) {
-                float nr[2][Q], rc[2], oa[Q], ob[Q];
+                R nr[2][Q], rc[2], oa[Q], ob[Q];
                 nbr_dens(an, 0, r, nr, rc);
-                collide2_core<float, UeqStage, FORCE, ST, OPEN>(a, UeqStage{A.u, 0}, wr(r) * a.pitch + X, pa, rc, nr, OPEN ? 0x1FFu : nbw[2], oa, ob);
+                collide(0, wr(r) * a.pitch + X, pa, rc, nr, OPEN ? 0x1FFu : nbw[2], oa, ob);
                 if constexpr (K > 1) {
 #pragma unroll
                     for (int i = 1; i < Q; ++i) {
@@ -570,7 +589,7 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
 #pragma unroll
                         for (int c = 0; c < 2; ++c)
 #pragma unroll
-                            for (int i = 0; i < Q; ++i) sg[q][c][i] = 0.f;
+                            for (int i = 0; i < Q; ++i) sg[q][c][i] = R(0);
                     }
 #pragma unroll
                     for (int c = 0; c < 2; ++c) L.d[s - 1][r3(an, g)][c][t] = dens(sg[q][c]);
@@ -581,9 +600,9 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
                 const unsigned nbh = nbw[2 * s];
                 if (s < K) {
                     if (h >= y0 - 2 * (K - s) && t >= 2 * s - 1 && t < tl - 2 * s + 1 && (OPEN || (nbh & 1u))) {
-                        float nr[2][Q], rc[2], oa[Q], ob[Q];
+                        R nr[2][Q], rc[2], oa[Q], ob[Q];
                         nbr_dens(an, s - 1, h, nr, rc);
-                        collide2_core<float, UeqStage, FORCE, ST, OPEN>(a, UeqStage{A.u, s - 1}, wr(h) * a.pitch + X, sx[q], rc, nr, OPEN ? 0x1FFu : nbh, oa, ob);
+                        collide(s - 1, wr(h) * a.pitch + X, sx[q], rc, nr, OPEN ? 0x1FFu : nbh, oa, ob);
 #pragma unroll
                         for (int i = 1; i < Q; ++i) {
                             pr(an, s - 1, h, 0, i, t) = ST::enc(oa[i]);
@@ -594,9 +613,9 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
                     }
                     __syncthreads();
                 } else if (h >= y0 && h < y1 && out_lane && (OPEN || (nbh & 1u))) {
-                    float nr[2][Q], rc[2], oa[Q], ob[Q];
+                    R nr[2][Q], rc[2], oa[Q], ob[Q];
                     nbr_dens(an, s - 1, h, nr, rc);
-                    collide2_core<float, UeqStage, FORCE, ST, OPEN>(a, UeqStage{A.u, s - 1}, h * a.pitch + X, sx[q], rc, nr, OPEN ? 0x1FFu : nbh, oa, ob);
+                    collide(s - 1, h * a.pitch + X, sx[q], rc, nr, OPEN ? 0x1FFu : nbh, oa, ob);
                     const OffT home = (OffT)h * pitch_b + (OffT)X * ES;
 #pragma unroll
                     for (int i = 0; i < Q; ++i) {   // 32-bit offsets: a component's buffer is < 4 GiB
@@ -627,7 +646,11 @@ __device__ inline void stepk_tile(const XkArgs<K>& A, const int bx, const int y0
 // bands of A.a.rows rows, workgroup ids remapped so that each XCD owns a
 // contiguous range of tiles.
 template <class ST, bool OPEN, int K, int T, bool FORCE>
-__device__ inline void stepk_grid(const XkArgs<K>& A, XkLds<ST, OPEN, K, T>& L) {
+__device__ inline void stepk_grid(const XkArgs<K, typename ST::T>& A, XkLds<ST, OPEN, K, T>& L) {
+    // FP64: an earlier launch of the sequence found a site the zero-density
+    // rule needs single steps for; its output is repeated by the host
+    if constexpr (!ST::kDev)
+        if (*G(A.flag) != 0u) return;
     const int nbx = A.a.gx;
     const int nby = (A.a.ny + A.a.rows - 1) / A.a.rows;
     int bid = blockIdx.x;
@@ -660,6 +683,20 @@ template <class ST, bool OPEN, bool FORCE>
 __global__ __launch_bounds__(kX2Threads) void k_step2x(const XkArgs<2> A) {
     __shared__ XkLds<ST, OPEN, 2, kX2Threads> L;
     stepk_grid<ST, OPEN, 2, kX2Threads, FORCE>(A, L);
+}
+
+// Two steps per launch in FP64 (the reference's arithmetic, bit for bit
+// that of two single steps): the K = 2 tile with strips of kX2dThreads
+// columns, kX2dThreads - 6 of them written (FP64 lines of 16 doubles split
+// at strip edges merge in L2: the final stores are cached).
+#ifndef SCL_X2D_THREADS
+#define SCL_X2D_THREADS 256
+#endif
+constexpr int kX2dThreads = SCL_X2D_THREADS;
+template <bool OPEN, bool FORCE>
+__global__ __launch_bounds__(kX2dThreads) void k_step2d(const XkArgs<2, double> A) {
+    __shared__ XkLds<StoreF64, OPEN, 2, kX2dThreads> L;
+    stepk_grid<StoreF64, OPEN, 2, kX2dThreads, FORCE>(A, L);
 }
 
 }  // namespace scl

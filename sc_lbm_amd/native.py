@@ -28,6 +28,7 @@ EXPORTS = [
     "scl_volume_force", "scl_stream1", "scl_stream2", "scl_step1", "scl_step2", "scl_moments", "scl_moments_noalloc",
     "scl_set_tile", "scl_set_lookahead", "scl_get_tile", "scl_autotune", "scl_timing", "scl_probe_copy",
     "scl_set_step_fusion",
+    "scl_set_fp64_fusion_watch",
     "scl_get_step_fusion",
     "scl_laplace_observables", "scl_contact_angle",
     "scl_field_checkpoint_size", "scl_field_checkpoint", "scl_field_restore",
@@ -105,6 +106,7 @@ def load():
         "scl_set_tile": [_vp, ctypes.c_int, ctypes.c_int],
         "scl_set_lookahead": [_vp, ctypes.c_int],
         "scl_set_step_fusion": [_vp, ctypes.c_int, ctypes.c_int],
+        "scl_set_fp64_fusion_watch": [_vp, ctypes.c_double],
         "scl_get_step_fusion": [_vp, ctypes.c_int, _ip, _ip],
         "scl_get_tile": [_vp, ctypes.c_int, ctypes.c_int, _ip, _ip, _ip],
         "scl_autotune": [_vp, ctypes.c_int, ctypes.c_int, _ip, _ip, _ip],
@@ -170,6 +172,11 @@ class Geometry:
         """Time steps per launch (0 automatic, 1, 2 or 3) and rows per band of
         the multi-step kernels (0 automatic)."""
         check(load().scl_set_step_fusion(self.h, int(steps_per_launch), int(rows)))
+
+    def set_fp64_fusion_watch(self, rho):
+        """Density below which FP64 steps hand over from two per launch to one
+        (at least 1e-8)."""
+        check(load().scl_set_fp64_fusion_watch(self.h, float(rho)))
 
     def step_fusion(self, precision=FP32_DEV):
         """(steps per launch, band rows) two-component steps of `precision` use."""

@@ -188,7 +188,8 @@ def main(argv=None):
     avg, cnt = nat.timing(0)            # per time step
     spl, band = g.step_fusion(nat.PRECISION[a.form])
     print(json.dumps({"form": a.form, "n": a.n, "porosity": a.porosity, "steps": a.steps,
-                      "kernel": {1: "k_step", 2: "k_step2x", 3: "k_stepk"}[spl], "steps_per_launch": spl, "band_rows": band,
+                      "kernel": {1: "k_step", 2: "k_step2d" if a.form == "fp64" else "k_step2x", 3: "k_stepk"}[spl],
+                      "steps_per_launch": spl, "band_rows": band,
                       "kernel_avg_ms": avg * a.steps / cnt if cnt else None, "launches": cnt,
                       "tile": list(g.tile(2, nat.PRECISION[a.form])),
                       "mlups": a.n * a.n / (avg * 1e-3) / 1e6 if avg else None}))

@@ -338,6 +338,91 @@ def test_two_steps_per_launch_equal_single_steps(prec, name):
         assert np.array_equal(out[0][c], out[2][c])
 
 
+def _fp64_run(case, fusion, steps, watch=None, chunks=None):
+    """FP64 populations after `steps` steps in calls of `chunks` steps, with
+    `fusion` steps per launch, and the number of launches timed."""
+    g = nat.Geometry(case["mask"])
+    g.set_step_fusion(fusion, 24)
+    if watch is not None:
+        g.set_fp64_fusion_watch(watch)
+    fl = [nat.Field(g, case["tau"][c], nat.FP64, 0.0, g_solid=case["g_solid"][c], g_fluid=case["g_fluid"][c])
+          for c in range(2)]
+    for c in range(2):
+        fl[c].set_f(case["f"][c])
+    nat.solid_surface_force(g, fl[0], fl[1])
+    nat.timing(1)
+    for n in chunks or [steps]:
+        nat.step2(g, fl[0], fl[1], case["force"][0], case["force"][1], n)
+    _, launches = nat.timing(0)
+    nat.timing(-1)
+    out = [f.get_f() for f in fl]
+    fused = g.step_fusion(nat.FP64)[0]
+    for f in fl:
+        f.close()
+    g.close()
+    return out, launches, fused
+
+
+@pytest.mark.parametrize("name", ["open", "porous", "dry"])
+def test_fp64_two_steps_per_launch_equal_single_steps(name):
+    """FP64 with two steps per launch (k_step2d, the reference's arithmetic)
+    gives the populations of single steps bit for bit, and so the oracle's:
+    solid-free, porous with adhesion and a force on A, and a lattice with dry
+    sites, where the zero-density rule hands every step back to single
+    launches.  Calls of every length mix both kernels."""
+    if name == "open":
+        case = wl.spinodal(300, G=1.2, seed=4)
+        case["force"] = [None, None]
+    elif name == "porous":
+        case = wl.porous(nx=700, ny=230, porosity=0.75, seed=6, rmin=5, rmax=12)
+    else:
+        case = _dry_case()
+    chunks = [1, 6, 9, 2, 5, 4, 3, 31]
+    want, n1, f1 = _fp64_run(case, 1, sum(chunks), chunks=chunks)
+    got, n2, f2 = _fp64_run(case, 2, sum(chunks), chunks=chunks)
+    assert (f1, f2) == (1, 2)
+    for c in range(2):
+        assert np.all(np.isfinite(got[c]))
+        assert np.array_equal(got[c].view(np.uint64), want[c].view(np.uint64)), c
+    assert n1 == sum(chunks)
+    if name == "dry":
+        assert n2 > n1 // 2   # handed back: launches of single steps
+    else:
+        assert n2 < n1 * 2 // 3
+    if name == "porous":   # the oracle, on the same inputs
+        s = po.System(case["mask"], case["tau"], case["g_fluid"], case["g_solid"])
+        for c in range(2):
+            s.set_f(c, case["f"][c])
+        s.solid_surface_force()
+        s.run(sum(chunks), case["force"][0], case["force"][1])
+        for c in range(2):
+            assert np.array_equal(got[c].reshape(-1), s.get_f(c))
+
+
+def test_fp64_two_steps_hand_over_in_the_middle_of_a_call():
+    """A density falling below the hand-over threshold after some two-step
+    launches (a spinodal lattice separating: its least density passes 0.5
+    at step 38 of 80, threshold 0.5): the launch that met it and those
+    queued after it are repeated one step per launch from its input, bit
+    for bit the single-step result."""
+    case = wl.spinodal(300, G=1.2, seed=8)
+    case["force"] = [None, None]
+    want, n1, _ = _fp64_run(case, 1, 80, chunks=[1, 79])
+    clean, n_clean, _ = _fp64_run(case, 2, 80, chunks=[1, 79])
+    got, n2, _ = _fp64_run(case, 2, 80, watch=0.5, chunks=[1, 79])
+    for c in range(2):
+        assert np.array_equal(clean[c].view(np.uint64), want[c].view(np.uint64))
+        assert np.array_equal(got[c].view(np.uint64), want[c].view(np.uint64))
+    # launches: the bootstrap, 39 two-step launches and the last step
+    # (n_clean = 41); with the hand-over at launch j (1-based) also the
+    # 79 - 2 (j - 1) steps from its input one per launch, 121 - 2 j in all;
+    # j = 1 (a hand-over at the call's first launch) would give 119
+    assert (n1, n_clean) == (80, 41)
+    assert n_clean < n2 < 119 and (121 - n2) % 2 == 0
+    rho = [want[c].reshape(9, -1).sum(0) for c in range(2)]
+    assert min(float(r.min()) for r in rho) < 0.5
+
+
 @pytest.mark.parametrize("prec", [nat.FP32_DEV, nat.FP16_DEV])
 @pytest.mark.parametrize("name", ["spinodal", "porous"])
 def test_multi_step_kernels_at_every_band_height_remainder(prec, name):
