@@ -488,7 +488,7 @@ int x2_occupancy(int precision, bool open) {
                                     : reinterpret_cast<const void*>(&k_step2x<StoreF32, false, false>))
                             : (open ? reinterpret_cast<const void*>(&k_step2x<StoreF16, true, false>)
                                     : reinterpret_cast<const void*>(&k_step2x<StoreF16, false, false>));
-        const int threads = precision == SCL_FP64 ? kX2dThreads : kX2Threads;
+        const int threads = precision != SCL_FP64 ? kX2Threads : (open ? x2d_threads<true>() : x2d_threads<false>());
         int n = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, threads, 0) != hipSuccess || n < 1) n = 1;
         o = n;
@@ -499,7 +499,8 @@ int x2_occupancy(int precision, bool open) {
 // Band height that makes the grid `waves` whole waves of resident workgroups
 // (a partial last wave leaves CUs idle at the end of every launch).
 int x2_wave_rows(const scl_geometry* g, int precision, int waves) {
-    const int out = precision == SCL_FP64       ? xk_out<StoreF64, 2, kX2dThreads>()
+    const int out = precision == SCL_FP64       ? (g->open ? xk_out<StoreF64, 2, x2d_threads<true>()>()
+                                                           : xk_out<StoreF64, 2, x2d_threads<false>()>())
                     : precision == SCL_FP32_DEV ? x2_out<StoreF32>()
                                                 : x2_out<StoreF16>();
     const int strips = (g->nx + out - 1) / out;
@@ -657,7 +658,8 @@ int launch_x3(const scl_geometry* g, scl_field* const* fl, const double* const* 
 // writes its number into g_lib.f64_flag, later launches return at once, and
 // fused_steps repeats the steps from that launch's input one per launch.
 bool x2d_supported(const scl_geometry* g) {
-    return g->nx >= kX2dThreads && g->ny >= xk_min_rows<2>() && !needs_wide(g, sizeof(double));
+    return g->nx >= (g->open ? x2d_threads<true>() : x2d_threads<false>()) && g->ny >= xk_min_rows<2>() &&
+           !needs_wide(g, sizeof(double));
 }
 
 bool x2d_wanted(const scl_geometry* g) {
@@ -671,7 +673,8 @@ bool x2d_ok(const scl_geometry* g, scl_field* const* fl, int nc) {
 }
 
 int launch_x2d(const scl_geometry* g, scl_field* const* fl, const double* const* forces, unsigned launch) {
-    constexpr int OUT = xk_out<StoreF64, 2, kX2dThreads>();
+    const int T = g->open ? x2d_threads<true>() : x2d_threads<false>();
+    const int OUT = g->open ? xk_out<StoreF64, 2, x2d_threads<true>()>() : xk_out<StoreF64, 2, x2d_threads<false>()>();
     const int rows = x2_rows_for(g, SCL_FP64);
     XkArgs<2, double> A{};
     A.a = make_args<double>(g, rows, 4);
@@ -691,7 +694,7 @@ int launch_x2d(const scl_geometry* g, scl_field* const* fl, const double* const*
         HIPCHK(hipEventRecord(e0, g_lib.stream));
     }
     const bool force = forces[0] || forces[1];
-    const dim3 grid(nblk), block(kX2dThreads);
+    const dim3 grid(nblk), block(T);
     if (g->open) {
         if (force) hipLaunchKernelGGL((k_step2d<true, true>), grid, block, 0, g_lib.stream, A);
         else hipLaunchKernelGGL((k_step2d<true, false>), grid, block, 0, g_lib.stream, A);

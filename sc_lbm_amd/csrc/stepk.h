@@ -686,17 +686,27 @@ __global__ __launch_bounds__(kX2Threads) void k_step2x(const XkArgs<2> A) {
 }
 
 // Two steps per launch in FP64 (the reference's arithmetic, bit for bit
-// that of two single steps): the K = 2 tile with strips of kX2dThreads
-// columns, kX2dThreads - 6 of them written (FP64 lines of 16 doubles split
-// at strip edges merge in L2: the final stores are cached).
-#ifndef SCL_X2D_THREADS
-#define SCL_X2D_THREADS 256
-#endif
-constexpr int kX2dThreads = SCL_X2D_THREADS;
+// that of two single steps): the K = 2 tile with strips of x2d_threads
+// columns, 6 fewer of them written (FP64 lines of 16 doubles split at strip
+// edges merge in L2: the final stores are cached).  Without solids 192
+// columns (66 KB of LDS, two workgroups per CU) held to 256 registers (two
+// waves per SIMD; it fits without spilling): 4096^2 with bands of 32 rows,
+// 1220 us per launch against 1305 us for one workgroup of 256 columns per
+// CU.  With solids 256 columns, one wave per SIMD (at 256 registers that
+// variant spills; porous 0.8: 1474 us against 1507 us).
+template <bool OPEN>
+__host__ __device__ constexpr int x2d_threads() {
+    return OPEN ? 192 : 256;
+}
+template <bool OPEN>
+__host__ __device__ constexpr int x2d_waves_per_eu() {
+    return OPEN ? 2 : 1;
+}
 template <bool OPEN, bool FORCE>
-__global__ __launch_bounds__(kX2dThreads) void k_step2d(const XkArgs<2, double> A) {
-    __shared__ XkLds<StoreF64, OPEN, 2, kX2dThreads> L;
-    stepk_grid<StoreF64, OPEN, 2, kX2dThreads, FORCE>(A, L);
+__global__ __launch_bounds__(x2d_threads<OPEN>()) __attribute__((amdgpu_waves_per_eu(x2d_waves_per_eu<OPEN>()))) void
+k_step2d(const XkArgs<2, double> A) {
+    __shared__ XkLds<StoreF64, OPEN, 2, x2d_threads<OPEN>()> L;
+    stepk_grid<StoreF64, OPEN, 2, x2d_threads<OPEN>(), FORCE>(A, L);
 }
 
 }  // namespace scl
