@@ -357,10 +357,12 @@ typedef struct scl_server_config {
     double mass_tol;            /* watchdog: |mass - mass0| > mass_tol*|mass0| fails a job;
                                    0 = by precision (1e-6 FP64, 1e-3 FP32, 0.1 FP16) */
     int max_bypass;             /* later jobs admitted past a waiting one before it blocks admission */
-    int steps_per_launch;       /* jobs at least 256 columns wide advance 2 steps per launch
-                                   (same bits as 1) with a deviation form; 0 = automatic: 2 with
-                                   FP32 storage, 1 with FP16 (two measured slower there) */
-    int band_rows;              /* rows per band of the two-step tiles; 0 = 32            */
+    int steps_per_launch;       /* deviation forms: jobs of at least 8 columns and 2K + 2 rows
+                                   advance K = 2 or 3 steps per launch (same bits as 1): 1, 2
+                                   (two), 3 (three, and two for jobs with two steps left in a
+                                   round); 0 = automatic: 3 */
+    int band_rows;              /* rows per band of the multi-step tiles; 0 = 32 (two-step),
+                                   64 (three-step) */
     int use_graphs;             /* 1: each round replays a captured HIP graph; 0: the same
                                    launches issued one by one (profilers, debugging)      */
 } scl_server_config;
@@ -378,9 +380,9 @@ typedef struct scl_server_stats {
     int resident, queued, max_resident;
     size_t arena_bytes, arena_used, arena_peak;
     int tile_cols, tile_rows, lookahead, round_steps, workgroups;
-    int steps_per_launch;       /* 2 when wide jobs take two steps per launch          */
+    int steps_per_launch;       /* the most steps per launch jobs take (1, 2 or 3)     */
     long long fluid_updates;    /* lattice updates of fluid sites only                 */
-    long long pair_updates;     /* lattice updates made by two-step launches           */
+    long long pair_updates;     /* lattice updates made by multi-step launches         */
 } scl_server_stats;
 
 /* Defaults: FP32 deviations, 8 GiB arena, 64 slots, 4096 pending, rounds of

@@ -16,14 +16,16 @@
 // both, so consecutive launches (and replays of a captured graph) start from
 // zero without a memset.
 //
-// Two steps per launch (FP32 deviations, jobs at least 256 columns wide with
-// two or more steps in the round): k_serve2x takes tiles of a second list
-// (strips of x2_out columns) and computes each with the K = 2 tile of
-// stepk_tile, the body of the single-lattice two-step kernel; launch q of a
-// round advances steps 2q and 2q+1 of such a job.  An odd last step goes
-// through k_serve.  A pair reads one buffer and writes the other, so after q
-// pairs the job's populations are in buffer cur ^ (q & 1).  Rounds in which
-// no two-step job has a body force launch the force-free variant.
+// Two or three steps per launch (deviation forms, jobs of at least 8 columns
+// and 2K + 2 rows with K or more steps in the round): k_serve2x / k_serve3x
+// take tiles of a second / third list (strips of the K-step tile's output
+// columns) and compute each with stepk_tile, the body of the single-lattice
+// multi-step kernels; launch q of a round advances steps Kq .. Kq + K - 1 of
+// such a job.  The steps past the last whole group of K (one for an odd
+// two-step job, one or two for a three-step job) go through k_serve.  A
+// launch reads one buffer and writes the other, so after q launches the
+// job's populations are in buffer cur ^ (q & 1).  Rounds in which no such
+// job has a body force launch the force-free variants.
 #pragma once
 #include "../host/serve_ic.h"
 #include "convert.h"
@@ -42,11 +44,12 @@ struct JobArgs {
     int cur;
     int steps;                  // steps of this job in the current round
     int x2;                     // 1: steps 2q, 2q+1 go through k_serve2x (q < steps / 2)
+    int x3;                     // 1: steps 3q .. 3q+2 go through k_serve3x (q < steps / 3)
     int fin;                    // buffer holding the populations at the end of the round
     int item_begin, item_end;   // its tiles in the work-list
     // deviation forms: rotating equilibrium-velocity buffers [k][component]
-    // [x/y]; step s reads buffer (ucur + s) % 3 and writes the next
-    double* ueq[3][2][2];
+    // [x/y]; step s reads buffer (ucur + s) % kUeqRing and writes the next
+    double* ueq[kUeqRing][2][2];
     int ucur;
     UeqRing ring;               // the same buffers as a ring (cur = ucur)
 };
@@ -60,10 +63,12 @@ struct ServeHdr {
     int bw;                     // tile width in columns
     const uint64_t* items2;     // two-step tiles (strips of x2_out columns)
     int n_items2;
+    const uint64_t* items3;     // three-step tiles (strips of xk_out<ST, 3, x3_threads> columns)
+    int n_items3;
     unsigned* flags;            // per job: the zero-density rule's flags of both components
     // per step s of the round, three ints P, B, E: k_serve's launch s takes
     // items [0, P) (the single-step jobs with more than s steps) and [B, E)
-    // (the two-step jobs whose odd last step is s)
+    // (the multi-step jobs that take step s singly)
     const int* sched;
 };
 
@@ -107,11 +112,12 @@ k_serve(const ServeHdr* __restrict__ hdr, int s, unsigned* __restrict__ ctr) {
         const SCL_AS_CONST JobArgs<T>& J = jobs[j];
         if (s >= J.steps) continue;
         if (J.x2 && s != J.steps - 1) continue;   // the pairs are k_serve2x's
+        if (J.x3 && s < J.steps / 3 * 3) continue;   // the triples are k_serve3x's
         StepArgs<T> a = J.a;
 #pragma unroll
         for (int c = 0; c < 2; ++c) a.c[c].step = J.a.c[c].step + (unsigned)s;
         if constexpr (ST::kDev) {
-            const int ki = (J.ucur + s) % 3, ko = (J.ucur + s + 1) % 3;
+            const int ki = (J.ucur + s) % kUeqRing, ko = (J.ucur + s + 1) % kUeqRing;
 #pragma unroll
             for (int c = 0; c < 2; ++c) {
                 a.c[c].ueqx_in = J.ueq[ki][c][0];
@@ -122,8 +128,9 @@ k_serve(const ServeHdr* __restrict__ hdr, int s, unsigned* __restrict__ ctr) {
         }
         Bufs b;
         // buffer read at step s: one swap per launch (after s / 2 pairs for a
-        // two-step job)
-        const int sw = (J.cur ^ (J.x2 ? (s >> 1) : s)) & 1;
+        // two-step job, after steps / 3 triples and s - 3 (steps / 3) single
+        // steps for a three-step one)
+        const int sw = (J.cur ^ (J.x2 ? (s >> 1) : (J.x3 ? s - 2 * (J.steps / 3) : s))) & 1;
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
             b.src[c] = sw ? a.c[c].dst : a.c[c].src;
@@ -180,6 +187,55 @@ __global__ __launch_bounds__(kX2Threads) void k_serve2x(const ServeHdr* __restri
             A.u.cur[c] = cur;
         }
         stepk_tile<ST, false, 2, kX2Threads, FORCE>(A, bx, y0, y1, lds);
+    }
+    if (threadIdx.x == 0) {
+        const unsigned d = __hip_atomic_fetch_add(ctr + 1, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (d == gridDim.x - 1) {
+            __hip_atomic_store(ctr, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(ctr + 1, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+// Launch q of a round: steps 3q .. 3q+2 of every three-step job that has
+// them (the tile with solids, strips of x3_threads columns).  FORCE = false:
+// no three-step job of the round has a body force.
+template <class ST, bool FORCE>
+__global__ __launch_bounds__((x3_threads<ST, false>())) void k_serve3x(const ServeHdr* __restrict__ hdr, int q,
+                                                                     unsigned* __restrict__ ctr) {
+    constexpr int T3 = x3_threads<ST, false>();
+    __shared__ XkLds<ST, false, 3, T3> lds;
+    __shared__ int ticket;
+    const int n = hdr->n_items3;
+    const SCL_AS_CONST JobArgs<float>* jobs = K((const JobArgs<float>*)hdr->jobs);
+    const SCL_AS_CONST uint64_t* items = K(hdr->items3);
+    for (;;) {
+        __syncthreads();   // the previous tile is done with `ticket` and the LDS
+        if (threadIdx.x == 0)
+            ticket = (int)__hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        const int t = __builtin_amdgcn_readfirstlane(ticket);
+        if (t >= n) break;
+        int j, bx, y0, y1;
+        unpack_item(items[t], j, bx, y0, y1);
+        j = __builtin_amdgcn_readfirstlane(j);
+        bx = __builtin_amdgcn_readfirstlane(bx);
+        y0 = __builtin_amdgcn_readfirstlane(y0);
+        y1 = __builtin_amdgcn_readfirstlane(y1);
+        const SCL_AS_CONST JobArgs<float>& J = jobs[j];
+        if (3 * q + 2 >= J.steps) continue;
+        XkArgs<3> A;
+        A.a = J.a;
+        A.u = J.ring;
+        const int rb = (J.cur ^ q) & 1;
+        const int cur = (J.ucur + 3 * q) % J.ring.nsets;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            A.a.c[c].src = rb ? J.a.c[c].dst : J.a.c[c].src;
+            A.dst[c] = rb ? const_cast<void*>(J.a.c[c].src) : J.a.c[c].dst;
+            A.u.cur[c] = cur;
+        }
+        stepk_tile<ST, false, 3, T3, FORCE>(A, bx, y0, y1, lds);
     }
     if (threadIdx.x == 0) {
         const unsigned d = __hip_atomic_fetch_add(ctr + 1, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);

@@ -62,8 +62,8 @@ size_t align256(size_t n) { return (n + 255) / 256 * 256; }
 struct Layout {
     int pitch = 0, wpr = 0;
     long long plane = 0;
-    size_t mask = 0, f[2][2] = {}, ueq[3][2][2] = {}, rho[2] = {}, stamp[2] = {}, stage = 0, total = 0;
-    int nueq = 1;   // equilibrium-velocity buffer sets: 1 (FP64) or 3 (deviation forms, rotating)
+    size_t mask = 0, f[2][2] = {}, ueq[kUeqRing][2][2] = {}, rho[2] = {}, stamp[2] = {}, stage = 0, total = 0;
+    int nueq = 1;   // equilibrium-velocity buffer sets: 1 (FP64) or kUeqRing (deviation forms, rotating)
     bool has_stage = false;
 };
 
@@ -81,7 +81,7 @@ Layout layout_of(const JobSpec& s, size_t es) {
     L.mask = take((size_t)s.ny * L.wpr * 8);
     for (int c = 0; c < 2; ++c)
         for (int b = 0; b < 2; ++b) L.f[c][b] = take((size_t)Q * L.plane * es);
-    L.nueq = es == 8 ? 1 : 3;
+    L.nueq = es == 8 ? 1 : kUeqRing;
     for (int k = 0; k < L.nueq; ++k)
         for (int c = 0; c < 2; ++c)
             for (int d = 0; d < 2; ++d) L.ueq[k][c][d] = take((size_t)L.plane * 8);
@@ -130,9 +130,10 @@ class GpuServe final : public serve::Backend {
         for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
         geo_.release();
         for (void* p : {(void*)arena_, (void*)hdr_d_, (void*)ctr_d_, table_d_, (void*)items_d_, (void*)partial_d_,
-                        (void*)mass_d_, (void*)items2_d_, (void*)flags_d_, (void*)sched_d_})
+                        (void*)mass_d_, (void*)items2_d_, (void*)items3_d_, (void*)flags_d_, (void*)sched_d_})
             if (p) (void)hipFree(p);
-        for (void* p : {(void*)hdr_h_, table_h_, (void*)items_h_, (void*)mass_h_, (void*)items2_h_, (void*)flags_h_,
+        for (void* p : {(void*)hdr_h_, table_h_, (void*)items_h_, (void*)mass_h_, (void*)items2_h_, (void*)items3_h_,
+                        (void*)flags_h_,
                         (void*)sched_h_})
             if (p) (void)hipHostFree(p);
         for (hipEvent_t e : ev_)
@@ -203,18 +204,26 @@ class GpuServe final : public serve::Backend {
         }
         rc = occupancy(nw_, la_, &grid_, err);
         if (rc != SCL_OK) return rc;
-        if (cfg_.steps_per_launch < 0 || cfg_.steps_per_launch > 2 || cfg_.band_rows < 0 || cfg_.band_rows > 4096) {
-            *err = "steps_per_launch is 0, 1 or 2 and band_rows 0..4096";
+        if (cfg_.steps_per_launch < 0 || cfg_.steps_per_launch > 3 || cfg_.band_rows < 0 || cfg_.band_rows > 4096) {
+            *err = "steps_per_launch is 0, 1, 2 or 3 and band_rows 0..4096";
             return SCL_ERR_ARG;
         }
         if constexpr (ST::kDev) {
-            // two steps per launch: the deviation forms (automatic: FP32)
-            x2_ = cfg_.steps_per_launch == 2 || (cfg_.steps_per_launch == 0 && std::is_same<ST, StoreF32>::value);
+            // multi-step launches: the deviation forms.  Automatic: three
+            // per launch (porous FP32 4096^2 at porosity 0.8: 58.7k against
+            // 46.5k MLUPS for two); jobs too short for three take two.
+            x3_ = cfg_.steps_per_launch == 3 || cfg_.steps_per_launch == 0;
+            x2_ = cfg_.steps_per_launch >= 2 || cfg_.steps_per_launch == 0;
             x2rows_ = cfg_.band_rows ? cfg_.band_rows : 32;
+            x3rows_ = cfg_.band_rows ? cfg_.band_rows : 64;
             int per_cu = 0;
             SHIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
                 &per_cu, reinterpret_cast<const void*>(&k_serve2x<ST, true>), kX2Threads, 0));
             grid2_ = std::max(1, per_cu) * cus_;
+            per_cu = 0;
+            SHIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per_cu, reinterpret_cast<const void*>(&k_serve3x<ST, true>), x3_threads<ST, false>(), 0));
+            grid3_ = std::max(1, per_cu) * cus_;
         }
         SHIP(hipStreamSynchronize(stream_));
         return SCL_OK;
@@ -239,40 +248,53 @@ class GpuServe final : public serve::Backend {
     int run_round(const std::vector<Job*>& jobs, const std::vector<int>& steps, int rsteps,
                   std::string* err) override {
         SHIP(hipSetDevice(dev_));
-        // two-step jobs: two steps in the round, at least kX2MinCols columns
-        // (a job narrower than a strip takes one strip, whose lanes past its
-        // columns and halos idle) and xk_min_rows<2>() rows
-        std::vector<char> two(jobs.size(), 0);
-        for (size_t k = 0; k < jobs.size(); ++k)
-            two[k] = x2_ && rsteps >= 2 && steps[k] >= 2 && jobs[k]->spec.nx >= kX2MinCols &&
-                     jobs[k]->spec.ny >= xk_min_rows<2>();
+        // multi-step jobs: K steps per launch (three where enabled, else two)
+        // with at least K steps in the round, kXkMinCols columns (a job
+        // narrower than a strip takes one strip, whose lanes past its columns
+        // and halos idle) and xk_min_rows<K>() rows
+        std::vector<char> two(jobs.size(), 0), three(jobs.size(), 0);
+        for (size_t k = 0; k < jobs.size(); ++k) {
+            const JobSpec& sp = jobs[k]->spec;
+            three[k] = x3_ && rsteps >= 3 && steps[k] >= 3 && sp.nx >= kXkMinCols && sp.ny >= xk_min_rows<3>();
+            two[k] = !three[k] && x2_ && rsteps >= 2 && steps[k] >= 2 && sp.nx >= kXkMinCols &&
+                     sp.ny >= xk_min_rows<2>();
+        }
+        // Steps a multi-step job takes singly (through k_serve): the tail
+        // [tail0, steps) past its last whole group of K.
+        auto tail0 = [&](int k) { return three[k] ? steps[k] / 3 * 3 : (two[k] ? steps[k] / 2 * 2 : 0); };
+        auto tail = [&](int k) { return steps[k] - tail0(k); };
         // work-list order: (0) the single-step jobs, most steps first; (1) the
-        // two-step jobs with an odd step count, fewest steps first; (2) the
-        // other two-step jobs (their tiles only serve the mass reduction).
-        // k_serve's launch s then takes a prefix of group 0 and the group-1
-        // jobs whose last step is s, both contiguous.
+        // multi-step jobs with a tail, by (first tail step, tail length); (2)
+        // the other multi-step jobs (their tiles only serve the mass
+        // reduction).  k_serve's launch s then takes a prefix of group 0 and
+        // the group-1 jobs whose tail holds s, both contiguous: tails are 1 or
+        // 2 steps long, and those holding s are the ones starting at s - 1
+        // with length 2 followed by those starting at s.
         std::vector<int> order(jobs.size());
         for (size_t k = 0; k < order.size(); ++k) order[k] = (int)k;
-        auto group = [&](int k) { return !two[k] ? 0 : ((steps[k] & 1) ? 1 : 2); };
+        auto group = [&](int k) { return !(two[k] || three[k]) ? 0 : (tail(k) ? 1 : 2); };
         std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
             if (group(a) != group(b)) return group(a) < group(b);
             if (group(a) == 0 && steps[a] != steps[b]) return steps[a] > steps[b];
-            if (group(a) == 1 && steps[a] != steps[b]) return steps[a] < steps[b];
+            if (group(a) == 1 && tail0(a) != tail0(b)) return tail0(a) < tail0(b);
+            if (group(a) == 1 && tail(a) != tail(b)) return tail(a) < tail(b);
             return jobs[a]->sites() > jobs[b]->sites();
         });
         int* sched = sched_h_;
         for (int s = 0; s < 3 * rsteps; ++s) sched[s] = 0;
-        size_t need = 0, need2 = 0;
+        size_t need = 0, need2 = 0, need3 = 0;
         for (size_t k = 0; k < jobs.size(); ++k) {
             need += tiles_of(jobs[k]->spec.nx, jobs[k]->spec.ny, nw_, rows_);
             if (two[k]) need2 += tiles2_of(jobs[k]->spec.nx, jobs[k]->spec.ny, x2rows_);
+            if (three[k]) need3 += tiles3_of(jobs[k]->spec.nx, jobs[k]->spec.ny, x3rows_);
         }
         int rc = grow_items(need, err);
-        if (rc == SCL_OK) rc = grow_items2(need2, err);
+        if (rc == SCL_OK) rc = grow_list(need2, &items2_d_, &items2_h_, &items2_cap_, err);
+        if (rc == SCL_OK) rc = grow_list(need3, &items3_d_, &items3_h_, &items3_cap_, err);
         if (rc != SCL_OK) return rc;
         JobArgs<T>* table = (JobArgs<T>*)table_h_;
-        int n = 0, n2 = 0;
-        bool force2 = false;   // a two-step job of the round has a body force
+        int n = 0, n2 = 0, n3 = 0;
+        bool force2 = false, force3 = false;   // a two- / three-step job of the round has a body force
         for (int k : order) {
             const GJob* g = (const GJob*)jobs[k]->dev;
             JobArgs<T>& ja = table[k];
@@ -280,7 +302,8 @@ class GpuServe final : public serve::Backend {
             ja.cur = g->cur;
             ja.steps = steps[k];
             ja.x2 = two[k];
-            ja.fin = g->cur ^ (launches(steps[k], two[k]) & 1);
+            ja.x3 = three[k];
+            ja.fin = g->cur ^ (launches(steps[k], per_launch(two[k], three[k])) & 1);
             fill_ueq(ja, *g);
             ja.item_begin = n;
             n = emit_tiles(k, g->nx, g->ny, nw_, rows_, items_h_, n);
@@ -288,33 +311,42 @@ class GpuServe final : public serve::Backend {
             if (group(k) == 0) {
                 for (int s = 0; s < steps[k]; ++s) sched[3 * s] = n;
             } else if (group(k) == 1) {
-                const int s = steps[k] - 1;
-                if (sched[3 * s + 2] == 0) sched[3 * s + 1] = ja.item_begin;
-                sched[3 * s + 2] = n;
+                for (int s = tail0(k); s < steps[k]; ++s) {
+                    if (sched[3 * s + 2] == 0) sched[3 * s + 1] = ja.item_begin;
+                    sched[3 * s + 2] = n;
+                }
             }
+            const bool force = jobs[k]->spec.has_force[0] || jobs[k]->spec.has_force[1];
             if (two[k]) {
-                n2 = emit_tiles2(k, g->nx, g->ny, x2rows_, items2_h_, n2);
-                force2 = force2 || jobs[k]->spec.has_force[0] || jobs[k]->spec.has_force[1];
+                n2 = emit_tiles_k(k, g->nx, g->ny, x2rows_, kOut2, items2_h_, n2);
+                force2 = force2 || force;
+            }
+            if (three[k]) {
+                n3 = emit_tiles_k(k, g->nx, g->ny, x3rows_, kOut3, items3_h_, n3);
+                force3 = force3 || force;
             }
         }
         *hdr_h_ = ServeHdr{table_d_, items_d_, partial_d_, mass_d_, n, (int)jobs.size(), 64 * nw_,
-                           items2_d_, n2, flags_d_, sched_d_};
+                           items2_d_, n2, items3_d_, n3, flags_d_, sched_d_};
         SHIP(hipMemcpyAsync(sched_d_, sched_h_, (size_t)3 * rsteps * sizeof(int), hipMemcpyHostToDevice, stream_));
         SHIP(hipMemcpyAsync(table_d_, table_h_, jobs.size() * sizeof(JobArgs<T>), hipMemcpyHostToDevice, stream_));
         SHIP(hipMemcpyAsync(items_d_, items_h_, (size_t)n * sizeof(uint64_t), hipMemcpyHostToDevice, stream_));
         if (n2) SHIP(hipMemcpyAsync(items2_d_, items2_h_, (size_t)n2 * sizeof(uint64_t), hipMemcpyHostToDevice,
                                     stream_));
+        if (n3) SHIP(hipMemcpyAsync(items3_d_, items3_h_, (size_t)n3 * sizeof(uint64_t), hipMemcpyHostToDevice,
+                                    stream_));
         SHIP(hipMemcpyAsync(hdr_d_, hdr_h_, sizeof(ServeHdr), hipMemcpyHostToDevice, stream_));
         hipGraphExec_t ge = nullptr;
+        const Multi mk{n2 > 0, force2, n3 > 0, force3};
         if (cfg_.use_graphs) {
-            rc = graph(rsteps, n2 > 0, force2, &ge, err);
+            rc = graph(rsteps, mk, &ge, err);
             if (rc != SCL_OK) return rc;
         }
         SHIP(hipEventRecord(ev_[0], stream_));
         if (ge) {
             SHIP(hipGraphLaunch(ge, stream_));
         } else {
-            launch_round(rsteps, n2 > 0, force2);
+            launch_round(rsteps, mk);
             SHIP(hipGetLastError());
         }
         SHIP(hipEventRecord(ev_[1], stream_));
@@ -328,10 +360,10 @@ class GpuServe final : public serve::Backend {
             GJob* g = (GJob*)jobs[k]->dev;
             jobs[k]->mass[0] = mass_h_[2 * k];
             jobs[k]->mass[1] = mass_h_[2 * k + 1];
-            g->cur ^= launches(steps[k], two[k]) & 1;
-            g->ucur = (g->ucur + steps[k]) % 3;
+            g->cur ^= launches(steps[k], per_launch(two[k], three[k])) & 1;
+            g->ucur = (g->ucur + steps[k]) % g->L.nueq;
             g->nstep += (unsigned)steps[k];
-            if (two[k]) pair_updates_ += (long long)(steps[k] & ~1) * g->nx * g->ny;
+            if (two[k] || three[k]) pair_updates_ += (long long)tail0(k) * g->nx * g->ny;
             if (flags_h_[k])
                 jobs[k]->device_error = "zero-density rule: a site's density fell from >= 1e-8 to <= 1e-16 "
                                         "within one step without its previous equilibrium velocity held";
@@ -379,13 +411,13 @@ class GpuServe final : public serve::Backend {
         *la = la_;
     }
     int workgroups() const override { return grid_; }
-    int steps_per_launch() const override { return x2_ ? 2 : 1; }
+    int steps_per_launch() const override { return x3_ ? 3 : (x2_ ? 2 : 1); }
     double device_ms() const override { return device_ms_.load(); }
     long long pair_updates() const override { return pair_updates_.load(); }
 
   private:
     static void fill_ueq(JobArgs<T>& ja, const GJob& g) {
-        for (int k = 0; k < 3; ++k)
+        for (int k = 0; k < kUeqRing; ++k)
             for (int c = 0; c < 2; ++c)
                 for (int d = 0; d < 2; ++d) ja.ueq[k][c][d] = g.ueq(k < g.L.nueq ? k : 0, c, d);
         ja.ucur = g.ucur;
@@ -407,36 +439,43 @@ class GpuServe final : public serve::Backend {
         return (size_t)((nx + 64 * nw - 1) / (64 * nw)) * (size_t)((ny + rows - 1) / rows);
     }
 
-    // Launches that swap a job's buffers in a round of `steps` steps.
-    static int launches(int steps, bool two) { return two ? steps / 2 + (steps & 1) : steps; }
+    // Steps per launch of a job in a round, and the launches that swap its
+    // buffers in a round of `steps` steps: steps / K groups and the rest
+    // one at a time.
+    static int per_launch(bool two, bool three) { return three ? 3 : (two ? 2 : 1); }
+    static int launches(int steps, int k) { return steps / k + steps % k; }
 
-    // Narrowest job of the two-step kernel: a strip's halo lanes wrap at
-    // most once (2 * K - 1 = 3 columns each side).
-    static constexpr int kX2MinCols = 8;
-    // output columns of a two-step strip in this storage form (the tile's)
+    // Narrowest job of the multi-step kernels: a strip's halo lanes wrap at
+    // most once (2 K - 1 <= 5 columns each side).
+    static constexpr int kXkMinCols = 8;
+    // output columns of a two- / three-step strip in this storage form (the tiles')
     static constexpr int kOut2 = x2_out<ST>();
+    static constexpr int kOut3 = xk_out<ST, 3, x3_threads<ST, false>()>();
     static size_t tiles2_of(int nx, int ny, int rows) {
         return (size_t)((nx + kOut2 - 1) / kOut2) * (size_t)((ny + rows - 1) / rows);
     }
-
-    // Two-step tiles of job k, band by band.
-    static int emit_tiles2(int k, int nx, int ny, int rows, uint64_t* out, int n) {
-        const int gx = (nx + kOut2 - 1) / kOut2;
+    static size_t tiles3_of(int nx, int ny, int rows) {
+        return (size_t)((nx + kOut3 - 1) / kOut3) * (size_t)((ny + rows - 1) / rows);
+    }
+    static int emit_tiles_k(int k, int nx, int ny, int rows, int out_cols, uint64_t* out, int n) {
+        const int gx = (nx + out_cols - 1) / out_cols;
         for (int y0 = 0; y0 < ny; y0 += rows)
             for (int bx = 0; bx < gx; ++bx) out[n++] = pack_item(k, bx, y0, std::min(y0 + rows, ny));
         return n;
     }
 
-    int grow_items2(size_t need, std::string* err) {
-        if (need <= items2_cap_) return SCL_OK;
-        const size_t cap = std::max<size_t>(items2_cap_ * 2, need);
-        if (items2_d_) SHIP(hipFree(items2_d_));
-        if (items2_h_) SHIP(hipHostFree(items2_h_));
-        items2_d_ = nullptr;
-        items2_h_ = nullptr;
-        SHIP(hipMalloc(&items2_d_, cap * sizeof(uint64_t)));
-        SHIP(hipHostMalloc(&items2_h_, cap * sizeof(uint64_t), hipHostMallocDefault));
-        items2_cap_ = cap;
+    // A multi-step tile list (device and pinned host copies) of at least
+    // `need` entries.
+    int grow_list(size_t need, uint64_t** dev, uint64_t** host, size_t* cap_io, std::string* err) {
+        if (need <= *cap_io) return SCL_OK;
+        const size_t cap = std::max<size_t>(*cap_io * 2, need);
+        if (*dev) SHIP(hipFree(*dev));
+        if (*host) SHIP(hipHostFree(*host));
+        *dev = nullptr;
+        *host = nullptr;
+        SHIP(hipMalloc(dev, cap * sizeof(uint64_t)));
+        SHIP(hipHostMalloc(host, cap * sizeof(uint64_t), hipHostMallocDefault));
+        *cap_io = cap;
         return SCL_OK;
     }
 
@@ -656,18 +695,26 @@ class GpuServe final : public serve::Backend {
         return SCL_OK;
     }
 
-    // The round as a graph: rsteps single-step launches, or (two) per pair q
-    // a two-step launch then the single-step launches of steps 2q and 2q+1;
-    // force: the two-step launches take the body-force variant.
-    int graph(int rsteps, bool two, bool force, hipGraphExec_t* out, std::string* err) {
-        const int key = 4 * rsteps + (two ? 1 : 0) + (two && force ? 2 : 0);
+    // The multi-step launches of a round: two- / three-step jobs present, and
+    // whether any of them has a body force.
+    struct Multi {
+        bool two, force2, three, force3;
+    };
+
+    // The round as a graph: rsteps single-step launches, with (two) per pair
+    // q a two-step launch before the single-step launch of step 2q and
+    // (three) per triple q a three-step launch before that of step 3q; the
+    // force flags pick the body-force variants.
+    int graph(int rsteps, const Multi& mk, hipGraphExec_t* out, std::string* err) {
+        const int key = 16 * rsteps + (mk.two ? 1 : 0) + (mk.two && mk.force2 ? 2 : 0) + (mk.three ? 4 : 0) +
+                        (mk.three && mk.force3 ? 8 : 0);
         auto it = graphs_.find(key);
         if (it != graphs_.end()) {
             *out = it->second;
             return SCL_OK;
         }
         SHIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-        launch_round(rsteps, two, force);
+        launch_round(rsteps, mk);
         hipGraph_t gr = nullptr;
         hipError_t e = hipStreamEndCapture(stream_, &gr);
         if (e != hipSuccess) {
@@ -687,14 +734,26 @@ class GpuServe final : public serve::Backend {
     }
 
     // The launches of a round: `rsteps` steps (pairs of steps through
-    // k_serve2x where `two`), then the per-job FP64 mass.
-    void launch_round(int rsteps, bool two, bool force) {
+    // k_serve2x, triples through k_serve3x where such jobs are present),
+    // then the per-job FP64 mass.
+    void launch_round(int rsteps, const Multi& mk) {
         for (int s = 0; s < rsteps; ++s) {
-            if (two && !(s & 1) && s + 1 < rsteps) launch_pair(s / 2, force);
+            if (mk.three && s % 3 == 0 && s + 2 < rsteps) launch_triple(s / 3, mk.force3);
+            if (mk.two && !(s & 1) && s + 1 < rsteps) launch_pair(s / 2, mk.force2);
             launch_step(nw_, la_, grid_, s);
         }
         hipLaunchKernelGGL(k_serve_mass<ST>, dim3(mass_grid_), dim3(256), 0, stream_, hdr_d_);
         hipLaunchKernelGGL(k_serve_mass_final<T>, dim3(cfg_.max_jobs), dim3(256), 0, stream_, hdr_d_);
+    }
+
+    void launch_triple(int q, bool force) {
+        if constexpr (ST::kDev) {
+            constexpr int T3 = x3_threads<ST, false>();
+            if (force)
+                hipLaunchKernelGGL((k_serve3x<ST, true>), dim3(grid3_), dim3(T3), 0, stream_, hdr_d_, q, ctr_d_);
+            else
+                hipLaunchKernelGGL((k_serve3x<ST, false>), dim3(grid3_), dim3(T3), 0, stream_, hdr_d_, q, ctr_d_);
+        }
     }
 
     void launch_pair(int q, bool force) {
@@ -780,6 +839,7 @@ class GpuServe final : public serve::Backend {
                         table[k].cur = g->cur;
                         table[k].steps = 10;
                         table[k].x2 = 0;
+                        table[k].x3 = 0;
                         table[k].fin = g->cur;
                         fill_ueq(table[k], *g);
                         table[k].item_begin = n;
@@ -787,7 +847,7 @@ class GpuServe final : public serve::Backend {
                         table[k].item_end = n;
                     }
                     *hdr_h_ = ServeHdr{table_d_, items_d_, partial_d_, mass_d_, n, 2, 64 * nw, items2_d_, 0,
-                                       flags_d_, sched_d_};
+                                       items3_d_, 0, flags_d_, sched_d_};
                     for (int s = 0; s < 10; ++s) {
                         sched_h_[3 * s] = n;
                         sched_h_[3 * s + 1] = sched_h_[3 * s + 2] = 0;
@@ -857,11 +917,16 @@ class GpuServe final : public serve::Backend {
     unsigned* flags_d_ = nullptr;   // zero-density rule flag per job of the round
     unsigned* flags_h_ = nullptr;
     int nw_ = 4, rows_ = 16, la_ = 1, grid_ = 0, mass_grid_ = 1024;
-    bool x2_ = false;                  // two steps per launch for wide jobs (deviation forms)
+    bool x2_ = false;                  // two steps per launch (deviation forms)
     int x2rows_ = 32, grid2_ = 0;      // band rows of the two-step tiles; its persistent grid
     uint64_t* items2_d_ = nullptr;
     uint64_t* items2_h_ = nullptr;
     size_t items2_cap_ = 0;
+    bool x3_ = false;                  // three steps per launch (deviation forms)
+    int x3rows_ = 64, grid3_ = 0;      // band rows of the three-step tiles; its persistent grid
+    uint64_t* items3_d_ = nullptr;
+    uint64_t* items3_h_ = nullptr;
+    size_t items3_cap_ = 0;
     std::map<int, hipGraphExec_t> graphs_;
     hipEvent_t ev_[2] = {nullptr, nullptr};
     std::atomic<double> device_ms_{0.0};   // written by the request loop, read by stats

@@ -51,14 +51,14 @@ def dens_ref(f):
 
 def footprint(nx, ny, es, stage):
     """Device bytes of one resident job (the backend's block layout: mask,
-    2 x 2 population buffers, equilibrium-velocity planes (4 in FP64, 3 x 4
+    2 x 2 population buffers, equilibrium-velocity planes (4 in FP64, 4 x 4
     rotating in the deviation forms), 2 density planes in FP64, in FP64 the
     zero-density rule's records (a 32-bit word per site and 2 more per
     component), optional FP64 staging; each region 256-byte aligned)."""
     a = lambda n: (n + 255) // 256 * 256
     pitch = (nx + 63) // 64 * 64
     plane = pitch * ny
-    nueq = 1 if es == 8 else 3
+    nueq = 1 if es == 8 else 4
     total = a(ny * pitch // 64 * 8) + 4 * a(9 * plane * es) + (4 * nueq + 2) * a(plane * 8)
     if es == 8:
         total += 2 * a((plane + 2) * 4)
@@ -225,20 +225,21 @@ def test_autotuned_server_runs_and_reports_its_shape():
 
 
 @pytest.mark.parametrize("form", ["fp32dev", "fp16dev"])
-def test_two_steps_per_launch_in_the_server_keep_results(form):
-    """Deviation forms: jobs advance two steps per launch of the persistent
-    kernel; an odd step count and a round shorter than two steps take single
-    steps.  The result is the same bits with one step per launch, in a mix,
-    and on the single-lattice path."""
+@pytest.mark.parametrize("k", [2, 3])
+def test_two_steps_per_launch_in_the_server_keep_results(form, k):
+    """Deviation forms: jobs advance k steps per launch of the persistent
+    kernel; the steps past the last whole group of k in a round (rounds of
+    16 and 5 steps) take single steps.  The result is the same bits with one
+    step per launch, in a mix, and on the single-lattice path."""
     wide = dict(POROUS, nx=300, ny=120, steps=151, geom_seed=4, ic_seed=6, deliver=serve.DELIVER_F)
-    two = run_alone(wide, form, round_steps=16, steps_per_launch=2)
+    two = run_alone(wide, form, round_steps=16, steps_per_launch=k)
     one = run_alone(wide, form, round_steps=16, steps_per_launch=1)
-    odd = run_alone(wide, form, round_steps=5, band_rows=17, steps_per_launch=2)
+    odd = run_alone(wide, form, round_steps=5, band_rows=17, steps_per_launch=k)
     for c in range(2):
         assert np.array_equal(two[c], one[c])
         assert np.array_equal(two[c], odd[c])
-    with gpu_server(form, steps_per_launch=2) as srv:
-        assert srv.stats()["steps_per_launch"] == 2
+    with gpu_server(form, steps_per_launch=k) as srv:
+        assert srv.stats()["steps_per_launch"] == k
         t0 = srv.stats()["clock_ms"]
         others = [dict(nx=257, ny=64, steps=90, ic_seed=3, radius=12, **DROP),
                   dict(nx=128, ny=200, steps=61, ic_seed=4, radius=20, **DROP),
@@ -265,15 +266,17 @@ def test_two_steps_per_launch_in_the_server_keep_results(form):
 
 
 @pytest.mark.parametrize("form", ["fp32dev", "fp16dev"])
-@pytest.mark.parametrize("nx", [8, 37, 128, 224, 225, 255])
-def test_narrow_jobs_in_two_step_strips(form, nx):
-    """A job narrower than a two-step strip (224 output columns in FP32, 250
-    in FP16) takes one strip, or a full one and a narrow one, whose lanes
-    past the job's columns and halos idle: the same bits as one step per
-    launch and as the single-lattice path's single steps."""
+@pytest.mark.parametrize("nx", [8, 37, 128, 192, 193, 224, 225, 255])
+@pytest.mark.parametrize("k", [2, 3])
+def test_narrow_jobs_in_two_step_strips(form, nx, k):
+    """A job narrower than a k-step strip (two steps: 224 output columns in
+    FP32, 250 in FP16; three: 192, 246) takes one strip, or a full one and a
+    narrow one, whose lanes past the job's columns and halos idle: the same
+    bits as one step per launch and as the single-lattice path's single
+    steps."""
     job = dict(POROUS, nx=nx, ny=96, steps=41, rmin=2, rmax=5, radius=min(nx, 96) / 4.0, geom_seed=5,
                ic_seed=9, deliver=serve.DELIVER_F)
-    two = run_alone(job, form, round_steps=16, steps_per_launch=2)
+    two = run_alone(job, form, round_steps=16, steps_per_launch=k)
     one = run_alone(job, form, round_steps=16, steps_per_launch=1)
     for c in range(2):
         assert np.array_equal(two[c], one[c])
@@ -288,6 +291,29 @@ def test_narrow_jobs_in_two_step_strips(form, nx):
     nat.step2(g, fl[0], fl[1], case["force"][0], case["force"][1], job["steps"])
     for c in range(2):
         assert np.array_equal(fl[c].get_f().reshape(-1), two[c].reshape(-1))
+
+
+def test_three_step_server_takes_two_steps_for_a_two_step_round():
+    """With three steps per launch, a job whose last round has two steps left
+    (34 steps in rounds of 16) takes them as one two-step launch, beside
+    three-step jobs of the same round: the same bits as one step per
+    launch."""
+    short = dict(POROUS, nx=260, ny=90, steps=34, geom_seed=7, ic_seed=2, deliver=serve.DELIVER_F)
+    longer = dict(POROUS, nx=300, ny=100, steps=47, geom_seed=8, ic_seed=3, deliver=serve.DELIVER_F)
+    got, want = [], []
+    for k, out in ((3, got), (1, want)):
+        with gpu_server("fp32dev", round_steps=16, steps_per_launch=k) as srv:
+            ids = [srv.submit(**short), srv.submit(**longer)]
+            for i in ids:
+                assert srv.wait(i, 60000)["state"] == "done"
+            out.extend(srv.result(i, serve.F, c) for i in ids for c in range(2))
+            if k == 3:
+                st = srv.stats()
+                # multi-step launches: 34 = (15 + 1) + (15 + 1) + 2 steps and
+                # 47 = (15 + 1) + (15 + 1) + 15, singles in brackets' tails
+                assert st["pair_updates"] == 32 * 260 * 90 + 45 * 300 * 100
+    for a, b in zip(got, want):
+        assert np.array_equal(a, b)
 
 
 def test_reference_density_of_the_deviation_forms():
