@@ -386,7 +386,8 @@ typedef struct scl_server_stats {
 } scl_server_stats;
 
 /* Defaults: FP32 deviations, 8 GiB arena, 64 slots, 4096 pending, rounds of
- * 32 steps, tile timed at start-up (16 rows), mass_tol by precision,
+ * 36 steps (a multiple of the two- and three-step launches' 2 and 3, so full
+ * rounds take no single steps), tile timed at start-up (16 rows), mass_tol by precision,
  * max_bypass 16, graphs on. */
 int scl_server_default_config(scl_server_config* cfg);
 /* Creates the server on the library's device and starts its request loop. */

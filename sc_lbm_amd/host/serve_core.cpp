@@ -906,7 +906,7 @@ int scl_server_default_config(scl_server_config* cfg) {
     cfg->budget_bytes = (size_t)8 << 30;
     cfg->max_jobs = 64;
     cfg->max_pending = 4096;
-    cfg->round_steps = 32;
+    cfg->round_steps = 36;   // a multiple of 2 and 3: no single-step tails in full rounds
     cfg->mass_tol = 0.0;   // by precision
     cfg->max_bypass = 16;
     cfg->use_graphs = 1;
