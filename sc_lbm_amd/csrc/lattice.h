@@ -108,6 +108,8 @@ struct StoreF64 {
     __device__ static inline T ld(const S* p) { return *G(p); }
     __device__ static inline void st(S* p, T v) { stream_store<S>(p, v); }
     __device__ static inline void stc(S* p, T v) { *G(p) = v; }
+    __device__ static inline void se(S* p, S e) { stream_store<S>(p, e); }   // store an encoded value
+    __device__ static inline void sce(S* p, S e) { *G(p) = e; }
     __device__ static inline S enc(T v) { return v; }   // value as stored
     __device__ static inline T dec(S s) { return s; }   // stored value as loaded
     __device__ static inline T rnd(T v) { return v; }   // v rounded to what the store keeps
@@ -121,6 +123,8 @@ struct StoreF32 {
     __device__ static inline T ld(const S* p) { return *G(p); }
     __device__ static inline void st(S* p, T v) { stream_store<S>(p, v); }
     __device__ static inline void stc(S* p, T v) { *G(p) = v; }
+    __device__ static inline void se(S* p, S e) { stream_store<S>(p, e); }   // store an encoded value
+    __device__ static inline void sce(S* p, S e) { *G(p) = e; }
     __device__ static inline S enc(T v) { return v; }
     __device__ static inline T dec(S s) { return s; }
     __device__ static inline T rnd(T v) { return v; }
@@ -134,6 +138,8 @@ struct StoreF16 {
     __device__ static inline T ld(const S* p) { return (float)(*G(p)) * kHalfInv; }
     __device__ static inline void st(S* p, T v) { stream_store<S>(p, (_Float16)(v * kHalfScale)); }
     __device__ static inline void stc(S* p, T v) { *G(p) = (_Float16)(v * kHalfScale); }
+    __device__ static inline void se(S* p, S e) { stream_store<S>(p, e); }   // store an encoded value
+    __device__ static inline void sce(S* p, S e) { *G(p) = e; }
     __device__ static inline S enc(T v) { return (_Float16)(v * kHalfScale); }
     __device__ static inline T dec(S s) { return (float)s * kHalfInv; }
     __device__ static inline T rnd(T v) { return dec(enc(v)); }

@@ -242,9 +242,11 @@ __device__ inline void feq_ref(double rho, double ux, double uy, double (&e)[Q])
 // rounding of d_1'..d_8' was a mass error, and where a population changes by
 // less than half its FP16 spacing the rounding has the same sign every step
 // (a 4096^2 spinodal lattice lost 0.45% of its mass in 1000 steps that way).
+// enc: the outputs as the storage form encodes them (their stores and the
+// closure share the one encoding).
 template <class ST>
 __device__ inline void relax_dev(const float (&p)[Q], float rho, float drho, float ux, float uy, float omega,
-                                 const float* force, bool has_force, float (&out)[Q]) {
+                                 const float* force, bool has_force, float (&out)[Q], typename ST::S (&enc)[Q]) {
     // Every fused multiply-add is written out (the file is built without
     // contraction): the compiler's own choice of which products to fuse
     // could differ between the kernels this is inlined into, and the
@@ -262,10 +264,14 @@ __device__ inline void relax_dev(const float (&p)[Q], float rho, float drho, flo
             ni += force[i];
             nj += force[j];
         }
-        const float oi = ST::rnd(fmaf(omega, ni, p[i]));
-        const float oj = ST::rnd(fmaf(omega, nj, p[j]));
+        const typename ST::S ei = ST::enc(fmaf(omega, ni, p[i]));
+        const typename ST::S ej = ST::enc(fmaf(omega, nj, p[j]));
+        const float oi = ST::dec(ei);   // ST::rnd of the relaxed value
+        const float oj = ST::dec(ej);
         out[i] = oi;
         out[j] = oj;
+        enc[i] = ei;
+        enc[j] = ej;
         r += (p[i] - oi) + (p[j] - oj);
     };
     pair(1, 3, ux, 1.0f / 9.0f, wr1);
@@ -273,6 +279,7 @@ __device__ inline void relax_dev(const float (&p)[Q], float rho, float drho, flo
     pair(5, 7, ux + uy, 1.0f / 36.0f, wr2);
     pair(6, 8, uy - ux, 1.0f / 36.0f, wr2);
     out[0] = has_force ? (p[0] + r) + force[0] : p[0] + r;
+    enc[0] = ST::enc(out[0]);
 }
 
 // Reciprocal for the deviation forms (hardware approximation, 1 ulp).
@@ -391,9 +398,10 @@ __device__ inline void finish1(const CompArgs<typename ST::T>& ca, void* dst, lo
             momentum(p, jx, jy);
             const float inv = rcp(rho);
             float out[Q];
-            relax_dev<ST>(p, rho, drho, jx * inv, jy * inv, ca.omega, ca.force, ca.has_force, out);
+            typename ST::S enc[Q];
+            relax_dev<ST>(p, rho, drho, jx * inv, jy * inv, ca.omega, ca.force, ca.has_force, out, enc);
 #pragma unroll
-            for (int i = 0; i < Q; ++i) st_at<ST>(dst, (OffT)i * plane_b + home, out[i]);
+            for (int i = 0; i < Q; ++i) ST::se((typename ST::S*)((char*)dst + ((OffT)i * plane_b + home)), enc[i]);
             if constexpr (GW) ghost_stores<ST, OffT>(dst, plane_b, nbr, nb, out);
         }
     }
@@ -435,18 +443,19 @@ __device__ inline UeqPtrs ueq_ptrs(const StepArgs<T>& a) {
 template <class T, class U, bool FORCE = true, class ST = StoreF32, bool OPEN = false>
 __device__ inline void collide2_core(const StepArgs<T>& a, const U& u, int s_idx, const float (&p)[2][Q],
                                      const float (&rc)[2], const float (&nr)[2][Q], unsigned nb, float (&oa)[Q],
-                                     float (&ob)[Q]);
+                                     float (&ob)[Q], typename ST::S (&ea)[Q], typename ST::S (&eb)[Q]);
 
 template <class T, class ST, bool OPEN = false>
 __device__ inline void collide2_dev(const StepArgs<T>& a, int s_idx, const float (&p)[2][Q], const float (&rc)[2],
-                                    const float (&nr)[2][Q], unsigned nb, float (&oa)[Q], float (&ob)[Q]) {
-    collide2_core<T, UeqPtrs, true, ST, OPEN>(a, ueq_ptrs(a), s_idx, p, rc, nr, nb, oa, ob);
+                                    const float (&nr)[2][Q], unsigned nb, float (&oa)[Q], float (&ob)[Q],
+                                    typename ST::S (&ea)[Q], typename ST::S (&eb)[Q]) {
+    collide2_core<T, UeqPtrs, true, ST, OPEN>(a, ueq_ptrs(a), s_idx, p, rc, nr, nb, oa, ob, ea, eb);
 }
 
 template <class T, class U, bool FORCE, class ST, bool OPEN>
 __device__ inline void collide2_core(const StepArgs<T>& a, const U& u, int s_idx, const float (&p)[2][Q],
                                      const float (&rc)[2], const float (&nr)[2][Q], unsigned nb, float (&oa)[Q],
-                                     float (&ob)[Q]) {
+                                     float (&ob)[Q], typename ST::S (&ea)[Q], typename ST::S (&eb)[Q]) {
     // fused multiply-adds written out, as in relax_dev
     const auto& A = a.c[0];
     const auto& B = a.c[1];
@@ -541,8 +550,8 @@ __device__ inline void collide2_core(const StepArgs<T>& a, const U& u, int s_idx
             G(u.oy(1))[s_idx] = ueb[1];
         }
     }
-    relax_dev<ST>(p[0], ra, da0, uea[0], uea[1], A.omega, A.force, FORCE && A.has_force, oa);
-    relax_dev<ST>(p[1], rb, db0, ueb[0], ueb[1], B.omega, B.force, FORCE && B.has_force, ob);
+    relax_dev<ST>(p[0], ra, da0, uea[0], uea[1], A.omega, A.force, FORCE && A.has_force, oa, ea);
+    relax_dev<ST>(p[1], rb, db0, ueb[0], ueb[1], B.omega, B.force, FORCE && B.has_force, ob, eb);
 }
 
 // The reference's two-component arithmetic in FP64, operation by operation
@@ -727,11 +736,12 @@ __device__ inline void finish2(const StepArgs<typename ST::T>& a, void* const (&
         // deviation forms, FP32 arithmetic
         if (!REPLAY && fl) {
             float oa[Q], ob[Q];
-            collide2_dev<typename ST::T, ST, OPEN>(a, s_idx, p, rc, nr, nb, oa, ob);
+            typename ST::S ea[Q], eb[Q];
+            collide2_dev<typename ST::T, ST, OPEN>(a, s_idx, p, rc, nr, nb, oa, ob, ea, eb);
 #pragma unroll
             for (int i = 0; i < Q; ++i) {
-                st_at<ST>(dst[0], (OffT)i * plane_b + home, oa[i]);
-                st_at<ST>(dst[1], (OffT)i * plane_b + home, ob[i]);
+                ST::se((typename ST::S*)((char*)dst[0] + ((OffT)i * plane_b + home)), ea[i]);
+                ST::se((typename ST::S*)((char*)dst[1] + ((OffT)i * plane_b + home)), eb[i]);
             }
             if constexpr (GW) {
                 ghost_stores<ST, OffT>(dst[0], plane_b, nbr, nb, oa);

@@ -319,12 +319,20 @@ __device__ inline void stepk_tile(const XkArgs<K, typename ST::T>& A, const int 
 
     // One fluid site's collision in stage q (0-based): the deviation forms'
     // collide2_core, or the reference's FP64 arithmetic (collide2_f64).
+    // oa / ob: the outputs; ea / eb: the same as the storage form encodes them
     auto collide = [&](int q, int s_idx, const R (&p)[2][Q], const R (&rc)[2], const R (&nr)[2][Q], unsigned nb,
-                       R (&oa)[Q], R (&ob)[Q]) {
-        if constexpr (ST::kDev)
-            collide2_core<float, UeqStage, FORCE, ST, OPEN>(a, UeqStage{A.u, q}, s_idx, p, rc, nr, nb, oa, ob);
-        else
+                       R (&oa)[Q], R (&ob)[Q], S (&ea)[Q], S (&eb)[Q]) {
+        if constexpr (ST::kDev) {
+            collide2_core<float, UeqStage, FORCE, ST, OPEN>(a, UeqStage{A.u, q}, s_idx, p, rc, nr, nb, oa, ob, ea,
+                                                            eb);
+        } else {
             collide2_f64<FORCE>(a, A.flag, A.launch, A.watch, p, rc, nr, nb, oa, ob);
+#pragma unroll
+            for (int i = 0; i < Q; ++i) {
+                ea[i] = oa[i];
+                eb[i] = ob[i];
+            }
+        }
     };
 
     if constexpr (two_phase<ST, OPEN, K>()) {
@@ -428,16 +436,17 @@ __device__ inline void stepk_tile(const XkArgs<K, typename ST::T>& A, const int 
             // (B) stage 1: collide row r into ring 0
             if (r <= r1_last && t >= 1 && t < tl - 1 && (OPEN || (nbw[2] & 1u))) {
                 R nr[2][Q], rc[2], oa[Q], ob[Q];
+                S ea[Q], eb[Q];
                 nbr_dens(an, 0, r, nr, rc);
                 collide(0, wr(r) * a.pitch + X, pa, rc, nr,
-                                                      OPEN ? 0x1FFu : nbw[2], oa, ob);
+                                                      OPEN ? 0x1FFu : nbw[2], oa, ob, ea, eb);
 #pragma unroll
                 for (int i = 1; i < Q; ++i) {
-                    pr(an, 0, r, 0, i, t) = ST::enc(oa[i]);
-                    pr(an, 0, r, 1, i, t) = ST::enc(ob[i]);
+                    pr(an, 0, r, 0, i, t) = ea[i];
+                    pr(an, 0, r, 1, i, t) = eb[i];
                 }
-                rest[0][0][0] = ST::dec(ST::enc(oa[0]));
-                rest[0][1][0] = ST::dec(ST::enc(ob[0]));
+                rest[0][0][0] = ST::dec(ea[0]);
+                rest[0][1][0] = ST::dec(eb[0]);
             }
             // (B) stage s >= 2: collide row h = r - 3(s - 1) from the populations
             // gathered one iteration earlier
@@ -450,28 +459,30 @@ __device__ inline void stepk_tile(const XkArgs<K, typename ST::T>& A, const int 
                     if (h >= y0 - 2 * (K - s) && h <= y1 + 2 * (K - s) - 1 && t >= 2 * s - 1 && t < tl - 2 * s + 1 &&
                         (OPEN || (nbh & 1u))) {
                         R nr[2][Q], rc[2], oa[Q], ob[Q];
+                S ea[Q], eb[Q];
                         nbr_dens(an, s - 1, h, nr, rc);
                         collide(s - 1, wr(h) * a.pitch + X, sx[q], rc,
-                                                              nr, OPEN ? 0x1FFu : nbh, oa, ob);
+                                                              nr, OPEN ? 0x1FFu : nbh, oa, ob, ea, eb);
 #pragma unroll
                         for (int i = 1; i < Q; ++i) {
-                            pr(an, s - 1, h, 0, i, t) = ST::enc(oa[i]);
-                            pr(an, s - 1, h, 1, i, t) = ST::enc(ob[i]);
+                            pr(an, s - 1, h, 0, i, t) = ea[i];
+                            pr(an, s - 1, h, 1, i, t) = eb[i];
                         }
-                        rest[s - 1][0][0] = ST::dec(ST::enc(oa[0]));
-                        rest[s - 1][1][0] = ST::dec(ST::enc(ob[0]));
+                        rest[s - 1][0][0] = ST::dec(ea[0]);
+                        rest[s - 1][1][0] = ST::dec(eb[0]);
                     }
                 } else if (h >= y0 && h < y1 && out_lane && (OPEN || (nbh & 1u))) {
                     R nr[2][Q], rc[2], oa[Q], ob[Q];
+                S ea[Q], eb[Q];
                     nbr_dens(an, s - 1, h, nr, rc);
                     collide(s - 1, h * a.pitch + X, sx[q], rc, nr,
-                                                          OPEN ? 0x1FFu : nbh, oa, ob);
+                                                          OPEN ? 0x1FFu : nbh, oa, ob, ea, eb);
                     const OffT home = (OffT)h * pitch_b + (OffT)X * ES;
 #pragma unroll
                     for (int i = 0; i < Q; ++i) {   // 32-bit offsets: a component's buffer is < 4 GiB
                         const OffT o = home + (OffT)i * plane_b;
-                        ST::stc((S*)((char*)A.dst[0] + o), oa[i]);
-                        ST::stc((S*)((char*)A.dst[1] + o), ob[i]);
+                        ST::sce((S*)((char*)A.dst[0] + o), ea[i]);
+                        ST::sce((S*)((char*)A.dst[1] + o), eb[i]);
                     }
                 }
             }
@@ -554,16 +565,17 @@ __device__ inline void stepk_tile(const XkArgs<K, typename ST::T>& A, const int 
             // stage 1: collide row r into ring 0
             if (t >= 1 && t < tl - 1 && (OPEN || (nbw[2] & 1u))) {
                 R nr[2][Q], rc[2], oa[Q], ob[Q];
+                S ea[Q], eb[Q];
                 nbr_dens(an, 0, r, nr, rc);
-                collide(0, wr(r) * a.pitch + X, pa, rc, nr, OPEN ? 0x1FFu : nbw[2], oa, ob);
+                collide(0, wr(r) * a.pitch + X, pa, rc, nr, OPEN ? 0x1FFu : nbw[2], oa, ob, ea, eb);
                 if constexpr (K > 1) {
 #pragma unroll
                     for (int i = 1; i < Q; ++i) {
-                        pr(an, 0, r, 0, i, t) = ST::enc(oa[i]);
-                        pr(an, 0, r, 1, i, t) = ST::enc(ob[i]);
+                        pr(an, 0, r, 0, i, t) = ea[i];
+                        pr(an, 0, r, 1, i, t) = eb[i];
                     }
-                    rest_now[0][0] = ST::dec(ST::enc(oa[0]));
-                    rest_now[0][1] = ST::dec(ST::enc(ob[0]));
+                    rest_now[0][0] = ST::dec(ea[0]);
+                    rest_now[0][1] = ST::dec(eb[0]);
                 }
             }
             __syncthreads();
@@ -601,27 +613,29 @@ __device__ inline void stepk_tile(const XkArgs<K, typename ST::T>& A, const int 
                 if (s < K) {
                     if (h >= y0 - 2 * (K - s) && t >= 2 * s - 1 && t < tl - 2 * s + 1 && (OPEN || (nbh & 1u))) {
                         R nr[2][Q], rc[2], oa[Q], ob[Q];
+                S ea[Q], eb[Q];
                         nbr_dens(an, s - 1, h, nr, rc);
-                        collide(s - 1, wr(h) * a.pitch + X, sx[q], rc, nr, OPEN ? 0x1FFu : nbh, oa, ob);
+                        collide(s - 1, wr(h) * a.pitch + X, sx[q], rc, nr, OPEN ? 0x1FFu : nbh, oa, ob, ea, eb);
 #pragma unroll
                         for (int i = 1; i < Q; ++i) {
-                            pr(an, s - 1, h, 0, i, t) = ST::enc(oa[i]);
-                            pr(an, s - 1, h, 1, i, t) = ST::enc(ob[i]);
+                            pr(an, s - 1, h, 0, i, t) = ea[i];
+                            pr(an, s - 1, h, 1, i, t) = eb[i];
                         }
-                        rest_now[s - 1][0] = ST::dec(ST::enc(oa[0]));
-                        rest_now[s - 1][1] = ST::dec(ST::enc(ob[0]));
+                        rest_now[s - 1][0] = ST::dec(ea[0]);
+                        rest_now[s - 1][1] = ST::dec(eb[0]);
                     }
                     __syncthreads();
                 } else if (h >= y0 && h < y1 && out_lane && (OPEN || (nbh & 1u))) {
                     R nr[2][Q], rc[2], oa[Q], ob[Q];
+                S ea[Q], eb[Q];
                     nbr_dens(an, s - 1, h, nr, rc);
-                    collide(s - 1, h * a.pitch + X, sx[q], rc, nr, OPEN ? 0x1FFu : nbh, oa, ob);
+                    collide(s - 1, h * a.pitch + X, sx[q], rc, nr, OPEN ? 0x1FFu : nbh, oa, ob, ea, eb);
                     const OffT home = (OffT)h * pitch_b + (OffT)X * ES;
 #pragma unroll
                     for (int i = 0; i < Q; ++i) {   // 32-bit offsets: a component's buffer is < 4 GiB
                         const OffT o = home + (OffT)i * plane_b;
-                        ST::stc((S*)((char*)A.dst[0] + o), oa[i]);
-                        ST::stc((S*)((char*)A.dst[1] + o), ob[i]);
+                        ST::sce((S*)((char*)A.dst[0] + o), ea[i]);
+                        ST::sce((S*)((char*)A.dst[1] + o), eb[i]);
                     }
                 }
             }
