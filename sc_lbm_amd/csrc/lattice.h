@@ -100,10 +100,19 @@ __device__ inline void stream_store(S* p, S v) {
 }
 
 // st: streamed store; stc: cached store (for scattered partial-line writes).
+// Working units (the fused kernels' arithmetic on the populations): the
+// stored value decoded without its scale, i.e. the deviation times kUnit, a
+// power of two, so every product and sum in those units is the one on the
+// deviations scaled exactly (ldw / decw / encw; densities, velocities and
+// forces stay in plain units).
 struct StoreF64 {
     using S = double;
     using T = double;
     static constexpr bool kDev = false;
+    static constexpr T kUnit = 1.0;
+    __device__ static inline T ldw(const S* p) { return *G(p); }
+    __device__ static inline T decw(S s) { return s; }
+    __device__ static inline S encw(T v) { return v; }
     static constexpr T kUeqWatch = kUeqWatch64;
     __device__ static inline T ld(const S* p) { return *G(p); }
     __device__ static inline void st(S* p, T v) { stream_store<S>(p, v); }
@@ -119,6 +128,10 @@ struct StoreF32 {
     using S = float;
     using T = float;
     static constexpr bool kDev = true;
+    static constexpr T kUnit = 1.0f;
+    __device__ static inline T ldw(const S* p) { return *G(p); }
+    __device__ static inline T decw(S s) { return s; }
+    __device__ static inline S encw(T v) { return v; }
     static constexpr T kUeqWatch = 1e-5f;
     __device__ static inline T ld(const S* p) { return *G(p); }
     __device__ static inline void st(S* p, T v) { stream_store<S>(p, v); }
@@ -134,6 +147,10 @@ struct StoreF16 {
     using S = _Float16;
     using T = float;
     static constexpr bool kDev = true;
+    static constexpr T kUnit = kHalfScale;   // the stored value itself: one conversion each way
+    __device__ static inline T ldw(const S* p) { return (float)(*G(p)); }
+    __device__ static inline T decw(S s) { return (float)s; }
+    __device__ static inline S encw(T v) { return (_Float16)v; }
     static constexpr T kUeqWatch = 2e-3f;
     __device__ static inline T ld(const S* p) { return (float)(*G(p)) * kHalfInv; }
     __device__ static inline void st(S* p, T v) { stream_store<S>(p, (_Float16)(v * kHalfScale)); }

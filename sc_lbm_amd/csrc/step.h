@@ -142,6 +142,11 @@ template <class ST, class OffT>
 __device__ inline typename ST::T ld_at(const void* base, OffT off) {
     return ST::ld((const typename ST::S*)((const char*)base + off));
 }
+// in working units (ST::kUnit)
+template <class ST, class OffT>
+__device__ inline typename ST::T ldw_at(const void* base, OffT off) {
+    return ST::ldw((const typename ST::S*)((const char*)base + off));
+}
 
 template <class ST, class OffT>
 __device__ inline void st_at(void* base, OffT off, typename ST::T v) {
@@ -164,6 +169,14 @@ __device__ inline float dens(const float (&p)[Q]) {
 #pragma unroll
     for (int i = 1; i < Q; ++i) r += p[i];
     return r;
+}
+// The density (deviation) of populations in working units, in plain units:
+// the sum in working units times 1/kUnit, exactly the sum of the decoded
+// values (a power-of-two scale commutes with every rounding of the sum).
+template <class ST>
+__device__ inline typename ST::T dens_w(const typename ST::T (&p)[Q]) {
+    if constexpr (ST::kUnit == typename ST::T(1)) return dens(p);
+    else return dens(p) * (typename ST::T(1) / ST::kUnit);
 }
 
 // Momentum with the reference's Cx / Cy sums; the zero terms (f*0) are
@@ -244,15 +257,18 @@ __device__ inline void feq_ref(double rho, double ux, double uy, double (&e)[Q])
 // (a 4096^2 spinodal lattice lost 0.45% of its mass in 1000 steps that way).
 // enc: the outputs as the storage form encodes them (their stores and the
 // closure share the one encoding).
+// p, out, r and the force terms are in working units (ST::kUnit times the
+// deviations, see lattice.h); the weights carry the scale, exactly.
 template <class ST>
 __device__ inline void relax_dev(const float (&p)[Q], float rho, float drho, float ux, float uy, float omega,
                                  const float* force, bool has_force, float (&out)[Q], typename ST::S (&enc)[Q]) {
+    constexpr float U = ST::kUnit;
     // Every fused multiply-add is written out (the file is built without
     // contraction): the compiler's own choice of which products to fuse
     // could differ between the kernels this is inlined into, and the
     // single-step and two-step kernels must give the same bits.
     const float usq15 = 1.5f * fmaf(ux, ux, uy * uy);
-    const float wr1 = (1.0f / 9.0f) * rho, wr2 = (1.0f / 36.0f) * rho;
+    const float wr1 = ((1.0f / 9.0f) * U) * rho, wr2 = ((1.0f / 36.0f) * U) * rho;
     float r = 0.f;   // sum over i > 0 of (d_i - d_i')
     auto pair = [&](int i, int j, float cu, float w, float wr) {
         const float t = fmaf(4.5f * cu, cu, -usq15);
@@ -261,25 +277,25 @@ __device__ inline void relax_dev(const float (&p)[Q], float rho, float drho, flo
         float ni = fmaf(wr, t + c3, fmaf(w, drho, -p[i]));
         float nj = fmaf(wr, t - c3, fmaf(w, drho, -p[j]));
         if (has_force) {
-            ni += force[i];
-            nj += force[j];
+            ni += force[i] * U;
+            nj += force[j] * U;
         }
-        const typename ST::S ei = ST::enc(fmaf(omega, ni, p[i]));
-        const typename ST::S ej = ST::enc(fmaf(omega, nj, p[j]));
-        const float oi = ST::dec(ei);   // ST::rnd of the relaxed value
-        const float oj = ST::dec(ej);
+        const typename ST::S ei = ST::encw(fmaf(omega, ni, p[i]));
+        const typename ST::S ej = ST::encw(fmaf(omega, nj, p[j]));
+        const float oi = ST::decw(ei);   // ST::rnd of the relaxed value, in working units
+        const float oj = ST::decw(ej);
         out[i] = oi;
         out[j] = oj;
         enc[i] = ei;
         enc[j] = ej;
         r += (p[i] - oi) + (p[j] - oj);
     };
-    pair(1, 3, ux, 1.0f / 9.0f, wr1);
-    pair(2, 4, uy, 1.0f / 9.0f, wr1);
-    pair(5, 7, ux + uy, 1.0f / 36.0f, wr2);
-    pair(6, 8, uy - ux, 1.0f / 36.0f, wr2);
-    out[0] = has_force ? (p[0] + r) + force[0] : p[0] + r;
-    enc[0] = ST::enc(out[0]);
+    pair(1, 3, ux, (1.0f / 9.0f) * U, wr1);
+    pair(2, 4, uy, (1.0f / 9.0f) * U, wr1);
+    pair(5, 7, ux + uy, (1.0f / 36.0f) * U, wr2);
+    pair(6, 8, uy - ux, (1.0f / 36.0f) * U, wr2);
+    out[0] = has_force ? (p[0] + r) + force[0] * U : p[0] + r;
+    enc[0] = ST::encw(out[0]);
 }
 
 // Reciprocal for the deviation forms (hardware approximation, 1 ulp).
@@ -348,8 +364,8 @@ __device__ inline void ghost_stores(void* dst, OffT plane_b, const Nbr<OffT>& n,
             const OffT row = cy(i) == 1 ? n.rn : (cy(i) == -1 ? n.rs : n.rc);
             const OffT col = cx(i) == 1 ? n.cp : (cx(i) == -1 ? n.cm : n.cc);
             typename ST::S* q = (typename ST::S*)((char*)dst + ((OffT)opp(i) * plane_b + row + col));
-            if (SCL_GHOST == 2) ST::stc(q, v[i]);
-            else ST::st(q, v[i]);
+            if (SCL_GHOST == 2) ST::sce(q, ST::encw(v[i]));
+            else ST::se(q, ST::encw(v[i]));
         }
     }
 }
@@ -395,11 +411,12 @@ __device__ inline void finish1(const CompArgs<typename ST::T>& ca, void* dst, lo
             const float drho = rho_in;
             const float rho = ca.rho0 + drho;
             float jx, jy;
-            momentum(p, jx, jy);
+            momentum(p, jx, jy);   // working units
             const float inv = rcp(rho);
+            constexpr float iu = 1.0f / ST::kUnit;
             float out[Q];
             typename ST::S enc[Q];
-            relax_dev<ST>(p, rho, drho, jx * inv, jy * inv, ca.omega, ca.force, ca.has_force, out, enc);
+            relax_dev<ST>(p, rho, drho, (jx * inv) * iu, (jy * inv) * iu, ca.omega, ca.force, ca.has_force, out, enc);
 #pragma unroll
             for (int i = 0; i < Q; ++i) ST::se((typename ST::S*)((char*)dst + ((OffT)i * plane_b + home)), enc[i]);
             if constexpr (GW) ghost_stores<ST, OffT>(dst, plane_b, nbr, nb, out);
@@ -507,8 +524,9 @@ __device__ inline void collide2_core(const StepArgs<T>& a, const U& u, int s_idx
     momentum(p[0], jax, jay);
     momentum(p[1], jbx, jby);
     const float inv_den = rcp(fmaf(ra, A.omega, rb * B.omega));
-    const float ucx = fmaf(jax, A.omega, jbx * B.omega) * inv_den;
-    const float ucy = fmaf(jay, A.omega, jby * B.omega) * inv_den;
+    constexpr float iu = 1.0f / ST::kUnit;   // the momenta are in working units
+    const float ucx = (fmaf(jax, A.omega, jbx * B.omega) * inv_den) * iu;
+    const float ucy = (fmaf(jay, A.omega, jby * B.omega) * inv_den) * iu;
     // Equilibrium velocities; at dry sites (|rho| <= 1e-16) the held ones
     // replace them below.
     float uea[2], ueb[2];
@@ -849,7 +867,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<typename ST::T>& a, con
 #pragma unroll
             for (int c = 0; c < NC; ++c) {
 #pragma unroll
-                for (int i = 0; i < Q; ++i) p[c][i] = ld_at<ST, OffT>(b.src[c], off[i]);
+                for (int i = 0; i < Q; ++i) p[c][i] = ldw_at<ST, OffT>(b.src[c], off[i]);
             }
         } else {
 #pragma unroll
@@ -862,7 +880,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<typename ST::T>& a, con
     auto file_rho = [&](const T (&p)[NC][Q], int slot, int col_slot) {
         if constexpr (NC == 2) {
 #pragma unroll
-            for (int c = 0; c < NC; ++c) lds[slot][c][col_slot] = dens(p[c]);
+            for (int c = 0; c < NC; ++c) lds[slot][c][col_slot] = dens_w<ST>(p[c]);
         }
     };
 
@@ -971,7 +989,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<typename ST::T>& a, con
                 nbr.cp = (OffT)xp * ES;
             }
             if constexpr (NC == 1) {
-                finish1<ST, REPLAY, OffT, GW>(a.c[0], b.dst[0], a.plane, plane_b, s_idx, home, p0[0], dens(p0[0]),
+                finish1<ST, REPLAY, OffT, GW>(a.c[0], b.dst[0], a.plane, plane_b, s_idx, home, p0[0], dens_w<ST>(p0[0]),
                                               OPEN || (nb0 & 1u), OPEN ? 0x1FFu : nb0, nbr);
             } else {
                 const int sS = k & 3, sC = (k + 1) & 3, sN = (k + 2) & 3;

@@ -283,7 +283,7 @@ __device__ inline void stepk_tile(const XkArgs<K, typename ST::T>& A, const int 
         for (int c = 0; c < 2; ++c)
 #pragma unroll
             for (int i = 0; i < Q; ++i) {
-                const R v = ld_at<ST, OffT>(a.c[c].src, fluid ? off[i] : home);
+                const R v = ldw_at<ST, OffT>(a.c[c].src, fluid ? off[i] : home);   // working units
                 p[c][i] = fluid ? v : R(0);
             }
     };
@@ -375,11 +375,11 @@ __device__ inline void stepk_tile(const XkArgs<K, typename ST::T>& A, const int 
             const unsigned nbq = nbits(r_first - 1);
             gather1(r_first - 1, nbq, pZ);
 #pragma unroll
-            for (int c = 0; c < 2; ++c) L.d[0][r3(an0, r_first - 1)][c][t] = dens(pZ[c]);
+            for (int c = 0; c < 2; ++c) L.d[0][r3(an0, r_first - 1)][c][t] = dens_w<ST>(pZ[c]);
             nbw[2] = nbits(r_first);
             gather1(r_first, nbw[2], pX);
 #pragma unroll
-            for (int c = 0; c < 2; ++c) L.d[0][r3(an0, r_first)][c][t] = dens(pX[c]);
+            for (int c = 0; c < 2; ++c) L.d[0][r3(an0, r_first)][c][t] = dens_w<ST>(pX[c]);
             nbw[1] = nbits(r_first + 1);
             gather1(r_first + 1, nbw[1], pY);
             mask_words(r_first + 3);
@@ -402,7 +402,7 @@ __device__ inline void stepk_tile(const XkArgs<K, typename ST::T>& A, const int 
             t_r2 = t_r3;
             gather1(min(r + 2, load_last), nbw[0], pn);
 #pragma unroll
-            for (int c = 0; c < 2; ++c) L.d[0][r3(an, r + 1)][c][t] = dens(pb[c]);
+            for (int c = 0; c < 2; ++c) L.d[0][r3(an, r + 1)][c][t] = dens_w<ST>(pb[c]);
             // (A) stage s >= 2: gather row g from ring q (stage s - 1), collided
             // by stage s - 1 in iterations up to r - 1
 #pragma unroll
@@ -419,7 +419,7 @@ __device__ inline void stepk_tile(const XkArgs<K, typename ST::T>& A, const int 
                             for (int i = 1; i < Q; ++i) {
                                 const bool src_fluid = OPEN || ((nbg >> opp(i)) & 1u);
                                 const S v = src_fluid ? pr(an, q, g - cy(i), c, i, t - cx(i)) : pr(an, q, g, c, opp(i), t);
-                                sg[q][c][i] = ST::dec(v);
+                                sg[q][c][i] = ST::decw(v);
                             }
                         }
                     } else {
@@ -429,7 +429,7 @@ __device__ inline void stepk_tile(const XkArgs<K, typename ST::T>& A, const int 
                             for (int i = 0; i < Q; ++i) sg[q][c][i] = R(0);
                     }
 #pragma unroll
-                    for (int c = 0; c < 2; ++c) L.d[s - 1][r3(an, g)][c][t] = dens(sg[q][c]);
+                    for (int c = 0; c < 2; ++c) L.d[s - 1][r3(an, g)][c][t] = dens_w<ST>(sg[q][c]);
                 }
             }
             __syncthreads();
@@ -445,8 +445,8 @@ __device__ inline void stepk_tile(const XkArgs<K, typename ST::T>& A, const int 
                     pr(an, 0, r, 0, i, t) = ea[i];
                     pr(an, 0, r, 1, i, t) = eb[i];
                 }
-                rest[0][0][0] = ST::dec(ea[0]);
-                rest[0][1][0] = ST::dec(eb[0]);
+                rest[0][0][0] = ST::decw(ea[0]);
+                rest[0][1][0] = ST::decw(eb[0]);
             }
             // (B) stage s >= 2: collide row h = r - 3(s - 1) from the populations
             // gathered one iteration earlier
@@ -468,8 +468,8 @@ __device__ inline void stepk_tile(const XkArgs<K, typename ST::T>& A, const int 
                             pr(an, s - 1, h, 0, i, t) = ea[i];
                             pr(an, s - 1, h, 1, i, t) = eb[i];
                         }
-                        rest[s - 1][0][0] = ST::dec(ea[0]);
-                        rest[s - 1][1][0] = ST::dec(eb[0]);
+                        rest[s - 1][0][0] = ST::decw(ea[0]);
+                        rest[s - 1][1][0] = ST::decw(eb[0]);
                     }
                 } else if (h >= y0 && h < y1 && out_lane && (OPEN || (nbh & 1u))) {
                     R nr[2][Q], rc[2], oa[Q], ob[Q];
@@ -534,11 +534,11 @@ __device__ inline void stepk_tile(const XkArgs<K, typename ST::T>& A, const int 
             const unsigned nbq = nbits(r_first - 1);
             gather1(r_first - 1, nbq, pZ);
 #pragma unroll
-            for (int c = 0; c < 2; ++c) L.d[0][r3(an0, r_first - 1)][c][t] = dens(pZ[c]);
+            for (int c = 0; c < 2; ++c) L.d[0][r3(an0, r_first - 1)][c][t] = dens_w<ST>(pZ[c]);
             nbw[2] = nbits(r_first);
             gather1(r_first, nbw[2], pX);
 #pragma unroll
-            for (int c = 0; c < 2; ++c) L.d[0][r3(an0, r_first)][c][t] = dens(pX[c]);
+            for (int c = 0; c < 2; ++c) L.d[0][r3(an0, r_first)][c][t] = dens_w<ST>(pX[c]);
             nbw[1] = nbits(r_first + 1);
             gather1(r_first + 1, nbw[1], pY);
             mask_words(r_first + 3);
@@ -560,7 +560,7 @@ __device__ inline void stepk_tile(const XkArgs<K, typename ST::T>& A, const int 
             t_r2 = t_r3;
             gather1(min(r + 2, load_last), nbw[0], pn);
 #pragma unroll
-            for (int c = 0; c < 2; ++c) L.d[0][r3(an, r + 1)][c][t] = dens(pb[c]);
+            for (int c = 0; c < 2; ++c) L.d[0][r3(an, r + 1)][c][t] = dens_w<ST>(pb[c]);
             __syncthreads();
             // stage 1: collide row r into ring 0
             if (t >= 1 && t < tl - 1 && (OPEN || (nbw[2] & 1u))) {
@@ -574,8 +574,8 @@ __device__ inline void stepk_tile(const XkArgs<K, typename ST::T>& A, const int 
                         pr(an, 0, r, 0, i, t) = ea[i];
                         pr(an, 0, r, 1, i, t) = eb[i];
                     }
-                    rest_now[0][0] = ST::dec(ea[0]);
-                    rest_now[0][1] = ST::dec(eb[0]);
+                    rest_now[0][0] = ST::decw(ea[0]);
+                    rest_now[0][1] = ST::decw(eb[0]);
                 }
             }
             __syncthreads();
@@ -594,7 +594,7 @@ __device__ inline void stepk_tile(const XkArgs<K, typename ST::T>& A, const int 
                             for (int i = 1; i < Q; ++i) {
                                 const bool src_fluid = OPEN || ((nbg >> opp(i)) & 1u);
                                 const S v = src_fluid ? pr(an, q, g - cy(i), c, i, t - cx(i)) : pr(an, q, g, c, opp(i), t);
-                                sg[q][c][i] = ST::dec(v);
+                                sg[q][c][i] = ST::decw(v);
                             }
                         }
                     } else {
@@ -604,7 +604,7 @@ __device__ inline void stepk_tile(const XkArgs<K, typename ST::T>& A, const int 
                             for (int i = 0; i < Q; ++i) sg[q][c][i] = R(0);
                     }
 #pragma unroll
-                    for (int c = 0; c < 2; ++c) L.d[s - 1][r3(an, g)][c][t] = dens(sg[q][c]);
+                    for (int c = 0; c < 2; ++c) L.d[s - 1][r3(an, g)][c][t] = dens_w<ST>(sg[q][c]);
                 }
                 __syncthreads();
                 // collision of stage s at row h = r - 2(s - 1)
@@ -621,8 +621,8 @@ __device__ inline void stepk_tile(const XkArgs<K, typename ST::T>& A, const int 
                             pr(an, s - 1, h, 0, i, t) = ea[i];
                             pr(an, s - 1, h, 1, i, t) = eb[i];
                         }
-                        rest_now[s - 1][0] = ST::dec(ea[0]);
-                        rest_now[s - 1][1] = ST::dec(eb[0]);
+                        rest_now[s - 1][0] = ST::decw(ea[0]);
+                        rest_now[s - 1][1] = ST::decw(eb[0]);
                     }
                     __syncthreads();
                 } else if (h >= y0 && h < y1 && out_lane && (OPEN || (nbh & 1u))) {
