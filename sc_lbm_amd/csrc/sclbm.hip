@@ -468,10 +468,24 @@ bool x2_supported(const scl_geometry* g, int precision) {
            !needs_wide(g, esz[precision]);
 }
 
-// whether two-component steps of `precision` on g go two per launch
+int x2_bands(const scl_geometry* g, int precision, int waves);
+
+// whether two-component steps of `precision` on g go two per launch:
+// fusion >= 2, or what scl_autotune measured fastest; untuned, always with
+// FP32 storage, and with FP16 storage where the lattice gives the two-step
+// grid's two whole waves bands of at least 2 rows (solid-free; 10 with
+// solids) before the 8-row minimum.  Below that the FP16 single step is
+// faster: two-component MLUPS, one step against two per launch, 512^2 open
+// 24.6k / 21.2k, 1024^2 with 20% solid disks 25.1k / 21.6k, 1024 x 512
+// porous 25.9k / 18.5k; above it two per launch win: 1024^2 open 38.5k /
+// 52.8k, 2048^2 with disks 30.6k / 37.4k.  (FP32 storage: two per launch
+// won on every lattice measured, 512^2 open 20.7k / 22.1k.)
 bool x2_wanted(const scl_geometry* g, int precision) {
     if (g->fusion == 1 || !x2_supported(g, precision)) return false;
-    return g->fusion >= 2 || g->x2_tuned[precision] >= 0;
+    if (g->fusion >= 2 || g->x2_tuned[precision] > 0) return true;
+    if (g->x2_tuned[precision] < 0) return false;
+    if (precision != SCL_FP16_DEV) return true;
+    return (double)g->ny / x2_bands(g, precision, 2) >= (g->open ? 2.0 : 10.0);
 }
 
 // Workgroups of the two-step kernel resident per CU (registers and LDS), of
@@ -496,23 +510,38 @@ int x2_occupancy(int precision, bool open) {
     return o;
 }
 
-// Band height that makes the grid `waves` whole waves of resident workgroups
-// (a partial last wave leaves CUs idle at the end of every launch).
-int x2_wave_rows(const scl_geometry* g, int precision, int waves) {
+// Bands that make the two-step grid `waves` whole waves of resident
+// workgroups (a partial last wave leaves CUs idle at the end of every
+// launch), and their height (at least 8 rows).
+int x2_bands(const scl_geometry* g, int precision, int waves) {
     const int out = precision == SCL_FP64       ? (g->open ? xk_out<StoreF64, 2, x2d_threads<true>()>()
                                                            : xk_out<StoreF64, 2, x2d_threads<false>()>())
                     : precision == SCL_FP32_DEV ? x2_out<StoreF32>()
                                                 : x2_out<StoreF16>();
     const int strips = (g->nx + out - 1) / out;
     const int slots = x2_occupancy(precision, g->open) * std::max(g_lib.cus, 1);
-    const int bands = std::max(1, waves * slots / strips);
+    return std::max(1, waves * slots / strips);
+}
+
+int x2_wave_rows(const scl_geometry* g, int precision, int waves) {
+    const int bands = x2_bands(g, precision, waves);
     return std::max(8, std::min((g->ny + bands - 1) / bands, g->ny));
 }
 
+// Default rows per band: two whole waves (52 rows at 4096^2 with FP32
+// storage, 35 with FP16).  FP64 (one workgroup per CU with solids, two
+// without) takes one wave where two would give bands under 48 rows: two-step
+// MLUPS at one against two waves, 1024 x 512 porous 15.7k (11 rows) / 10.4k
+// (8), 2048^2 open 27.6k (49) / 26.4k (25), 2048^2 with disks 23.6k (74) /
+// 22.5k (37); at 4096^2 open two waves (94 rows) 26.2k against one (188)
+// 25.8k.
 int x2_rows_for(const scl_geometry* g, int precision) {
-    // default: two whole waves (52 rows at 4096^2 with FP32 storage, 35 with FP16)
-    const int r = g->x2_rows ? g->x2_rows
-                             : (g->x2_tuned[precision] > 0 ? g->x2_tuned[precision] : x2_wave_rows(g, precision, 2));
+    int r = g->x2_rows;
+    if (!r && g->x2_tuned[precision] > 0) r = g->x2_tuned[precision];
+    if (!r) {
+        r = x2_wave_rows(g, precision, 2);
+        if (precision == SCL_FP64 && r < 48) r = x2_wave_rows(g, precision, 1);
+    }
     return std::min(r, g->ny);
 }
 
@@ -574,12 +603,17 @@ int launch_x2(const scl_geometry* g, scl_field* const* fl, const double* const* 
 // Three fused steps (k_stepk with K = 3): P_{n+2} from P_{n-1} in one
 // launch.  Automatic choice (fusion 0): what scl_autotune measured fastest,
 // else three per launch with FP32 storage (72k against 61k MLUPS for two on
-// a 4096^2 lattice) and two with FP16 storage (74k against 82k).
+// a 4096^2 lattice) where two whole waves of the three-step grid give bands
+// of at least 16 rows (2048^2 open: 72.5k against 52.9k for two; 2048^2
+// with disks 47.9k / 43.0k), two below (1024^2 open 43.5k / 51.2k; 1024 x
+// 512 porous 20.5k / 24.2k), and two with FP16 storage (74k against 82k).
+int x3_wave_rows(const scl_geometry* g, int precision, int waves);
+
 bool x3_wanted(const scl_geometry* g, int precision) {
     if (!x2_supported(g, precision) || g->ny < xk_min_rows<3>() || g->fusion == 1 || g->fusion == 2) return false;
     if (g->fusion == 3) return true;
     if (g->x3_tuned[precision]) return g->x3_tuned[precision] > 0;
-    return precision == SCL_FP32_DEV;
+    return precision == SCL_FP32_DEV && x3_wave_rows(g, precision, 2) >= 16;
 }
 
 template <class ST, bool OPEN>
