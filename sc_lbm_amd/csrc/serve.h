@@ -199,9 +199,11 @@ __global__ __launch_bounds__(kX2Threads) void k_serve2x(const ServeHdr* __restri
 
 // Launch q of a round: steps 3q .. 3q+2 of every three-step job that has
 // them (the tile with solids, strips of x3_threads columns).  FORCE = false:
-// no three-step job of the round has a body force.
+// no three-step job of the round has a body force.  Two waves per SIMD are
+// asked for: left free, the compiler gave the FP16 variant with a force 256
+// vector and 2 accumulation registers, one wave per SIMD.
 template <class ST, bool FORCE>
-__global__ __launch_bounds__((x3_threads<ST, false>())) void k_serve3x(const ServeHdr* __restrict__ hdr, int q,
+__global__ __launch_bounds__((x3_threads<ST, false>())) __attribute__((amdgpu_waves_per_eu(2))) void k_serve3x(const ServeHdr* __restrict__ hdr, int q,
                                                                      unsigned* __restrict__ ctr) {
     constexpr int T3 = x3_threads<ST, false>();
     __shared__ XkLds<ST, false, 3, T3> lds;
