@@ -227,14 +227,18 @@ int scl_set_lookahead(scl_geometry* g, int rows_ahead);
  * K = 2 (every form) or 3 (deviation forms) steps per launch (the
  * intermediate populations stay on chip; results are bit for bit those of
  * single steps).  0 (the default): what scl_autotune measured fastest, else
- * three per launch with FP32 storage and two with FP16 storage or FP64; 1, 2
+ * by lattice size: FP32 storage three per launch where two whole waves of
+ * the three-step grid give bands of at least 16 rows, else two; FP16 two
+ * where the two-step grid's bands reach 2 rows without solids or 10 with
+ * them, else one; FP64 two; 1, 2
  * or 3: that many per launch wherever supported (a call's remaining steps go
  * two or one per launch; FP64 takes two while at least three remain, so the
  * call's last step is a single one).  FP64 fields hand over to single steps
  * where a density falls below 1e-8 (the zero-density rule's held velocities
  * need them; the launch that met it is repeated).  rows: rows per band of
  * the multi-step kernels (0 = autotuned, else the height that makes the grid
- * two whole waves of resident workgroups). */
+ * two whole waves of resident workgroups; FP64 one wave where two would give
+ * bands under 48 rows). */
 int scl_set_step_fusion(scl_geometry* g, int steps_per_launch, int rows);
 /* Density below which FP64 steps on g hand over from two per launch to one
  * (no reference counterpart; default and least value 1e-8, the zero-density
