@@ -516,6 +516,30 @@ def test_checkpoint_restore_continues_bit_for_bit(prec, name):
     g3.close()
 
 
+def test_automatic_step_fusion_follows_the_lattice_size():
+    """Untuned (fusion 0, no autotuning) the steps per launch follow the
+    lattice size on this 256-CU part: on the 1024 x 512 porous lattice FP32
+    storage takes two per launch (three-step bands would be under 16 rows),
+    FP16 one (two-step bands under 10 rows with solids) and FP64 two, at
+    bands of one whole wave (two would give 8 rows); on the 4096^2 open
+    lattice FP32 three, FP16 two and FP64 two at two whole waves.  Which one
+    runs never changes the bits (the fused-against-single tests)."""
+    small = nat.Geometry(wl.porous(nx=1024, ny=512, porosity=0.70, seed=2024)["mask"])
+    big = nat.Geometry(np.ones((4096, 4096), dtype=np.int32))
+    try:
+        assert small.step_fusion(nat.FP32_DEV)[0] == 2
+        assert small.step_fusion(nat.FP16_DEV)[0] == 1
+        k, rows = small.step_fusion(nat.FP64)
+        assert k == 2 and rows > 8
+        assert big.step_fusion(nat.FP32_DEV)[0] == 3
+        assert big.step_fusion(nat.FP16_DEV)[0] == 2
+        k, rows = big.step_fusion(nat.FP64)
+        assert k == 2 and rows >= 48
+    finally:
+        small.close()
+        big.close()
+
+
 @pytest.mark.parametrize("prec", [nat.FP64, nat.FP32_DEV, nat.FP16_DEV])
 @pytest.mark.parametrize("shape", [(300, 5), (300, 6), (300, 8), (256, 40), (257, 33)])
 def test_multi_step_kernels_on_edge_shapes(prec, shape):
