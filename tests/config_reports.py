@@ -1,9 +1,11 @@
 """Per-configuration reports on the GPU (writes a markdown summary).
 
-    python tests/config_reports.py --out gpurun_out/configs_report.md [--only c1,c2,c3,c5]
+    python tests/config_reports.py [--out FILE] [--only c1,c2,c3,c5]
+
+(default output: profiles/configs_report.md)
 
 For each standard configuration this runs the library on the GPU and states
-the quantities the configuration is judged on:
+the quantities that characterise the configuration:
   C1  Poiseuille 200x50, FP64, 10000 steps: full population field against
       the oracle (bit for bit), u_x(y) against the analytic profile;
   C2  static droplet 256^2, FP64 and FP32 deviations, 20000 steps: FP64
@@ -344,7 +346,7 @@ def c5(rep, n=4096, steps=10000, every=500):
 
 def main():
     p = argparse.ArgumentParser()
-    p.add_argument("--out", default=os.path.join(ROOT, "gpurun_out", "configs_report.md"))
+    p.add_argument("--out", default=os.path.join(ROOT, "profiles", "configs_report.md"))
     p.add_argument("--only", default="c1,c2,c3,c5")
     a = p.parse_args()
     nat.init(0)

@@ -2,7 +2,7 @@
 
 - FP32 deviations on the porous two-component case (1024x512, walls and
   disks, adhesion +-0.1, body force 1e-5 on A, 10^4 steps) stay within 1e-5
-  of the FP64 run in density, and within 1e-5 in
+  of the FP64 run in density (the accuracy bar of the deviation forms), and within 1e-5 in
   velocity;
 - scl_moments (FP64 device reduction) equals an independent sum of the
   downloaded populations: math.fsum (exact) at 1024^2, extended-precision
