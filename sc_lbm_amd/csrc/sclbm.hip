@@ -206,6 +206,16 @@ struct Replay {
     bool gather = false;
     scl_field* fl[2] = {nullptr, nullptr};
     StepArgs<double> args{};
+    // A call that ended in a two-step FP64 launch (P_{n+1} from P_{n-1}):
+    // the input of its last step, P_n, was never stored.  `pre` recomputes
+    // it from P_{n-1}, still intact in the launch's input buffer, into the
+    // buffer of the launch's output; the replay (args) then reads it, and
+    // `post` steps it again into the input buffer, which then holds P_{n+1}
+    // bit for bit as the launch wrote it (cur_after: that buffer).
+    bool two = false;
+    StepArgs<double> pre{}, post{};
+    int nw = 4, la = 1;
+    int cur_after[2] = {0, 0};
 };
 
 void geometry_release(scl_geometry* g);
@@ -706,6 +716,31 @@ bool x2d_ok(const scl_geometry* g, scl_field* const* fl, int nc) {
            !fl[0]->dilute && !fl[1]->dilute && x2d_wanted(g);
 }
 
+// The replay record of a call whose last launch is the two-step one about
+// to run on fl (their cur and nstep are still the launch's input's): the
+// single steps n and n + 1 with the shape launch_fused gives them.
+std::shared_ptr<Replay> x2d_replay(const scl_geometry* g, scl_field* const* fl, const double* const* forces) {
+    const Shape sh = resolve_shape(g, 2, SCL_FP64);
+    auto r = std::make_shared<Replay>();
+    r->nc = 2;
+    r->gather = true;
+    r->two = true;
+    r->nw = sh.nw;
+    r->la = sh.la;
+    r->pre = make_args<double>(g, sh.rows, sh.nw);
+    r->post = make_args<double>(g, sh.rows, sh.nw);
+    for (int c = 0; c < 2; ++c) {
+        r->fl[c] = fl[c];
+        r->cur_after[c] = fl[c]->cur;
+        fill_comp(r->pre.c[c], fl[c], fl[c]->cur, forces[c]);        // P_{n-1} -> P_n
+        fill_comp(r->post.c[c], fl[c], fl[c]->cur ^ 1, forces[c]);   // P_n -> P_{n+1}
+        r->post.c[c].step += 1;
+    }
+    r->args = make_args<double>(g, sh.rows, 4);
+    for (int c = 0; c < 2; ++c) r->args.c[c] = r->post.c[c];
+    return r;
+}
+
 int launch_x2d(const scl_geometry* g, scl_field* const* fl, const double* const* forces, unsigned launch) {
     const int T = g->open ? x2d_threads<true>() : x2d_threads<false>();
     const int OUT = g->open ? xk_out<StoreF64, 2, x2d_threads<true>()>() : xk_out<StoreF64, 2, x2d_threads<false>()>();
@@ -770,6 +805,17 @@ int run_replay(Replay& r) {
     const scl_geometry* g = r.fl[0]->g;
     const int nblk = nblocks(g, 4, r.args.rows);
     const bool wide = needs_wide(g, sizeof(double));
+    if (r.two) {   // two-component, gathering, not wide (two-step launches)
+        CHK((launch_nw<2, StoreF64, true>(r.nw, r.la, g->open, false, r.pre, nblocks(g, r.nw, r.pre.rows))));
+        HIPCHK(hipGetLastError());
+        replay_t<2, true>(r, nblk, false);
+        HIPCHK(hipGetLastError());
+        CHK((launch_nw<2, StoreF64, true>(r.nw, r.la, g->open, false, r.post, nblocks(g, r.nw, r.post.rows))));
+        HIPCHK(hipGetLastError());
+        for (int c = 0; c < 2; ++c)
+            if (r.fl[c]) r.fl[c]->cur = r.cur_after[c];
+        return SCL_OK;
+    }
     if (r.nc == 1) {
         if (r.gather) replay_t<1, true>(r, nblk, wide);
         else replay_t<1, false>(r, nblk, wide);
@@ -891,11 +937,14 @@ int fused_steps(const scl_geometry* g, scl_field** fl, const double* const* forc
     unsigned run = 0;
     int run_t = 0, run_cur[NC];
     unsigned run_nstep[NC];
-    for (int t = 0; t < nsteps;) {
+    std::shared_ptr<Replay> last2;   // the replay record if the call ends in a two-step launch
+    for (int t = 0; t < nsteps || run;) {
         if constexpr (NC == 2) {
-            // FP64: two steps per launch while at least three remain (the last
-            // step of a call records the replay of the per-operator getters)
-            if (nsteps - t >= 3 && x2d_ok(g, fl, NC)) {
+            // FP64: two steps per launch while at least two remain; a call
+            // that ends in one keeps a replay record that recomputes its
+            // last step's input (x2d_replay), one that ends in a single step
+            // that step's (launch_fused)
+            if (nsteps - t >= 2 && x2d_ok(g, fl, NC)) {
                 if (run == 0) {
                     run_t = t;
                     for (int c = 0; c < NC; ++c) {
@@ -903,6 +952,7 @@ int fused_steps(const scl_geometry* g, scl_field** fl, const double* const* forc
                         run_nstep[c] = fl[c]->nstep;
                     }
                 }
+                if (t + 2 == nsteps) last2 = x2d_replay(g, fl, forces);
                 CHK(launch_x2d(g, fl, forces, ++run));
                 for (int c = 0; c < NC; ++c) {
                     fl[c]->cur ^= 1;
@@ -922,7 +972,10 @@ int fused_steps(const scl_geometry* g, scl_field** fl, const double* const* forc
                         fl[c]->nstep = run_nstep[c] + 2u * (unsigned)j;
                         fl[c]->dilute = true;
                     }
+                } else if (last2 && t == nsteps) {
+                    for (int c = 0; c < NC; ++c) fl[c]->replay = last2;
                 }
+                last2.reset();
                 run = 0;
                 continue;
             }

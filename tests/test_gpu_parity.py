@@ -399,6 +399,47 @@ def test_fp64_two_steps_per_launch_equal_single_steps(name):
             assert np.array_equal(got[c].reshape(-1), s.get_f(c))
 
 
+@pytest.mark.parametrize("name", ["open", "porous"])
+def test_fp64_reads_after_a_call_ending_in_two_steps(name):
+    """An FP64 call that ends in a two-step launch never stores its last
+    step's input; a read of the reference's intermediate arrays (density,
+    velocities, equilibrium velocities, repulsion, feq) recomputes it from
+    the launch's input and replays the last step.  With reads after every
+    call (the first call ends in a single step, the next two in two-step
+    launches) the arrays and the populations equal those of single steps bit
+    for bit, and the last ones the oracle's."""
+    if name == "open":
+        case = wl.spinodal(256, G=1.2, seed=5)
+        case["force"] = [None, None]
+    else:
+        case = wl.porous(nx=300, ny=64, porosity=0.8, seed=3, rmin=4, rmax=7)
+    names = ("rho", "ux", "uy", "ueqx", "ueqy", "frx", "fry", "feq")
+    out = {}
+    for fusion in (1, 2):
+        g, fl = gh.make(case)
+        g.set_step_fusion(fusion, 0)
+        assert g.step_fusion(nat.FP64)[0] == fusion
+        reads = []
+        for n in (4, 6, 2):
+            nat.step2(g, fl[0], fl[1], case["force"][0], case["force"][1], n)
+            reads.append([[fl[c].download(gh.NAMES[nm]) for nm in names] + [fl[c].get_f()] for c in range(2)])
+        out[fusion] = reads
+        for f in fl:
+            f.close()
+        g.close()
+    for k in range(3):
+        for c in range(2):
+            for i, nm in enumerate(names + ("f",)):
+                assert np.array_equal(out[1][k][c][i], out[2][k][c][i]), (k, c, nm)
+    s = _oracle_run(case, 12)
+    fields = {"rho": po.RHO, "ux": po.UX, "uy": po.UY, "ueqx": po.UEQX, "ueqy": po.UEQY,
+              "frx": po.FRX, "fry": po.FRY, "feq": po.FEQ}
+    for c in range(2):
+        for i, nm in enumerate(names):
+            assert np.array_equal(out[2][2][c][i], s.get(c, fields[nm])), (nm, c)
+        assert np.array_equal(out[2][2][c][-1], s.get_f(c))
+
+
 def test_fp64_two_steps_hand_over_in_the_middle_of_a_call():
     """A density falling below the hand-over threshold after some two-step
     launches (a spinodal lattice separating: its least density passes 0.5
