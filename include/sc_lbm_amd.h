@@ -232,8 +232,9 @@ int scl_set_lookahead(scl_geometry* g, int rows_ahead);
  * where the two-step grid's bands reach 2 rows without solids or 10 with
  * them, else one; FP64 two; 1, 2
  * or 3: that many per launch wherever supported (a call's remaining steps go
- * two or one per launch; FP64 takes two while at least three remain, so the
- * call's last step is a single one).  FP64 fields hand over to single steps
+ * two or one per launch; FP64 takes two while at least two remain, and a
+ * read of the per-operator arrays after a call that ended in a two-step
+ * launch recomputes that call's last step).  FP64 fields hand over to single steps
  * where a density falls below 1e-8 (the zero-density rule's held velocities
  * need them; the launch that met it is repeated).  rows: rows per band of
  * the multi-step kernels (0 = autotuned, else the height that makes the grid
