@@ -780,15 +780,18 @@ __device__ inline void finish2(const StepArgs<typename ST::T>& a, void* const (&
 // the bounce-back selects and the adhesion sums fold away at compile time).
 // ---------------------------------------------------------------------------
 // Occupancy request: the solid-free FP32-arithmetic kernels fit in 168
-// registers, i.e. three waves per SIMD (four measured no faster, and with the
-// FP64 force sums they would spill).
+// registers, i.e. three waves per SIMD (four measured no faster with FP32
+// storage, and with the FP64 force sums they would spill); with FP16
+// storage in 128, four waves per SIMD (left at three, the kernel took 138
+// registers and ran 31% slower at 4096^2).
 template <class ST, bool OPEN, int LA>
 #ifndef SCL_GATHER_WAVES
 #define SCL_GATHER_WAVES 1
 #endif
 constexpr int step_waves_per_eu() {
-    return (OPEN && LA == 1 && sizeof(typename ST::T) == 4) ? 3
-                                                            : ((!OPEN && sizeof(typename ST::T) == 4) ? SCL_GATHER_WAVES : 1);
+    return (OPEN && LA == 1 && sizeof(typename ST::T) == 4)
+               ? (sizeof(typename ST::S) == 2 ? 4 : 3)
+               : ((!OPEN && sizeof(typename ST::T) == 4) ? SCL_GATHER_WAVES : 1);
 }
 
 // Source and destination population buffers of one launch, per component.
