@@ -303,15 +303,18 @@ def _dry_case(nx=320, ny=96):
 
 
 @pytest.mark.parametrize("prec", [nat.FP32_DEV, nat.FP16_DEV])
-@pytest.mark.parametrize("name", ["open", "porous", "dry"])
+@pytest.mark.parametrize("name", ["open", "open_force", "porous", "dry"])
 def test_two_steps_per_launch_equal_single_steps(prec, name):
     """The two- and three-step kernels (the intermediate steps stay on chip)
     give the populations of single-step launches bit for bit, on lattices
     whose width is not a multiple of the strip, with solids, adhesion, a
-    force and dry sites."""
+    force and dry sites, and solid-free with and without a force."""
     if name == "open":
         case = wl.spinodal(300, G=1.2, seed=4)
         case["force"] = [None, None]
+    elif name == "open_force":
+        case = wl.spinodal(300, G=1.2, seed=7)
+        case["force"] = [wl.body_force(1e-5), None]
     elif name == "porous":
         case = wl.porous(nx=700, ny=230, porosity=0.75, seed=6, rmin=5, rmax=12)
     else:
