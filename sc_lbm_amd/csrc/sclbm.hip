@@ -3,10 +3,11 @@
 // Owns the device state of geometries and fields, tracks the population
 // phase of each field (reference streamed state S, or post-collision P whose
 // streamed state is Gather(P)), dispatches the fused and per-operator
-// kernels (two-component deviation-form fields take two steps per launch
-// with k_step2x where the autotuner found that faster), and recomputes the
-// reference's intermediate arrays of the last fused step on demand
-// ("replay": the input buffer of that step is still intact, so rerunning it
+// kernels (two-component fields take several steps per launch: k_stepk,
+// k_step2x and k_step2d, as the autotuner measured fastest or by lattice
+// size), and recomputes the reference's intermediate arrays of the last
+// fused step on demand ("replay": the input buffer of that step, or of the
+// two-step launch that ended the call, is still intact, so rerunning from it
 // in replay mode writes exactly the arrays the reference would hold).
 #include <hip/hip_runtime.h>
 
