@@ -544,14 +544,17 @@ int x2_wave_rows(const scl_geometry* g, int precision, int waves) {
 // without) takes one wave where two would give bands under 48 rows: two-step
 // MLUPS at one against two waves, 1024 x 512 porous 15.7k (11 rows) / 10.4k
 // (8), 2048^2 open 27.6k (49) / 26.4k (25), 2048^2 with disks 23.6k (74) /
-// 22.5k (37); at 4096^2 open two waves (94 rows) 26.2k against one (188)
-// 25.8k.
+// 22.5k (37).  Above that, solid-free FP64 lattices take bands of 16 rows
+// (many waves of short bands): 4096^2 28.2k against 26.3k at two waves (94
+// rows) and 25.8k at one (188); 3072^2 28.0k (16) / 28.5k (70) / 24.9k
+// (140).  With solids two waves stay (4096^2 with disks: 23.2k at 137 rows
+// against 20.6k at 16).
 int x2_rows_for(const scl_geometry* g, int precision) {
     int r = g->x2_rows;
     if (!r && g->x2_tuned[precision] > 0) r = g->x2_tuned[precision];
     if (!r) {
         r = x2_wave_rows(g, precision, 2);
-        if (precision == SCL_FP64 && r < 48) r = x2_wave_rows(g, precision, 1);
+        if (precision == SCL_FP64) r = r < 48 ? x2_wave_rows(g, precision, 1) : (g->open ? 16 : r);
     }
     return std::min(r, g->ny);
 }
