@@ -239,7 +239,8 @@ int scl_set_lookahead(scl_geometry* g, int rows_ahead);
  * need them; the launch that met it is repeated).  rows: rows per band of
  * the multi-step kernels (0 = autotuned, else the height that makes the grid
  * two whole waves of resident workgroups; FP64 one wave where two would give
- * bands under 48 rows, and 16 rows above that on solid-free lattices). */
+ * bands under 48 rows, and 16 rows above that on solid-free lattices; three
+ * steps per launch with FP32 storage one wave on solid-free lattices). */
 int scl_set_step_fusion(scl_geometry* g, int steps_per_launch, int rows);
 /* Density below which FP64 steps on g hand over from two per launch to one
  * (no reference counterpart; default and least value 1e-8, the zero-density

@@ -661,10 +661,15 @@ int x3_wave_rows(const scl_geometry* g, int precision, int waves) {
     return g->open ? x3_wave_rows_t<StoreF16, true>(g, waves) : x3_wave_rows_t<StoreF16, false>(g, waves);
 }
 
-// rows per band: the caller's, else the autotuned, else two whole waves
+// rows per band: the caller's, else the autotuned, else two whole waves, or
+// one with FP32 storage on a solid-free lattice: three-step MLUPS at one
+// against two waves, 4096^2 94.4k (158 rows) / 90.6k (78), 2048^2 78.7k
+// (42) / 69.2k (21); with solids two stay (4096^2 with disks 57.2k at 90
+// rows against 36.3k at 178).
 int x3_rows_for(const scl_geometry* g, int precision) {
+    const int waves = precision == SCL_FP32_DEV && g->open ? 1 : 2;
     const int r = g->x2_rows ? g->x2_rows
-                             : (g->x3_tuned[precision] > 0 ? g->x3_tuned[precision] : x3_wave_rows(g, precision, 2));
+                             : (g->x3_tuned[precision] > 0 ? g->x3_tuned[precision] : x3_wave_rows(g, precision, waves));
     return std::min(r, g->ny);
 }
 
